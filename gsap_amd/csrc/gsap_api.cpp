@@ -1,0 +1,649 @@
+// gsap_api.cpp -- host side of the C ABI declared in include/gsap.h.
+//
+// Owns the device workspaces, sequences the stage kernels on the caller's
+// stream, validates arguments and turns device status bits into status codes.
+// The reference orchestration this replaces is Sap::collision
+// (physicboxSap2.16_Fermi/sapDetection/Sap.cpp:211-335: >= 8 blocking
+// device->host copies and per-call CUDPP plan allocations) and
+// ParticleRigidBody (ParticleRigidBody/ParticleRigidBody.cpp:15-191).  Here an
+// update is one stream-ordered chain of kernels with a single device->host
+// copy of the counts at the end (none in the async / batched-graph paths).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gsap.h"
+#include "gsap_internal.h"
+
+using namespace gsap;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+hipError_t grow(Buf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return hipSuccess;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return e;
+    }
+    b.bytes = want;
+    return hipSuccess;
+}
+
+void release(Buf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+uint32_t bits_for(uint64_t n) {
+    uint32_t b = 1;
+    while (b < 32 && (1ull << b) < n) ++b;
+    return b;
+}
+
+}  // namespace
+
+struct gsap_sap {
+    gsap_sap_desc desc{};
+    std::string err;
+    int items = 8;       // object sort: keys per thread (tile = 256 * items)
+    int pair_items = 16; // pair sort
+    Buf keys[2], vals[2], rec, partials, axis, ticket, hist_obj, hist_pair, lb_obj, lb_pair;
+    Buf ctrl, counts, ptmp, rowstart, has, block_sums, learned_axis;
+    Buf seg_start, seg_n, seg_tile0, tile_seg;
+    unsigned long long* h_counts = nullptr;  // pinned
+    uint32_t cap_n = 0, cap_tiles = 0, cap_seg = 0;
+    uint64_t cap_pair_tiles = 0;
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    int nev = 0;
+    float stage_ms[8] = {};
+    // batched graph cache
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    std::vector<uint32_t> g_offsets;
+    const void* g_in = nullptr;
+    void* g_pairs = nullptr;
+    uint64_t g_cap = 0;
+    unsigned long long* g_scene_out = nullptr;
+};
+
+static int32_t fail(gsap_sap* h, int32_t code, const std::string& msg) {
+    if (h) h->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(h, e_ == hipErrorOutOfMemory ? GSAP_ERR_OOM : GSAP_ERR_HIP,          \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+extern "C" const char* gsap_last_error(void) { return g_err.c_str(); }
+
+namespace gsap_err_bridge {
+void set(const std::string& m) { g_err = m; }
+}
+
+extern "C" const char* gsap_version(void) { return "gsap-mi355x 0.1 (gfx950)"; }
+
+extern "C" int32_t gsap_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
+    if (!desc || !out) return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: null argument");
+    *out = nullptr;
+    if (desc->input_kind != GSAP_INPUT_AABB && desc->input_kind != GSAP_INPUT_SPHERE)
+        return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad input_kind");
+    if (desc->axis_mode < GSAP_AXIS_PCA || desc->axis_mode > GSAP_AXIS_LEARNED)
+        return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad axis_mode");
+    if (desc->output_kind != GSAP_OUT_CANON && desc->output_kind != GSAP_OUT_SYM_CSR)
+        return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad output_kind");
+    if (desc->digit_bits != 0 && desc->digit_bits != 8)
+        return fail(nullptr, GSAP_ERR_UNSUPPORTED, "gsap_sap_create: digit_bits must be 0 or 8");
+    if (desc->sort_items != 0 && desc->sort_items != 4 && desc->sort_items != 8 &&
+        desc->sort_items != 12 && desc->sort_items != 16)
+        return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: sort_items in {4,8,12,16}");
+    if (desc->axis_mode == GSAP_AXIS_FIXED &&
+        !(std::isfinite(desc->fixed_axis[0]) && std::isfinite(desc->fixed_axis[1]) &&
+          std::isfinite(desc->fixed_axis[2])) )
+        return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: non-finite fixed axis");
+    if (gsap_device_count() <= 0) return fail(nullptr, GSAP_ERR_HIP, "gsap_sap_create: no HIP device");
+    gsap_sap* h = new gsap_sap();
+    h->desc = *desc;
+    if (desc->sort_items) h->items = desc->sort_items;
+    hipError_t e = grow(h->ctrl, sizeof(Ctrl));
+    if (e == hipSuccess) e = hipMemset(h->ctrl.p, 0, sizeof(Ctrl));
+    if (e == hipSuccess) e = grow(h->counts, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_counts, 8 * sizeof(unsigned long long), 0);
+    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        std::string msg = std::string("gsap_sap_create: ") + hipGetErrorString(e);
+        gsap_sap_destroy(h);
+        return fail(nullptr, GSAP_ERR_HIP, msg);
+    }
+    *out = h;
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
+    if (!h) return GSAP_OK;
+    (void)hipDeviceSynchronize();
+    Buf* bufs[] = {&h->keys[0], &h->keys[1], &h->vals[0], &h->vals[1], &h->rec, &h->partials,
+                   &h->axis, &h->ticket, &h->hist_obj, &h->hist_pair, &h->lb_obj, &h->lb_pair,
+                   &h->ctrl, &h->counts, &h->ptmp, &h->rowstart, &h->has, &h->block_sums,
+                   &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0, &h->tile_seg};
+    for (Buf* b : bufs) release(*b);
+    if (h->h_counts) (void)hipHostFree(h->h_counts);
+    for (int i = 0; i < 8; ++i)
+        if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+    delete h;
+    return GSAP_OK;
+}
+
+// Object workspace for n objects, nseg scenes, ntiles tiles.
+static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntiles) {
+    if (n > h->cap_n) {
+        uint32_t want = std::max(n, h->desc.max_n);
+        for (int i = 0; i < 2; ++i) {
+            HIPCHK(h, grow(h->keys[i], (size_t)want * 4));
+            HIPCHK(h, grow(h->vals[i], (size_t)want * 4));
+        }
+        HIPCHK(h, grow(h->rec, (size_t)want * 32));
+        HIPCHK(h, grow(h->rowstart, (size_t)want * 4));
+        HIPCHK(h, grow(h->has, (size_t)want * 4));
+        HIPCHK(h, hipMemset(h->has.p, 0, (size_t)want * 4));
+        HIPCHK(h, grow(h->block_sums, ((size_t)want / 2048 + 2) * 4));
+        h->cap_n = want;
+    }
+    if (ntiles > h->cap_tiles) {
+        HIPCHK(h, grow(h->partials, (size_t)ntiles * 10 * sizeof(double)));
+        HIPCHK(h, grow(h->lb_obj, (size_t)ntiles * 256 * 8));
+        HIPCHK(h, hipMemset(h->lb_obj.p, 0, (size_t)ntiles * 256 * 8));
+        HIPCHK(h, grow(h->tile_seg, (size_t)ntiles * 4));
+        h->cap_tiles = ntiles;
+    }
+    if (nseg > h->cap_seg) {
+        HIPCHK(h, grow(h->axis, (size_t)nseg * 16));
+        HIPCHK(h, grow(h->ticket, (size_t)nseg * 4));
+        HIPCHK(h, hipMemset(h->ticket.p, 0, (size_t)nseg * 4));
+        HIPCHK(h, grow(h->hist_obj, (size_t)nseg * 4 * 256 * 4));
+        HIPCHK(h, grow(h->seg_start, (size_t)nseg * 4));
+        HIPCHK(h, grow(h->seg_n, (size_t)nseg * 4));
+        HIPCHK(h, grow(h->seg_tile0, (size_t)(nseg + 1) * 4));
+        h->cap_seg = nseg;
+    }
+    if (!h->hist_pair.p) HIPCHK(h, grow(h->hist_pair, kMaxPasses * 256 * 4));
+    return GSAP_OK;
+}
+
+static int32_t ensure_pairs(gsap_sap* h, uint64_t cap) {
+    HIPCHK(h, grow(h->ptmp, (size_t)cap * 8));
+    const uint64_t tiles = (cap + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
+    if (tiles > h->cap_pair_tiles) {
+        HIPCHK(h, grow(h->lb_pair, (size_t)tiles * 256 * 8));
+        HIPCHK(h, hipMemset(h->lb_pair.p, 0, (size_t)tiles * 256 * 8));
+        h->cap_pair_tiles = tiles;
+    }
+    return GSAP_OK;
+}
+
+static void mark(gsap_sap* h, hipStream_t st) {
+    if (h->timing && h->nev < 8) (void)hipEventRecord(h->ev[h->nev++], st);
+}
+
+// Enqueue one complete update (all stages) on st.  seg == nullptr: one scene.
+static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint32_t n,
+                              const SegTable& seg, uint32_t ntiles, const uint32_t* group,
+                              void* pairs, uint64_t cap, int32_t* entries, bool count_only,
+                              unsigned long long* d_counts, unsigned long long* scene_out) {
+    const int kind = h->desc.input_kind;
+    const bool sym = h->desc.output_kind == GSAP_OUT_SYM_CSR;
+    const uint32_t tile = sort_tile(h->items);
+    const uint32_t bbits = bits_for(n);
+    const uint32_t npass_pair = (2 * bbits + 7) / 8;
+    Ctrl* ctrl = (Ctrl*)h->ctrl.p;
+    h->nev = 0;
+    mark(h, st);
+
+    ObjPipeArgs a{};
+    a.in = in;
+    a.kind = kind;
+    a.n = n;
+    a.seg = seg;
+    a.tile = tile;
+    a.ntiles = ntiles;
+    a.axis_mode = h->desc.axis_mode;
+    for (int k = 0; k < 3; ++k) a.fixed_axis[k] = h->desc.fixed_axis[k];
+    a.axis = (float*)h->axis.p;
+    a.partials = (double*)h->partials.p;
+    a.seg_ticket = (uint32_t*)h->ticket.p;
+    a.ctrl = ctrl;
+    a.hist_obj = (uint32_t*)h->hist_obj.p;
+    a.npass_obj = 4;
+    a.hist_pair = (uint32_t*)h->hist_pair.p;
+    a.npass_pair = count_only ? 0 : npass_pair;
+    a.keys = (uint32_t*)h->keys[0].p;
+    a.vals = (uint32_t*)h->vals[0].p;
+    a.keys2 = (uint32_t*)h->keys[1].p;
+    a.vals2 = (uint32_t*)h->vals[1].p;
+    a.rec = h->rec.p;
+    a.learned_axis = (h->desc.axis_mode == GSAP_AXIS_LEARNED) ? (const float*)h->learned_axis.p
+                                                              : nullptr;
+    HIPCHK(h, launch_moments(a, st));
+    HIPCHK(h, launch_project(a, st));
+    mark(h, st);
+
+    SortArgs s{};
+    s.n = n;
+    s.seg = seg;
+    s.ntiles = ntiles;
+    s.hist = a.hist_obj;
+    s.npass = 4;
+    s.lookback = (unsigned long long*)h->lb_obj.p;
+    s.ctrl = ctrl;
+    s.ctr_base = 0;
+    s.bits = 8;
+    s.items = h->items;
+    s.grid = 0;
+    int which = 0;
+    HIPCHK(h, sort_kv32(s, a.keys, a.vals, a.keys2, a.vals2, &which, st));
+    mark(h, st);
+    HIPCHK(h, launch_gather(a, which ? a.keys2 : a.keys, which ? a.vals2 : a.vals, st));
+    mark(h, st);
+
+    SweepArgs w{};
+    w.rec = h->rec.p;
+    w.n = n;
+    w.seg = seg;
+    w.tile = tile;
+    w.ntiles = ntiles;
+    w.group = (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP) ? group : nullptr;
+    w.ctrl = ctrl;
+    w.kind = kind;
+    w.sym = sym;
+    w.count_only = count_only;
+    w.cap = count_only ? 0 : cap;
+    uint2* outp = (uint2*)pairs;
+    uint2* tmpp = (uint2*)h->ptmp.p;
+    uint2* first = (npass_pair % 2 == 0) ? outp : tmpp;
+    uint2* second = (npass_pair % 2 == 0) ? tmpp : outp;
+    w.out = count_only ? nullptr : first;
+    HIPCHK(h, launch_sweep(w, st));
+    mark(h, st);
+    if (!count_only) {
+        HIPCHK(h, launch_pair_hist(first, &ctrl->num_pairs, cap, a.hist_pair, npass_pair, bbits, st));
+        SortArgs p{};
+        p.n_dev = &ctrl->num_pairs;
+        p.cap = cap;
+        p.hist = a.hist_pair;
+        p.npass = npass_pair;
+        p.lookback = (unsigned long long*)h->lb_pair.p;
+        p.ctrl = ctrl;
+        p.ctr_base = kMaxPasses;
+        p.bits = 8;
+        p.items = h->pair_items;
+        p.bbits = (int)bbits;
+        p.grid = 2048;
+        int pw = 0;
+        HIPCHK(h, sort_pairs(p, first, second, &pw, st));
+        if (sym)
+            HIPCHK(h, launch_entries(outp, &ctrl->num_pairs, cap, n, (uint32_t*)h->rowstart.p,
+                                     (uint32_t*)h->has.p, (uint32_t*)h->block_sums.p, entries,
+                                     ctrl, st));
+        if (scene_out)
+            HIPCHK(h, launch_scene_offsets(outp, ctrl, cap, seg.seg_start, seg.nseg, scene_out, st));
+    }
+    mark(h, st);
+    HIPCHK(h, launch_finish(ctrl, count_only ? ~0ull : cap, d_counts, sym, st));
+    mark(h, st);
+    return GSAP_OK;
+}
+
+static int32_t check_common(gsap_sap* h, const void* in, uint32_t n, const uint32_t* group) {
+    if (!h) return fail(nullptr, GSAP_ERR_INVALID_ARG, "null handle");
+    if (n > 0 && !in) return fail(h, GSAP_ERR_INVALID_ARG, "d_in is NULL");
+    if (n > 0x7FFFFFFFu) return fail(h, GSAP_ERR_INVALID_ARG, "n exceeds 2^31-1");
+    if ((h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP) && n > 0 && !group)
+        return fail(h, GSAP_ERR_INVALID_ARG, "GSAP_FLAG_SKIP_SAME_GROUP needs d_group");
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                                          const uint32_t* d_group, void* d_pairs,
+                                          uint64_t pair_capacity, int32_t* d_entries,
+                                          uint64_t* d_counts) {
+    int32_t rc = check_common(h, d_in, n, d_group);
+    if (rc) return rc;
+    if (!d_counts) return fail(h, GSAP_ERR_INVALID_ARG, "d_counts is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    const bool count_only = (pair_capacity == 0 || d_pairs == nullptr);
+    if (!count_only && h->desc.output_kind == GSAP_OUT_SYM_CSR && !d_entries)
+        return fail(h, GSAP_ERR_INVALID_ARG, "SYM_CSR output needs d_entries");
+    if (n == 0) {
+        HIPCHK(h, launch_zero_u64((unsigned long long*)d_counts, 5, st));
+        return GSAP_OK;
+    }
+    const uint32_t tile = sort_tile(h->items);
+    const uint32_t ntiles = (n + tile - 1) / tile;
+    rc = ensure_obj(h, n, 1, ntiles);
+    if (rc) return rc;
+    if (!count_only) {
+        rc = ensure_pairs(h, pair_capacity);
+        if (rc) return rc;
+    }
+    SegTable seg{};
+    seg.nseg = 1;
+    return enqueue_update(h, st, d_in, n, seg, ntiles, d_group, d_pairs, pair_capacity, d_entries,
+                          count_only, (unsigned long long*)d_counts, nullptr);
+}
+
+static int32_t read_counts(gsap_sap* h, hipStream_t st, uint64_t cap, bool count_only,
+                           uint64_t* h_num_pairs, uint32_t* h_num_entries, gsap_stats* stats,
+                           uint32_t n) {
+    HIPCHK(h, hipMemcpyAsync(h->h_counts, h->counts.p, 5 * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, st));
+    if (stats) HIPCHK(h, hipMemcpyAsync(stats->axis, h->axis.p, 3 * sizeof(float),
+                                        hipMemcpyDeviceToHost, st));
+    float src = 0.0f;
+    if (stats) HIPCHK(h, hipMemcpyAsync(&src, (float*)h->axis.p + 3, sizeof(float),
+                                        hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    const unsigned long long* c = h->h_counts;
+    if (h->timing && h->nev >= 2) {
+        for (int i = 0; i + 1 < h->nev; ++i)
+            (void)hipEventElapsedTime(&h->stage_ms[i], h->ev[i], h->ev[i + 1]);
+    }
+    if (h_num_pairs) *h_num_pairs = c[0];
+    if (h_num_entries) *h_num_entries = (uint32_t)c[1];
+    if (stats) {
+        const bool sym = h->desc.output_kind == GSAP_OUT_SYM_CSR;
+        stats->num_pairs = sym ? c[0] / 2 : c[0];
+        stats->sweep_tests = c[2];
+        stats->axis_source = (int32_t)src;
+        stats->n = n;
+        stats->ms_total = 0.0f;
+        if (h->timing && h->nev >= 2) (void)hipEventElapsedTime(&stats->ms_total, h->ev[0], h->ev[h->nev - 1]);
+    }
+    if (c[3] & kFlagTimeout) return fail(h, GSAP_ERR_HIP, "internal: sort look-back wait expired");
+    if (c[3] & kFlagNonFinite) return fail(h, GSAP_ERR_NONFINITE, "non-finite value in input boxes");
+    if (!count_only && c[0] > cap) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "pair capacity %llu too small: %llu required",
+                 (unsigned long long)cap, (unsigned long long)c[0]);
+        return fail(h, GSAP_ERR_CAPACITY, buf);
+    }
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_collide(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                                    const uint32_t* d_group, void* d_pairs, uint64_t pair_capacity,
+                                    int32_t* d_entries, uint64_t* h_num_pairs,
+                                    uint32_t* h_num_entries, gsap_stats* stats) {
+    int32_t rc = check_common(h, d_in, n, d_group);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const bool count_only = (pair_capacity == 0 || d_pairs == nullptr);
+    if (n == 0) {
+        if (h_num_pairs) *h_num_pairs = 0;
+        if (h_num_entries) *h_num_entries = 0;
+        if (stats) memset(stats, 0, sizeof(*stats));
+        return GSAP_OK;
+    }
+    rc = gsap_sap_collide_async(h, stream, d_in, n, d_group, d_pairs, pair_capacity, d_entries,
+                                (uint64_t*)h->counts.p);
+    if (rc) return rc;
+    return read_counts(h, st, pair_capacity, count_only, h_num_pairs, h_num_entries, stats, n);
+}
+
+// Scene tables for the batched path: tiles never straddle scenes.
+static int32_t upload_scenes(gsap_sap* h, hipStream_t st, const uint32_t* off, uint32_t ns,
+                             uint32_t* ntiles_out) {
+    const uint32_t tile = sort_tile(h->items);
+    std::vector<uint32_t> start(ns), cnt(ns), t0(ns + 1), tseg;
+    uint32_t nt = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        if (off[s + 1] < off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
+        start[s] = off[s];
+        cnt[s] = off[s + 1] - off[s];
+        t0[s] = nt;
+        const uint32_t k = (cnt[s] + tile - 1) / tile;
+        for (uint32_t i = 0; i < k; ++i) tseg.push_back(s);
+        nt += k;
+    }
+    t0[ns] = nt;
+    int32_t rc = ensure_obj(h, off[ns], ns, std::max(nt, 1u));
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpy(h->seg_start.p, start.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->seg_n.p, cnt.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->seg_tile0.p, t0.data(), (ns + 1) * 4, hipMemcpyHostToDevice));
+    if (nt) HIPCHK(h, hipMemcpy(h->tile_seg.p, tseg.data(), nt * 4, hipMemcpyHostToDevice));
+    (void)st;
+    *ntiles_out = nt;
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
+                                            const uint32_t* h_scene_offsets, uint32_t n_scenes,
+                                            void* d_pairs, uint64_t pair_capacity,
+                                            uint64_t* d_scene_pair_offsets, uint64_t* h_num_pairs,
+                                            gsap_stats* stats) {
+    if (!h) return fail(nullptr, GSAP_ERR_INVALID_ARG, "null handle");
+    if (!h_scene_offsets || n_scenes == 0) return fail(h, GSAP_ERR_INVALID_ARG, "no scenes");
+    if (h->desc.output_kind != GSAP_OUT_CANON)
+        return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode emits canonical pairs only");
+    if (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP)
+        return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no group filter");
+    const uint32_t n = h_scene_offsets[n_scenes];
+    if (h_scene_offsets[0] != 0) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets must start at 0");
+    int32_t rc = check_common(h, d_in, n, nullptr);
+    if (rc) return rc;
+    const bool count_only = (pair_capacity == 0 || d_pairs == nullptr);
+    if (!count_only && !d_scene_pair_offsets)
+        return fail(h, GSAP_ERR_INVALID_ARG, "d_scene_pair_offsets is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    if (n == 0) {
+        if (h_num_pairs) *h_num_pairs = 0;
+        if (d_scene_pair_offsets)
+            HIPCHK(h, launch_zero_u64((unsigned long long*)d_scene_pair_offsets, n_scenes + 1, st));
+        return GSAP_OK;
+    }
+    const bool same = h->gexec && h->g_in == d_in && h->g_pairs == d_pairs &&
+                      h->g_cap == pair_capacity &&
+                      h->g_scene_out == (unsigned long long*)d_scene_pair_offsets &&
+                      h->g_offsets.size() == n_scenes + 1 &&
+                      std::equal(h->g_offsets.begin(), h->g_offsets.end(), h_scene_offsets) &&
+                      !h->timing;
+    if (!same) {
+        uint32_t nt = 0;
+        HIPCHK(h, hipStreamSynchronize(st));
+        rc = upload_scenes(h, st, h_scene_offsets, n_scenes, &nt);
+        if (rc) return rc;
+        if (!count_only) {
+            rc = ensure_pairs(h, pair_capacity);
+            if (rc) return rc;
+        }
+        SegTable seg{};
+        seg.seg_start = (const uint32_t*)h->seg_start.p;
+        seg.seg_n = (const uint32_t*)h->seg_n.p;
+        seg.seg_tile0 = (const uint32_t*)h->seg_tile0.p;
+        seg.tile_seg = (const uint32_t*)h->tile_seg.p;
+        seg.nseg = n_scenes;
+        seg.ntiles = nt;
+        if (h->gexec) {
+            (void)hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+        }
+        if (h->timing || count_only) {
+            rc = enqueue_update(h, st, d_in, n, seg, nt, nullptr, d_pairs, pair_capacity, nullptr,
+                                count_only, (unsigned long long*)h->counts.p,
+                                count_only ? nullptr : (unsigned long long*)d_scene_pair_offsets);
+            if (rc) return rc;
+        } else {
+            if (!h->cap_stream) HIPCHK(h, hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+            HIPCHK(h, hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_update(h, h->cap_stream, d_in, n, seg, nt, nullptr, d_pairs, pair_capacity,
+                                nullptr, false, (unsigned long long*)h->counts.p,
+                                (unsigned long long*)d_scene_pair_offsets);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(h->cap_stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIPCHK(h, e);
+            e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(h, e);
+            h->g_in = d_in;
+            h->g_pairs = d_pairs;
+            h->g_cap = pair_capacity;
+            h->g_scene_out = (unsigned long long*)d_scene_pair_offsets;
+            h->g_offsets.assign(h_scene_offsets, h_scene_offsets + n_scenes + 1);
+            HIPCHK(h, hipGraphLaunch(h->gexec, st));
+        }
+    } else {
+        HIPCHK(h, hipGraphLaunch(h->gexec, st));
+    }
+    return read_counts(h, st, pair_capacity, count_only, h_num_pairs, nullptr, stats, n);
+}
+
+extern "C" int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable) {
+    if (!h) return fail(nullptr, GSAP_ERR_INVALID_ARG, "null handle");
+    h->timing = enable != 0;
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages,
+                                        int32_t* n_stages) {
+    if (!h || !h_ms || !n_stages) return fail(h, GSAP_ERR_INVALID_ARG, "null argument");
+    int k = h->nev > 1 ? h->nev - 1 : 0;
+    if (k > max_stages) k = max_stages;
+    for (int i = 0; i < k; ++i) h_ms[i] = h->stage_ms[i];
+    *n_stages = k;
+    return GSAP_OK;
+}
+
+// ---------------------------------------------------------------- radix sort
+struct gsap_radix_sort {
+    std::string err;
+    uint32_t max_n = 0;
+    int keys_only = 0;
+    Buf tk, tv, dv, hist, lb, ctrl;
+};
+
+static int32_t rs_fail(gsap_radix_sort* s, int32_t code, const std::string& m) {
+    if (s) s->err = m;
+    g_err = m;
+    return code;
+}
+
+extern "C" int32_t gsap_radix_sort_create(uint32_t max_elements, int32_t keys_only,
+                                          gsap_radix_sort** out) {
+    if (!out) return rs_fail(nullptr, GSAP_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    if (gsap_device_count() <= 0) return rs_fail(nullptr, GSAP_ERR_HIP, "no HIP device");
+    gsap_radix_sort* s = new gsap_radix_sort();
+    s->max_n = max_elements;
+    s->keys_only = keys_only;
+    const uint32_t tiles = (max_elements + sort_tile(8) - 1) / sort_tile(8) + 1;
+    hipError_t e = grow(s->tk, (size_t)max_elements * 4);
+    if (e == hipSuccess) e = grow(s->tv, (size_t)max_elements * 4);
+    if (e == hipSuccess) e = grow(s->dv, (size_t)max_elements * 4);
+    if (e == hipSuccess) e = grow(s->hist, kMaxPasses * 256 * 4);
+    if (e == hipSuccess) e = grow(s->lb, (size_t)tiles * 256 * 8);
+    if (e == hipSuccess) e = hipMemset(s->lb.p, 0, (size_t)tiles * 256 * 8);
+    if (e == hipSuccess) e = grow(s->ctrl, sizeof(Ctrl));
+    if (e == hipSuccess) e = hipMemset(s->ctrl.p, 0, sizeof(Ctrl));
+    if (e != hipSuccess) {
+        std::string m = std::string("gsap_radix_sort_create: ") + hipGetErrorString(e);
+        gsap_radix_sort_destroy(s);
+        return rs_fail(nullptr, e == hipErrorOutOfMemory ? GSAP_ERR_OOM : GSAP_ERR_HIP, m);
+    }
+    *out = s;
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_radix_sort_destroy(gsap_radix_sort* s) {
+    if (!s) return GSAP_OK;
+    (void)hipDeviceSynchronize();
+    release(s->tk); release(s->tv); release(s->dv); release(s->hist); release(s->lb); release(s->ctrl);
+    delete s;
+    return GSAP_OK;
+}
+
+static int32_t rs_sort(gsap_radix_sort* s, hipStream_t st, uint32_t* keys, uint32_t* vals, uint32_t n,
+                       uint32_t key_bits, bool is_float) {
+    if (!s) return rs_fail(nullptr, GSAP_ERR_INVALID_ARG, "null sorter");
+    if (n > s->max_n) return rs_fail(s, GSAP_ERR_CAPACITY, "n exceeds maxElements");
+    if (n == 0) return GSAP_OK;
+    if (!keys || (!vals && !s->keys_only)) return rs_fail(s, GSAP_ERR_INVALID_ARG, "null keys/values");
+    if (key_bits == 0 || key_bits > 32) return rs_fail(s, GSAP_ERR_INVALID_ARG, "key_bits in 1..32");
+    uint32_t* v = vals ? vals : (uint32_t*)s->dv.p;
+    const uint32_t npass = (key_bits + 7) / 8;
+    hipError_t e = hipSuccess;
+#define RSCHK(x)                                                                 \
+    do {                                                                         \
+        e = (x);                                                                 \
+        if (e != hipSuccess) return rs_fail(s, GSAP_ERR_HIP, hipGetErrorString(e)); \
+    } while (0)
+    if (is_float) RSCHK(launch_flip(keys, n, 0, st));
+    RSCHK(launch_sort_prep((Ctrl*)s->ctrl.p, (uint32_t*)s->hist.p, npass * 256, 0, st));
+    RSCHK(launch_key_hist(keys, n, (uint32_t*)s->hist.p, (int)npass, 8, 0, st));
+    SortArgs a{};
+    a.n = n;
+    a.hist = (uint32_t*)s->hist.p;
+    a.npass = npass;
+    a.lookback = (unsigned long long*)s->lb.p;
+    a.ctrl = (Ctrl*)s->ctrl.p;
+    a.bits = 8;
+    a.items = 8;
+    int which = 0;
+    RSCHK(sort_kv32(a, keys, v, (uint32_t*)s->tk.p, (uint32_t*)s->tv.p, &which, st));
+    if (which) {
+        RSCHK(hipMemcpyAsync(keys, s->tk.p, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        if (vals) RSCHK(hipMemcpyAsync(vals, s->tv.p, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    if (is_float) RSCHK(launch_flip(keys, n, 1, st));
+#undef RSCHK
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_radix_sort_u32(gsap_radix_sort* s, void* stream, uint32_t* d_keys,
+                                       uint32_t* d_values, uint32_t n, uint32_t key_bits) {
+    return rs_sort(s, (hipStream_t)stream, d_keys, d_values, n, key_bits, false);
+}
+
+extern "C" int32_t gsap_radix_sort_f32(gsap_radix_sort* s, void* stream, float* d_keys,
+                                       uint32_t* d_values, uint32_t n, uint32_t key_bits,
+                                       int32_t negative_keys) {
+    (void)key_bits;
+    (void)negative_keys;  // the order-preserving map handles either sign
+    return rs_sort(s, (hipStream_t)stream, (uint32_t*)d_keys, d_values, n, 32, true);
+}

@@ -1,0 +1,173 @@
+// gsap_internal.h -- device-side data layout and host launch wrappers shared by
+// the HIP translation units of libgsap.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsap {
+
+constexpr int kWG = 256;          // threads per workgroup for the object stages
+constexpr int kWaves = kWG / 64;  // wave64
+constexpr int kSweepTile = 256;   // sorted objects per sweep workgroup
+constexpr int kMaxPasses = 8;
+
+// Scene (segment) table for the batched path.  A "tile" is a run of at most
+// `tile` consecutive objects of one scene; every object stage maps one tile
+// (or a 256-object sub-tile) to a workgroup.  tile_seg == nullptr means a
+// single scene [0, n).
+struct SegTable {
+    const uint32_t* seg_start;  // [nseg] first object
+    const uint32_t* seg_n;      // [nseg] objects
+    const uint32_t* seg_tile0;  // [nseg+1] first tile of each scene
+    const uint32_t* tile_seg;   // [ntiles] scene of each tile
+    uint32_t nseg;
+    uint32_t ntiles;
+};
+
+// Sorted-object record consumed by the sweep (32 bytes, one scalar load).
+struct alignas(16) RecAabb {
+    float lo[3];
+    float hi[3];
+    uint32_t orig;  // index into the caller's input (global in batched mode)
+    uint32_t end;   // first sorted position whose key exceeds this object's upper key
+};
+struct alignas(16) RecSphere {
+    float c[4];     // x y z r
+    uint32_t orig;
+    uint32_t end;
+    uint32_t pad[2];
+};
+
+// Device control block of one SaP handle.  Zeroed once at allocation; the
+// first kernel of every update resets what the update accumulates.
+struct Ctrl {
+    unsigned long long num_pairs;    // pairs (or directed pairs) emitted by the sweep
+    unsigned long long tests;        // sweep-test count
+    unsigned long long num_entries;  // SYM_CSR row starts
+    unsigned long long exec_tests;   // candidate tests executed (wave granularity)
+    uint32_t flags;                  // 1 = capacity overflow, 2 = non-finite input
+    uint32_t epoch;                  // bumped per update; lookback tags
+    uint32_t tile_ctr[2 * kMaxPasses];
+    uint32_t ticket;                 // last-block ticket (single scene)
+    uint32_t pad[3];
+};
+
+enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u };
+
+// ---- launch wrappers (k_sap.hip) ----
+struct ObjPipeArgs {
+    const void* in;         // caller input
+    int kind;               // 0 aabb, 1 sphere
+    uint32_t n;             // total objects
+    SegTable seg;           // scene table (single scene when tile_seg == nullptr)
+    uint32_t tile;          // objects per tile (= sort tile)
+    uint32_t ntiles;
+    int axis_mode;          // GSAP_AXIS_*
+    float fixed_axis[3];
+    float* axis;            // [nseg][4] device: axis used (w = source)
+    double* partials;       // [ntiles][10] moment partials
+    uint32_t* seg_ticket;   // [nseg] last-tile tickets (self-resetting)
+    Ctrl* ctrl;
+    uint32_t* hist_obj;     // [nseg][npass][256]
+    uint32_t npass_obj;
+    uint32_t* hist_pair;    // [npass_pair][256]
+    uint32_t npass_pair;
+    uint32_t* keys;         // [n] keys (ping)
+    uint32_t* vals;         // [n] orig index (ping)
+    uint32_t* keys2;
+    uint32_t* vals2;
+    void* rec;              // [n] sorted records
+    const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
+};
+
+hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st);
+hipError_t launch_project(const ObjPipeArgs& a, hipStream_t st);
+hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* sorted_keys,
+                         const uint32_t* perm, hipStream_t st);
+
+struct SweepArgs {
+    const void* rec;
+    uint32_t n;
+    SegTable seg;
+    uint32_t tile;
+    uint32_t ntiles;
+    const uint32_t* group;   // per original index, or null
+    uint2* out;
+    unsigned long long cap;
+    Ctrl* ctrl;
+    int kind;
+    int sym;                 // emit both orientations
+    int count_only;
+};
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
+
+hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
+                            unsigned long long cap, uint32_t* hist, int npass, int bbits,
+                            hipStream_t st);
+hipError_t launch_entries(const uint2* sorted_pairs, const unsigned long long* n_dev,
+                          unsigned long long cap, uint32_t nobj, uint32_t* rowstart,
+                          uint32_t* has, uint32_t* block_sums, int32_t* entries, Ctrl* ctrl,
+                          hipStream_t st);
+hipError_t launch_scene_offsets(const uint2* P, Ctrl* ctrl, unsigned long long cap,
+                                const uint32_t* seg_start, uint32_t nseg, unsigned long long* out,
+                                hipStream_t st);
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
+hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts,
+                         int sym, hipStream_t st);
+
+// ---- radix sort (k_sort.hip) ----
+struct SortArgs {
+    uint32_t n;                         // host element count (ignored if n_dev)
+    const unsigned long long* n_dev;    // device element count (pairs) or null
+    unsigned long long cap;             // clamp for n_dev
+    SegTable seg;                       // segmented (tile table) or single
+    uint32_t ntiles;                    // tiles of the segmented table (if seg.tile_seg)
+    const uint32_t* hist;               // [nseg][npass][256]
+    uint32_t npass;
+    unsigned long long* lookback;       // [max_tiles][256]
+    Ctrl* ctrl;
+    int ctr_base;                       // 0 object sort, kMaxPasses pair sort
+    int bits;                           // digit width
+    int items;                          // keys per thread
+    int bbits;                          // pairs: bits of the second index
+    uint32_t grid;                      // persistent grid size
+};
+// u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
+// the buffer index (0/1) that holds the result.
+hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                     int* which, hipStream_t st);
+hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, int* which, hipStream_t st);
+// Histogram of u32 keys for a plain (non-fused) sort, all passes.
+hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
+                           int flip_float, hipStream_t st);
+hipError_t launch_flip(uint32_t* keys, uint32_t n, int unflip, hipStream_t st);
+hipError_t launch_sort_prep(Ctrl* ctrl, uint32_t* hist, uint32_t nhist, int ctr_base,
+                            hipStream_t st);
+uint32_t sort_tile(int items);
+
+// ---- rigid bodies (k_prb.hip) ----
+struct PrbDev {
+    float4* pos; float4* quat; float4* linvel; float4* angvel;
+    int32_t* shape; int32_t* pstart; int32_t* npart; float* inv_mass;
+    float4* inv_inertia;    // 3 float4 per body
+    float4* ref_pos; float* ref_mass; float4* shape_inv_inertia;
+    float4* ppos; float4* pvel; float4* pforce; float* pmass; int32_t* pbody; int32_t* pip;
+    int32_t nb, np, max_per_shape;
+};
+struct PrbPhys {
+    float gravity[3];
+    float sp, dp, k_abs, c_abs, floor_scale;
+    float wall_lo[2], wall_hi[2];
+    float max_lin_vel;
+    uint32_t flags;
+};
+hipError_t prb_integrate(const PrbDev& d, const PrbPhys& p, float dt, hipStream_t st);
+hipError_t prb_update_particles(const PrbDev& d, hipStream_t st);
+hipError_t prb_prepare(const PrbDev& d, hipStream_t st);
+hipError_t prb_forces(const PrbDev& d, const PrbPhys& p, const int2* sym, const int32_t* entries,
+                      const unsigned long long* counts, unsigned long long max_pairs, float dt,
+                      hipStream_t st);
+hipError_t prb_update_velocity(const PrbDev& d, const PrbPhys& p, float dt, hipStream_t st);
+hipError_t prb_particle_aabbs(const PrbDev& d, float* out, hipStream_t st);
+
+}  // namespace gsap

@@ -1,0 +1,299 @@
+// k_sort.hip -- stable LSD radix sort for gfx950 ("onesweep" organisation).
+//
+// Replaces nvRadixSort::RadixSort (physicboxSap2.16_Fermi/sapDetection/
+// radixsort.h:110-278, radixsort.cu:1116-1498).  The reference runs, per
+// 4-bit digit, a block-local split sort, an offset pass, a CUDPP scan and a
+// scatter (4 launches + scan per digit).  Here one histogram of every digit
+// position is taken up front (fused into the projection kernel for the object
+// sort), then each digit is ONE kernel:
+//   tile load (striped, coalesced) -> wave64 ballot match ranking (stable)
+//   -> per-tile digit counts published to a decoupled look-back array ->
+//   look-back for the exclusive prefix -> keys staged in LDS in digit order
+//   -> coalesced scatter.
+// Tiles are claimed through an atomic counter so a workgroup only ever waits
+// on tiles that were claimed before it (no deadlock with a persistent grid).
+// Segments (batched scenes) restart the look-back chain at their first tile.
+#include "gsap_device.h"
+
+namespace gsap {
+
+struct KU32 {
+    using T = uint32_t;
+    static constexpr bool kHasVals = true;
+    __device__ static __forceinline__ uint32_t digit(T k, int shift, int, uint32_t mask) {
+        return (k >> shift) & mask;
+    }
+};
+
+// Packed pair {a, b} (a in the low word) ordered by the logical key (a << bbits) | b.
+struct KPair {
+    using T = uint2;
+    static constexpr bool kHasVals = false;
+    __device__ static __forceinline__ uint32_t digit(T k, int shift, int bbits, uint32_t mask) {
+        unsigned long long L = ((unsigned long long)k.x << bbits) | (unsigned long long)k.y;
+        return (uint32_t)(L >> shift) & mask;
+    }
+};
+
+__device__ __forceinline__ unsigned long long lb_word(uint32_t tag, bool inc, uint32_t v) {
+    return ((unsigned long long)tag << 33) | (inc ? (1ull << 32) : 0ull) | (unsigned long long)v;
+}
+
+template <class KT, int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restrict__ kin,
+                                                  typename KT::T* __restrict__ kout,
+                                                  const uint32_t* __restrict__ vin,
+                                                  uint32_t* __restrict__ vout, SortArgs a,
+                                                  int pass) {
+    constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    using K = typename KT::T;
+    __shared__ uint32_t s_wh[kWaves][RADIX];
+    __shared__ uint32_t s_tstart[RADIX];
+    __shared__ uint32_t s_gbase[RADIX];
+    __shared__ uint32_t s_tmp[2 * kWaves];
+    __shared__ K s_k[TILE];
+    __shared__ uint32_t s_v[KT::kHasVals ? TILE : 1];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const int shift = pass * BITS;
+    uint32_t n = a.n;
+    if (a.n_dev) {
+        unsigned long long nd = *a.n_dev;
+        n = (uint32_t)(nd > a.cap ? a.cap : nd);
+    }
+    const uint32_t ntiles = a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE;
+    const uint32_t ctr_idx = (uint32_t)(a.ctr_base + pass);
+    const uint32_t tag = ((a.ctrl->epoch << 4) | ctr_idx) & 0x7FFFFFFFu;
+    const unsigned long long ltm = lanemask_lt();
+    const unsigned long long gtm = ~ltm & ~(1ull << lane);
+
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_ctr[ctr_idx], 1u);
+        for (uint32_t i = tid; i < kWaves * RADIX; i += kWG) (&s_wh[0][0])[i] = 0u;
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= ntiles) break;
+        const TileInfo ti = tile_info(a.seg, n, tile, TILE);
+        const bool first = a.seg.tile_seg ? (tile == a.seg.seg_tile0[ti.seg]) : (tile == 0u);
+
+        K key[ITEMS];
+        uint32_t val[ITEMS], rank[ITEMS], dig[ITEMS];
+        const uint32_t base = ti.lo + wave * 64u * ITEMS + lane;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t idx = base + 64u * k;
+            if (idx < ti.hi) {
+                key[k] = kin[idx];
+                if (KT::kHasVals) val[k] = vin[idx];
+            }
+        }
+        // Stable ranking: items in tile order (wave, item, lane); peers with
+        // the same digit found by BITS ballots.
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const bool valid = (base + 64u * k) < ti.hi;
+            const uint32_t d = valid ? KT::digit(key[k], shift, a.bbits, MASK) : 0u;
+            dig[k] = d;
+            unsigned long long m = ballot(valid);
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long bb = ballot(bit);
+                m &= bit ? bb : ~bb;
+            }
+            const uint32_t before = s_wh[wave][d];
+            rank[k] = before + mbcnt(m);
+            if (valid && (m & gtm) == 0ull) s_wh[wave][d] = before + (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+
+        uint32_t tcount = 0, excl = 0;
+        if (tid < RADIX) {
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = s_wh[w][tid];
+                s_wh[w][tid] = run;
+                run += c;
+            }
+            tcount = run;
+            st_rel(a.lookback + (size_t)tile * RADIX + tid, lb_word(tag, first, tcount));
+            if (!first) {
+                uint32_t p = tile - 1u, spins = 0;
+                for (;;) {
+                    const unsigned long long v = ld_acq(a.lookback + (size_t)p * RADIX + tid);
+                    if ((uint32_t)(v >> 33) != tag) {
+                        // Bounded wait: a predecessor publishes within microseconds;
+                        // if it never does, flag the update and let the kernel end.
+                        if (++spins > (1u << 22)) {
+                            atomicOr(&a.ctrl->flags, kFlagTimeout);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    excl += (uint32_t)v;
+                    if (v & (1ull << 32)) break;
+                    --p;
+                }
+                st_rel(a.lookback + (size_t)tile * RADIX + tid, lb_word(tag, true, excl + tcount));
+            }
+        }
+        const uint32_t hv =
+            tid < RADIX ? a.hist[((size_t)ti.seg * a.npass + (uint32_t)pass) * RADIX + tid] : 0u;
+        uint32_t tot;
+        const uint32_t gex = block_excl_scan(hv, s_tmp, &tot);
+        const uint32_t tex = block_excl_scan(tcount, s_tmp + kWaves, &tot);
+        if (tid < RADIX) {
+            s_gbase[tid] = ti.seg_start + gex + excl;
+            s_tstart[tid] = tex;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            if ((base + 64u * k) < ti.hi) {
+                const uint32_t d = dig[k];
+                const uint32_t lp = s_tstart[d] + s_wh[wave][d] + rank[k];
+                s_k[lp] = key[k];
+                if (KT::kHasVals) s_v[lp] = val[k];
+            }
+        }
+        __syncthreads();
+        const uint32_t tn = ti.hi - ti.lo;
+        for (uint32_t i = tid; i < tn; i += kWG) {
+            const K kk = s_k[i];
+            const uint32_t d = KT::digit(kk, shift, a.bbits, MASK);
+            const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
+            kout[g] = kk;
+            if (KT::kHasVals) vout[g] = s_v[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Histogram of every digit position (plain sorts and the pair sort).
+template <class KT, int BITS>
+__global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__ keys, uint32_t n_host,
+                                              const unsigned long long* __restrict__ n_dev,
+                                              unsigned long long cap, uint32_t* __restrict__ hist,
+                                              int npass, int bbits, int flip_float) {
+    constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u;
+    __shared__ uint32_t s_h[kMaxPasses * RADIX];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)npass * RADIX; i += kWG) s_h[i] = 0u;
+    __syncthreads();
+    uint32_t n = n_host;
+    if (n_dev) {
+        unsigned long long nd = *n_dev;
+        n = (uint32_t)(nd > cap ? cap : nd);
+    }
+    for (uint32_t i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
+        typename KT::T k = keys[i];
+        if constexpr (KT::kHasVals) {
+            if (flip_float) k = flip_f32(__uint_as_float(k));
+        }
+        for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * RADIX + KT::digit(k, p * BITS, bbits, MASK)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < (uint32_t)npass * RADIX; i += kWG)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+__global__ void k_flip(uint32_t* keys, uint32_t n, int unflip) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint32_t k = keys[i];
+        keys[i] = unflip ? __float_as_uint(unflip_f32(k)) : flip_f32(__uint_as_float(k));
+    }
+}
+
+__global__ void k_sort_prep(Ctrl* ctrl, uint32_t* hist, uint32_t nhist, int ctr_base) {
+    for (uint32_t i = threadIdx.x; i < nhist; i += blockDim.x) hist[i] = 0u;
+    if (threadIdx.x < kMaxPasses) ctrl->tile_ctr[ctr_base + threadIdx.x] = 0u;
+    if (threadIdx.x == 0) ctrl->epoch += 1u;
+}
+
+uint32_t sort_tile(int items) { return (uint32_t)(kWG * items); }
+
+template <class KT, int BITS, int ITEMS>
+static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0,
+                             typename KT::T* k1, uint32_t* v1, int* which, hipStream_t st) {
+    const uint32_t TILE = kWG * ITEMS;
+    uint32_t grid = a.grid;
+    if (!a.n_dev) {
+        uint32_t nt = a.seg.tile_seg ? a.ntiles : (a.n + TILE - 1) / TILE;
+        if (nt == 0) { *which = 0; return hipSuccess; }
+        if (grid == 0 || grid > nt) grid = nt;
+    }
+    if (grid == 0) grid = 1;
+    int cur = 0;
+    for (uint32_t p = 0; p < a.npass; ++p) {
+        typename KT::T* ki = cur ? k1 : k0;
+        typename KT::T* ko = cur ? k0 : k1;
+        uint32_t* vi = cur ? v1 : v0;
+        uint32_t* vo = cur ? v0 : v1;
+        hipLaunchKernelGGL((k_onesweep<KT, BITS, ITEMS>), dim3(grid), dim3(kWG), 0, st, ki, ko, vi, vo,
+                           a, (int)p);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        cur ^= 1;
+    }
+    *which = cur;
+    return hipSuccess;
+}
+
+hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                     int* which, hipStream_t st) {
+    if (a.bits != 8) return hipErrorInvalidValue;
+    switch (a.items) {
+        case 4: return run_passes<KU32, 8, 4>(a, k0, v0, k1, v1, which, st);
+        case 8: return run_passes<KU32, 8, 8>(a, k0, v0, k1, v1, which, st);
+        case 12: return run_passes<KU32, 8, 12>(a, k0, v0, k1, v1, which, st);
+        case 16: return run_passes<KU32, 8, 16>(a, k0, v0, k1, v1, which, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, int* which, hipStream_t st) {
+    if (a.bits != 8) return hipErrorInvalidValue;
+    switch (a.items) {
+        case 8: return run_passes<KPair, 8, 8>(a, p0, nullptr, p1, nullptr, which, st);
+        case 16: return run_passes<KPair, 8, 16>(a, p0, nullptr, p1, nullptr, which, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
+                           int flip_float, hipStream_t st) {
+    if (bits != 8) return hipErrorInvalidValue;
+    uint32_t grid = (n + kWG * 16 - 1) / (kWG * 16);
+    if (grid == 0) grid = 1;
+    if (grid > 2048) grid = 2048;
+    hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist,
+                       npass, 0, flip_float);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
+                            unsigned long long cap, uint32_t* hist, int npass, int bbits,
+                            hipStream_t st) {
+    unsigned long long g = (cap + kWG * 16 - 1) / (kWG * 16);
+    uint32_t grid = (uint32_t)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
+    hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist,
+                       npass, bbits, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_flip(uint32_t* keys, uint32_t n, int unflip, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flip, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, unflip);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_prep(Ctrl* ctrl, uint32_t* hist, uint32_t nhist, int ctr_base,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_sort_prep, dim3(1), dim3(256), 0, st, ctrl, hist, nhist, ctr_base);
+    return hipGetLastError();
+}
+
+}  // namespace gsap

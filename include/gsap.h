@@ -1,0 +1,229 @@
+/*
+ * gsap.h -- C ABI of the MI355X (gfx950) sweep-and-prune broadphase and
+ * particle rigid-body step.
+ *
+ * Conventions
+ *  - Every entry point returns an int32 status (GSAP_OK = 0).  The message of
+ *    the last failure on the calling thread is returned by gsap_last_error().
+ *  - All array arguments named d_* are device pointers owned by the caller;
+ *    h_* are host pointers.  `stream` is a hipStream_t passed as void*
+ *    (NULL = the null stream).
+ *  - A handle is not thread-safe: use one handle per stream.
+ *  - No torch or C++ types cross this boundary.
+ *
+ * Reference interfaces replaced (paths relative to physicboxSap2.16_Fermi/):
+ *  gsap_sap_*         class Sap (sapDetection/Sap.h:12-84; collision
+ *                     Sap.cpp:211-335) and its extern "C" stage wrappers
+ *                     (sapDetection/SapKernel.cuh:4-56)
+ *  gsap_radix_sort_*  nvRadixSort::RadixSort (sapDetection/radixsort.h:110-278)
+ *  gsap_prb_*         class ParticleRigidBody (ParticleRigidBody/
+ *                     ParticleRigidBody.h:46-165) and PRBWrap::*
+ *                     (ParticleRigidBody/ParticleRigidBodyWrapper.h:11-54)
+ */
+#ifndef GSAP_H
+#define GSAP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSAP_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ status */
+enum {
+    GSAP_OK = 0,
+    GSAP_ERR_INVALID_ARG = 1, /* bad pointer / size / mode                    */
+    GSAP_ERR_CAPACITY = 2,    /* output too small; required size returned     */
+    GSAP_ERR_NONFINITE = 3,   /* NaN or Inf in the input boxes                */
+    GSAP_ERR_HIP = 4,         /* a HIP runtime call failed                    */
+    GSAP_ERR_UNSUPPORTED = 5, /* mode not available (e.g. no trained model)   */
+    GSAP_ERR_OOM = 6          /* device allocation failed                      */
+};
+
+/* Message of the last failure of any gsap_* call on the calling thread. */
+const char* gsap_last_error(void);
+/* Library version string (build id). */
+const char* gsap_version(void);
+/* Number of visible devices (0 when no GPU); never fails. */
+int32_t gsap_device_count(void);
+
+/* ------------------------------------------------------------------ SaP */
+enum { GSAP_INPUT_AABB = 0,   /* 6 floats per object: lo.x lo.y lo.z hi.x hi.y hi.z */
+       GSAP_INPUT_SPHERE = 1  /* float4 per object: x y z r (reference layout)      */ };
+
+enum { GSAP_AXIS_PCA = 0,          /* major eigenvector of the fp64 covariance     */
+       GSAP_AXIS_COORD_MAXVAR = 1, /* reference rule: coordinate axis of max var.  */
+       GSAP_AXIS_FIXED = 2,        /* desc.fixed_axis                              */
+       GSAP_AXIS_LEARNED = 3       /* learned selector, PCA fallback               */ };
+
+enum { GSAP_OUT_CANON = 0,  /* uint32x2 (i<j), lexicographic, each pair once    */
+       GSAP_OUT_SYM_CSR = 1 /* reference format: int32x2 both orientations,
+                               sorted by (i,j), plus pairEntries row starts     */ };
+
+enum { GSAP_FLAG_SKIP_SAME_GROUP = 1u /* drop pairs whose d_group ids are equal */ };
+
+typedef struct gsap_sap gsap_sap;
+
+typedef struct gsap_sap_desc {
+    uint32_t max_n;      /* objects per call the workspace is sized for (grows) */
+    int32_t input_kind;  /* GSAP_INPUT_*                                          */
+    int32_t axis_mode;   /* GSAP_AXIS_*                                           */
+    int32_t output_kind; /* GSAP_OUT_*                                            */
+    float fixed_axis[3]; /* GSAP_AXIS_FIXED (need not be normalised)              */
+    uint32_t flags;      /* GSAP_FLAG_*                                           */
+    int32_t digit_bits;  /* radix digit width (0 = tuned default; 4..8 allowed)   */
+    int32_t sort_items;  /* keys per thread in a sort tile (0 = tuned default)    */
+} gsap_sap_desc;
+
+typedef struct gsap_stats {
+    uint64_t num_pairs;      /* unordered pairs found                             */
+    uint64_t sweep_tests;    /* sum over sorted objects of candidates inside the
+                                projected interval (the sweep-test count)         */
+    float axis[3];           /* sweep axis used                                   */
+    int32_t axis_source;     /* GSAP_AXIS_* actually used (LEARNED may fall back) */
+    float ms_total;          /* hipEvent time of the whole update (stream-ordered)*/
+    uint32_t n;              /* objects in the call                               */
+} gsap_stats;
+
+/* Sap(uint n) (Sap.h:15): creates a handle and its workspace. */
+int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out);
+/* ~Sap() */
+int32_t gsap_sap_destroy(gsap_sap* h);
+
+/* Sap::collision (Sap.cpp:211-335).  One complete update from device boxes to
+ * a device pair list.
+ *   d_in           n objects (layout by desc.input_kind)
+ *   d_group        optional n uint32 group ids (GSAP_FLAG_SKIP_SAME_GROUP)
+ *   d_pairs        CANON: uint32x2 pairs; SYM_CSR: int32x2 directed pairs
+ *   pair_capacity  capacity of d_pairs in pairs (CANON) or directed pairs
+ *                  (SYM_CSR); 0 or d_pairs == NULL => count only (pre-count)
+ *   d_entries      SYM_CSR only: int32 row starts, capacity n
+ *   h_num_pairs    out: P (CANON) or 2P (SYM_CSR); on GSAP_ERR_CAPACITY the
+ *                  required capacity
+ *   h_num_entries  out (SYM_CSR): number of distinct first indices
+ *   stats          optional
+ * Blocks until the counts are on the host (one device->host copy). */
+int32_t gsap_sap_collide(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                         const uint32_t* d_group, void* d_pairs, uint64_t pair_capacity,
+                         int32_t* d_entries, uint64_t* h_num_pairs, uint32_t* h_num_entries,
+                         gsap_stats* stats);
+
+/* Stream-ordered variant: nothing is copied to the host and the call does not
+ * block.  d_counts (device, 5 x uint64) receives [0] = P or 2P, [1] = entries,
+ * [2] = sweep tests, [3] = status bits (1 = capacity overflow, 2 = non-finite
+ * input), [4] = candidate tests executed.  Suitable for hipGraph capture. */
+int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                               const uint32_t* d_group, void* d_pairs, uint64_t pair_capacity,
+                               int32_t* d_entries, uint64_t* d_counts);
+
+/* Batched update over many independent scenes packed back to back.
+ *   h_scene_offsets  n_scenes+1 offsets into d_in (host array)
+ *   d_pairs          CANON pairs with GLOBAL indices (into d_in), sorted,
+ *                    so scene s occupies d_scene_pair_offsets[s..s+1)
+ *   d_scene_pair_offsets  device, n_scenes+1 uint64
+ * One launch per stage covers every scene; the stage list is recorded into a
+ * hipGraph on first use for a given scene table and replayed afterwards. */
+int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
+                                 const uint32_t* h_scene_offsets, uint32_t n_scenes,
+                                 void* d_pairs, uint64_t pair_capacity,
+                                 uint64_t* d_scene_pair_offsets, uint64_t* h_num_pairs,
+                                 gsap_stats* stats);
+
+/* Per-stage timing of the last gsap_sap_collide (ms, hipEvents), for tuning.
+ * Stage names: moments, project, sort, gather, sweep, pair_sort, finish. */
+int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages, int32_t* n_stages);
+int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable);
+
+/* ------------------------------------------------------------------ radix sort */
+typedef struct gsap_radix_sort gsap_radix_sort;
+
+/* RadixSort(maxElements, keysOnly) (radixsort.h:121-145) */
+int32_t gsap_radix_sort_create(uint32_t max_elements, int32_t keys_only, gsap_radix_sort** out);
+int32_t gsap_radix_sort_destroy(gsap_radix_sort* s);
+/* RadixSort::sort(uint* keys, uint* values, uint n, uint keyBits)
+ * (radixsort.h:151-178): stable, in place; d_values may be NULL when keys_only. */
+int32_t gsap_radix_sort_u32(gsap_radix_sort* s, void* stream, uint32_t* d_keys,
+                            uint32_t* d_values, uint32_t n, uint32_t key_bits);
+/* RadixSort::sort(float* keys, uint* values, uint n, uint keyBits, bool
+ * negativeKeys) (radixsort.h:180-198): float keys ordered numerically. */
+int32_t gsap_radix_sort_f32(gsap_radix_sort* s, void* stream, float* d_keys, uint32_t* d_values,
+                            uint32_t n, uint32_t key_bits, int32_t negative_keys);
+
+/* ------------------------------------------------------------------ rigid bodies */
+typedef struct gsap_prb gsap_prb;
+
+enum {
+    GSAP_PRB_ABS_SPRING = 1u,     /* k, c absolute (BASELINE scene) not mass/dt derived */
+    GSAP_PRB_WALLS = 2u,          /* x and z walls                                       */
+    GSAP_PRB_FLOOR_CLAMP = 4u,    /* reference y clamp in integrate                      */
+    GSAP_PRB_CAP_VEL = 8u,        /* velocity cap (setMaxVelocity)                       */
+    GSAP_PRB_SKIP_SAME_BODY = 16u,/* broadphase drops pairs inside one body             */
+    GSAP_PRB_INERTIA = 32u        /* omega += Iw^-1 tau dt (reference: tau dt)          */
+};
+
+typedef struct gsap_prb_physics {
+    float gravity[3];   /* reference: (0,-5,0)                                     */
+    float k_abs, c_abs; /* GSAP_PRB_ABS_SPRING constants                           */
+    float floor_scale;  /* reference 0.1                                           */
+    float wall_lo[2];   /* x, z                                                    */
+    float wall_hi[2];
+    uint32_t flags;     /* GSAP_PRB_*                                              */
+    int32_t broadphase_kind; /* GSAP_INPUT_SPHERE (reference) or GSAP_INPUT_AABB  */
+} gsap_prb_physics;
+
+typedef struct gsap_prb_ptrs {
+    float* rb_pos;       /* float4 per body                                 */
+    float* rb_quat;      /* float4 per body (x,y,z,w)                       */
+    float* rb_linvel;
+    float* rb_angvel;
+    float* rb_inv_inertia; /* 12 floats per body (3 rows of float4)       */
+    float* p_pos;        /* float4 per particle, w = radius                 */
+    float* p_vel;
+    float* p_force;
+    float* p_mass;
+    int32_t* p_body;     /* body of each particle                           */
+    int32_t* pairs;      /* int32x2 symmetric sorted pair list              */
+    int32_t* pair_entries;
+    int32_t num_bodies, num_particles;
+    int64_t num_pairs, num_pair_entries;
+} gsap_prb_ptrs;
+
+/* ParticleRigidBody(maxNumBodies, maxNumParticles, maxPairs) */
+int32_t gsap_prb_create(int32_t max_bodies, int32_t max_particles, int64_t max_pairs,
+                        gsap_prb** out);
+int32_t gsap_prb_destroy(gsap_prb* h);
+/* setRbData (ParticleRigidBody.cpp:78-110): host arrays, normalises quats. */
+int32_t gsap_prb_set_rb_data(gsap_prb* h, int32_t num_bodies, const float* h_pos,
+                             const float* h_quat, const float* h_linvel, const float* h_angvel,
+                             const int32_t* h_shape, const int32_t* h_pstart,
+                             const int32_t* h_npart, const float* h_inv_mass,
+                             const float* h_inv_inertia, int32_t total_particles);
+/* setParticleData: per-shape particle templates, max_per_shape slots/shape. */
+int32_t gsap_prb_set_particle_data(gsap_prb* h, const float* h_ref_pos, const float* h_ref_mass,
+                                   int32_t num_elements, int32_t max_per_shape);
+int32_t gsap_prb_set_shape_inv_inertia(gsap_prb* h, const float* h_inv_inertia, int32_t n_shapes);
+int32_t gsap_prb_set_params(gsap_prb* h, float sp_multip, float dp_multip);
+int32_t gsap_prb_set_max_velocity(gsap_prb* h, float min_radius, float dt);
+int32_t gsap_prb_set_physics(gsap_prb* h, const gsap_prb_physics* phys);
+int32_t gsap_prb_update_particle_data(gsap_prb* h, void* stream);
+int32_t gsap_prb_prepare(gsap_prb* h, void* stream);
+/* comupteFoceOnParticles (sic, ParticleRigidBody.cpp:148-167) */
+int32_t gsap_prb_compute_force(gsap_prb* h, void* stream, float dt);
+int32_t gsap_prb_update_velocity(gsap_prb* h, void* stream, float dt);
+int32_t gsap_prb_integrate(gsap_prb* h, void* stream, float dt);
+/* Broadphase on the particles through a SaP handle, result kept in the
+ * handle's pair buffers (the reference calls Sap::collision + setNumPairs). */
+int32_t gsap_prb_collide(gsap_prb* h, void* stream, gsap_sap* sap);
+/* setNumPairs / setNumPairEntries after an external broadphase. */
+int32_t gsap_prb_set_num_pairs(gsap_prb* h, int64_t num_pairs, int64_t num_pair_entries);
+/* One DemoParticles::step (demoParticles.h:856-928): integrate, update
+ * particles, broadphase, prepare, forces, update velocity. */
+int32_t gsap_prb_step(gsap_prb* h, void* stream, gsap_sap* sap, float dt);
+int32_t gsap_prb_get_ptrs(gsap_prb* h, gsap_prb_ptrs* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSAP_H */

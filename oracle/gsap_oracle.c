@@ -5,11 +5,24 @@
  * Reference paths are relative to physicboxSap2.16_Fermi/ (PB/).
  * Build: gcc -O2 -std=c99 -ffp-contract=off -fPIC -shared (oracle/Makefile).
  */
+#define _POSIX_C_SOURCE 199309L
 #include "gsap_oracle.h"
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Phase times of the last or_sap call (setup = axis+project+sort, sweep,
+ * canonicalisation), for the CPU baseline. */
+static double g_times[3];
+void or_last_times(double out[3]) { out[0] = g_times[0]; out[1] = g_times[1]; out[2] = g_times[2]; }
 
 /* ------------------------------------------------------------------------ */
 /* Order-preserving float <-> uint32 map (Herf "radix tricks"), applied by    */
@@ -347,6 +360,7 @@ int64_t or_sap(const float* boxes, int kind, size_t n, int axis_mode, const floa
         if (tests_out) *tests_out = 0;
         return 0;
     }
+    double t0 = now_s();
     choose_axis(boxes, kind, n, axis_mode, fixed_axis, u);
     if (axis_out) { axis_out[0] = u[0]; axis_out[1] = u[1]; axis_out[2] = u[2]; }
     uint32_t* keys = (uint32_t*)malloc(n * 4);
@@ -357,6 +371,7 @@ int64_t or_sap(const float* boxes, int kind, size_t n, int axis_mode, const floa
     or_radix_sort_kv(keys, perm, n, 8);
     uint64_t P = 0, tests = 0;
     uint32_t stride = sample_stride ? sample_stride : 1;
+    double t1 = now_s();
     for (size_t s = 0; s < n; s += stride) {
         uint32_t i = perm[s];
         size_t end = upper_bound_u32(keys, n, hik[i]);
@@ -373,7 +388,12 @@ int64_t or_sap(const float* boxes, int kind, size_t n, int axis_mode, const floa
             }
         }
     }
-    if (P <= cap && stride == 1) or_canon_sort(pairs, P);
+    double t2 = now_s();
+    if (P <= cap) or_canon_sort(pairs, P);
+    double t3 = now_s();
+    g_times[0] = t1 - t0;
+    g_times[1] = t2 - t1;
+    g_times[2] = t3 - t2;
     if (tests_out) *tests_out = tests;
     free(keys); free(hik); free(perm);
     return (int64_t)P;

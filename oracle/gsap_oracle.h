@@ -69,10 +69,14 @@ int or_overlap_sphere(const float* a, const float* b);
  * which case only cap pairs are written and the result is unsorted-truncated).
  * tests_out: sum over sorted objects of the sweep interval length.
  * sample_stride > 1 sweeps only sorted objects s with s % stride == 0 (CPU
- * baseline timing of a bounded sample); then pairs are not canonicalised. */
+ * baseline timing of a bounded sample); the pairs found are still sorted. */
 int64_t or_sap(const float* boxes, int kind, size_t n, int axis_mode, const float* fixed_axis,
                const uint32_t* group, uint32_t* pairs, uint64_t cap, uint64_t* tests_out,
                float axis_out[3], uint32_t sample_stride);
+
+/* Wall times (s) of the last or_sap phases: setup (axis, projection, sort),
+ * sweep, canonical sort. */
+void or_last_times(double out[3]);
 
 /* Sweep-test count for a given axis (label source for the learned axis). */
 uint64_t or_sweep_tests(const float* boxes, int kind, size_t n, const float axis[3]);

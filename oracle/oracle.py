@@ -47,6 +47,7 @@ def lib():
         L.or_radix_sort_pairs.argtypes = [P, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_sap.restype = i64
         L.or_sap.argtypes = [P, ctypes.c_int, sz, ctypes.c_int, P, P, P, u64, P, P, ctypes.c_uint32]
+        L.or_last_times.argtypes = [P]
         L.or_sweep_tests.restype = u64
         L.or_sweep_tests.argtypes = [P, ctypes.c_int, sz, P]
         L.or_brute.restype = i64
@@ -148,6 +149,13 @@ def sap(boxes, kind=KIND_AABB, axis_mode=AXIS_PCA, fixed_axis=None, group=None, 
         if P <= cap:
             return pairs[:P].copy(), int(tests.value), ax
         cap = int(P)
+
+
+def last_times():
+    """(setup_s, sweep_s, canon_s) of the last sap() call."""
+    t = np.zeros(3, np.float64)
+    lib().or_last_times(_p(t))
+    return tuple(float(x) for x in t)
 
 
 def sweep_tests(boxes, axis, kind=KIND_AABB):
