@@ -350,8 +350,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         if (GROUP) hit = hit && (a.group[corig] != gme);
         const unsigned long long m = ballot(hit);
         if (m) {
-            if constexpr (MODE == 2) {
-                wcount += (unsigned long long)__popcll(m);
+            if constexpr (MODE == 2) {  // pre-count: directed pairs when symmetric
+                wcount += (unsigned long long)__popcll(m) << (a.sym ? 1 : 0);
             } else {
                 if (hit) {
                     const uint32_t p = wcnt + (MODE == 1 ? 2u : 1u) * mbcnt(m);
