@@ -66,11 +66,12 @@ struct gsap_sap {
     int items = 8;       // object sort: keys per thread (tile = 256 * items)
     int pair_items = 16; // pair sort
     Buf keys[2], vals[2], rec, partials, axis, ticket, hist_obj, hist_pair, lb_obj, lb_pair;
-    Buf ctrl, counts, ptmp, rowstart, has, block_sums, learned_axis;
+    Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
     unsigned long long* h_counts = nullptr;  // pinned
     uint32_t cap_n = 0, cap_tiles = 0, cap_seg = 0;
     uint64_t cap_pair_tiles = 0;
+    uint64_t slot_cap = 0;  // capacity of each internal pair buffer
     bool timing = false;
     hipEvent_t ev[8] = {};
     int nev = 0;
@@ -155,8 +156,9 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
     (void)hipDeviceSynchronize();
     Buf* bufs[] = {&h->keys[0], &h->keys[1], &h->vals[0], &h->vals[1], &h->rec, &h->partials,
                    &h->axis, &h->ticket, &h->hist_obj, &h->hist_pair, &h->lb_obj, &h->lb_pair,
-                   &h->ctrl, &h->counts, &h->ptmp, &h->rowstart, &h->has, &h->block_sums,
-                   &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0, &h->tile_seg};
+                   &h->ctrl, &h->counts, &h->ptmp[0], &h->ptmp[1], &h->rowstart, &h->has,
+                   &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
+                   &h->tile_seg, &h->grp, &h->grp_end};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
     for (int i = 0; i < 8; ++i)
@@ -176,6 +178,8 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
             HIPCHK(h, grow(h->vals[i], (size_t)want * 4));
         }
         HIPCHK(h, grow(h->rec, (size_t)want * 32));
+        HIPCHK(h, grow(h->grp, (size_t)want * 32));
+        HIPCHK(h, grow(h->grp_end, (size_t)want * 4));
         HIPCHK(h, grow(h->rowstart, (size_t)want * 4));
         HIPCHK(h, grow(h->has, (size_t)want * 4));
         HIPCHK(h, hipMemset(h->has.p, 0, (size_t)want * 4));
@@ -203,9 +207,18 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     return GSAP_OK;
 }
 
-static int32_t ensure_pairs(gsap_sap* h, uint64_t cap) {
-    HIPCHK(h, grow(h->ptmp, (size_t)cap * 8));
-    const uint64_t tiles = (cap + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
+// Internal pair buffers: the sweep reserves slots in runs, so they hold the
+// caller's capacity plus one run per sweep wave (the sentinel-padded tails).
+static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg) {
+    const uint64_t waves = (uint64_t)n / 64 + 8ull * nseg + 64;
+    const uint64_t slots = cap + waves * kSweepSlotRun;
+    if (slots >= (1ull << 32) - (1ull << 20))
+        return fail(h, GSAP_ERR_UNSUPPORTED, "pair capacity too large (slot indices are 32-bit)");
+    if (slots > h->slot_cap) {
+        for (int i = 0; i < 2; ++i) HIPCHK(h, grow(h->ptmp[i], (size_t)slots * 8));
+        h->slot_cap = slots;
+    }
+    const uint64_t tiles = (h->slot_cap + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
     if (tiles > h->cap_pair_tiles) {
         HIPCHK(h, grow(h->lb_pair, (size_t)tiles * 256 * 8));
         HIPCHK(h, hipMemset(h->lb_pair.p, 0, (size_t)tiles * 256 * 8));
@@ -254,6 +267,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.keys2 = (uint32_t*)h->keys[1].p;
     a.vals2 = (uint32_t*)h->vals[1].p;
     a.rec = h->rec.p;
+    a.grp = h->grp.p;
+    a.grp_end = (uint32_t*)h->grp_end.p;
     a.learned_axis = (h->desc.axis_mode == GSAP_AXIS_LEARNED) ? (const float*)h->learned_axis.p
                                                               : nullptr;
     HIPCHK(h, launch_moments(a, st));
@@ -280,6 +295,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
 
     SweepArgs w{};
     w.rec = h->rec.p;
+    w.grp = h->grp.p;
+    w.grp_end = (const uint32_t*)h->grp_end.p;
     w.n = n;
     w.seg = seg;
     w.tile = tile;
@@ -289,19 +306,18 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     w.kind = kind;
     w.sym = sym;
     w.count_only = count_only;
-    w.cap = count_only ? 0 : cap;
     uint2* outp = (uint2*)pairs;
-    uint2* tmpp = (uint2*)h->ptmp.p;
-    uint2* first = (npass_pair % 2 == 0) ? outp : tmpp;
-    uint2* second = (npass_pair % 2 == 0) ? tmpp : outp;
-    w.out = count_only ? nullptr : first;
+    uint2* t0p = (uint2*)h->ptmp[0].p;
+    uint2* t1p = (uint2*)h->ptmp[1].p;
+    w.out = count_only ? nullptr : t0p;
+    w.slot_cap = count_only ? 0 : h->slot_cap;
     HIPCHK(h, launch_sweep(w, st));
     mark(h, st);
     if (!count_only) {
-        HIPCHK(h, launch_pair_hist(first, &ctrl->num_pairs, cap, a.hist_pair, npass_pair, bbits, st));
+        HIPCHK(h, launch_pair_hist(t0p, &ctrl->slots, h->slot_cap, a.hist_pair, npass_pair, bbits, st));
         SortArgs p{};
-        p.n_dev = &ctrl->num_pairs;
-        p.cap = cap;
+        p.n_dev = &ctrl->slots;
+        p.cap = h->slot_cap;
         p.hist = a.hist_pair;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
@@ -311,8 +327,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.items = h->pair_items;
         p.bbits = (int)bbits;
         p.grid = 2048;
-        int pw = 0;
-        HIPCHK(h, sort_pairs(p, first, second, &pw, st));
+        p.last_n = &ctrl->num_pairs;
+        p.last_cap = cap;
+        HIPCHK(h, sort_pairs(p, t0p, t1p, outp, st));
         if (sym)
             HIPCHK(h, launch_entries(outp, &ctrl->num_pairs, cap, n, (uint32_t*)h->rowstart.p,
                                      (uint32_t*)h->has.p, (uint32_t*)h->block_sums.p, entries,
@@ -355,7 +372,7 @@ extern "C" int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void*
     rc = ensure_obj(h, n, 1, ntiles);
     if (rc) return rc;
     if (!count_only) {
-        rc = ensure_pairs(h, pair_capacity);
+        rc = ensure_pairs(h, pair_capacity, n, 1);
         if (rc) return rc;
     }
     SegTable seg{};
@@ -486,7 +503,7 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
         rc = upload_scenes(h, st, h_scene_offsets, n_scenes, &nt);
         if (rc) return rc;
         if (!count_only) {
-            rc = ensure_pairs(h, pair_capacity);
+            rc = ensure_pairs(h, pair_capacity, n, n_scenes);
             if (rc) return rc;
         }
         SegTable seg{};

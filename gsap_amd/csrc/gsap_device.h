@@ -13,7 +13,9 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t l = lane_id();
     return (l == 0) ? 0ull : (~0ull >> (64 - l));
 }
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// Lane mask of p.  The wave64 builtin takes the i1 directly, so a mask that
+// is already in SGPRs (v_cmp + s_and) is used as is.
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 // popcount of a 64-bit mask restricted to lanes below this one (v_mbcnt pair)
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),

@@ -24,17 +24,20 @@ struct SegTable {
     uint32_t ntiles;
 };
 
-// Sorted-object record consumed by the sweep (32 bytes, one scalar load).
+// Sorted-object record (32 bytes).  Two copies are written after the sort:
+// the candidate stream in sorted order (aux unused) and, per 256-object
+// sub-tile, the objects re-ordered by sweep end (aux = sorted position), read
+// by the sweep through the scalar cache.
 struct alignas(16) RecAabb {
     float lo[3];
     float hi[3];
     uint32_t orig;  // index into the caller's input (global in batched mode)
-    uint32_t end;   // first sorted position whose key exceeds this object's upper key
+    uint32_t aux;
 };
 struct alignas(16) RecSphere {
     float c[4];     // x y z r
     uint32_t orig;
-    uint32_t end;
+    uint32_t aux;
     uint32_t pad[2];
 };
 
@@ -45,6 +48,7 @@ struct Ctrl {
     unsigned long long tests;        // sweep-test count
     unsigned long long num_entries;  // SYM_CSR row starts
     unsigned long long exec_tests;   // candidate tests executed (wave granularity)
+    unsigned long long slots;        // pair-buffer slots reserved by the sweep
     uint32_t flags;                  // 1 = capacity overflow, 2 = non-finite input
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
@@ -76,7 +80,9 @@ struct ObjPipeArgs {
     uint32_t* vals;         // [n] orig index (ping)
     uint32_t* keys2;
     uint32_t* vals2;
-    void* rec;              // [n] sorted records
+    void* rec;              // [n] sorted records (candidate stream)
+    void* grp;              // [n] records grouped by sweep end per 256-object sub-tile
+    uint32_t* grp_end;      // [n] sweep end of each grouped record
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
 };
 
@@ -86,14 +92,16 @@ hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* sorted_keys,
                          const uint32_t* perm, hipStream_t st);
 
 struct SweepArgs {
-    const void* rec;
+    const void* rec;         // candidates, sorted order
+    const void* grp;         // objects, end-grouped per 256-object sub-tile
+    const uint32_t* grp_end;
     uint32_t n;
     SegTable seg;
     uint32_t tile;
     uint32_t ntiles;
     const uint32_t* group;   // per original index, or null
-    uint2* out;
-    unsigned long long cap;
+    uint2* out;              // slot buffer (sentinel padded)
+    unsigned long long slot_cap;
     Ctrl* ctrl;
     int kind;
     int sym;                 // emit both orientations
@@ -131,12 +139,16 @@ struct SortArgs {
     int items;                          // keys per thread
     int bbits;                          // pairs: bits of the second index
     uint32_t grid;                      // persistent grid size
+    const unsigned long long* last_n;   // final pass writes only positions < min(*last_n,
+    unsigned long long last_cap;        //   last_cap) (pairs: drop the sentinel tail)
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
 hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      int* which, hipStream_t st);
-hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, int* which, hipStream_t st);
+// Pairs: passes ping-pong between p0 and p1; the final pass writes `out`.
+hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st);
+constexpr uint32_t kSweepSlotRun = 512;  // pair-buffer slots reserved per sweep atomic
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
                            int flip_float, hipStream_t st);

@@ -170,20 +170,38 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    if (tid == 0) {
+    {
+        // Last tile of the scene: reduce every tile's partials in a fixed
+        // order (per-thread strided sums, then the wave/LDS tree).
         const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg] : 0u;
         const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg + 1] : a.ntiles;
-        double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t t = t0; t < t1; ++t)
+        double m[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) m[k] = 0.0;
+        for (uint32_t t = t0 + tid; t < t1; t += kWG)
+#pragma unroll
             for (int k = 0; k < 9; ++k)
                 m[k] += __hip_atomic_load(&a.partials[(size_t)t * 10 + k], __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
-        float u[3], src;
-        const float* learned = a.learned_axis ? a.learned_axis + 4 * (size_t)ti.seg : nullptr;
-        choose_axis(m, ti.seg_end - ti.seg_start, a.axis_mode, a.fixed_axis, learned, u, &src);
-        float* ax = a.axis + 4 * (size_t)ti.seg;
-        ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
-        a.seg_ticket[ti.seg] = 0u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            double v = wave_sum(m[k]);
+            if (l == 0) s_red[w][k] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double mm[9];
+            for (int k = 0; k < 9; ++k) {
+                mm[k] = 0.0;
+                for (int ww = 0; ww < kWaves; ++ww) mm[k] += s_red[ww][k];
+            }
+            float u[3], src;
+            const float* learned = a.learned_axis ? a.learned_axis + 4 * (size_t)ti.seg : nullptr;
+            choose_axis(mm, ti.seg_end - ti.seg_start, a.axis_mode, a.fixed_axis, learned, u, &src);
+            float* ax = a.axis + 4 * (size_t)ti.seg;
+            ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
+            a.seg_ticket[ti.seg] = 0u;
+        }
     }
 }
 
@@ -219,187 +237,219 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
 }
 
 // ---------------------------------------------------------------- gather
+// Sorted position s -> candidate record rec[s]; sweep end (first position
+// whose key exceeds s's upper key) by binary search; the sweep-test count.
+// Each 256-object sub-tile is also written re-ordered by sweep end into
+// grp/grp_end (aux = s), the object side of the sweep.
+template <int KIND>
 __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* __restrict__ K,
                                                 const uint32_t* __restrict__ V) {
+    using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
+    __shared__ uint32_t s_end[kWG];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const float* ax = a.axis + 4 * (size_t)ti.seg;
     const float u[3] = {ax[0], ax[1], ax[2]};
-    const bool exact = axis_exact(a.kind, u);
+    const bool exact = axis_exact(KIND, u);
     unsigned long long tests = 0;
-    for (uint32_t s = ti.lo + tid; s < ti.hi; s += kWG) {
-        const uint32_t orig = V[s];
-        float lo[3], hi[3], c4[4];
-        load_lohi(a.in, a.kind, orig, lo, hi, c4);
-        float pl, ph;
-        project_lohi(a.kind, lo, hi, c4, u, exact, &pl, &ph);
-        const uint32_t hk = flip_f32(ph);
-        uint32_t b = s + 1, e = ti.seg_end;  // first t in [s+1, seg_end) with K[t] > hk
-        while (b < e) {
-            const uint32_t mid = b + ((e - b) >> 1);
-            if (K[mid] <= hk) b = mid + 1; else e = mid;
-        }
-        tests += (unsigned long long)(b - s - 1);
-        if (a.kind == 0) {
-            RecAabb r;
-            r.lo[0] = lo[0]; r.lo[1] = lo[1]; r.lo[2] = lo[2];
-            r.hi[0] = hi[0]; r.hi[1] = hi[1]; r.hi[2] = hi[2];
+    for (uint32_t sub = ti.lo; sub < ti.hi; sub += kWG) {
+        const uint32_t s = sub + tid;
+        const bool valid = s < ti.hi;
+        Rec r;
+        uint32_t end = 0xFFFFFFFFu;  // invalid lanes rank last
+        if (valid) {
+            const uint32_t orig = V[s];
+            float lo[3], hi[3], c4[4];
+            load_lohi(a.in, KIND, orig, lo, hi, c4);
+            float pl, ph;
+            project_lohi(KIND, lo, hi, c4, u, exact, &pl, &ph);
+            const uint32_t hk = flip_f32(ph);
+            uint32_t b = s + 1, e = ti.seg_end;  // first t in [s+1, seg_end) with K[t] > hk
+            while (b < e) {
+                const uint32_t mid = b + ((e - b) >> 1);
+                if (K[mid] <= hk) b = mid + 1; else e = mid;
+            }
+            end = b;
+            tests += (unsigned long long)(b - s - 1);
+            if constexpr (KIND == 0) {
+                r.lo[0] = lo[0]; r.lo[1] = lo[1]; r.lo[2] = lo[2];
+                r.hi[0] = hi[0]; r.hi[1] = hi[1]; r.hi[2] = hi[2];
+            } else {
+                r.c[0] = c4[0]; r.c[1] = c4[1]; r.c[2] = c4[2]; r.c[3] = c4[3];
+                r.pad[0] = r.pad[1] = 0u;
+            }
             r.orig = orig;
-            r.end = b;
-            reinterpret_cast<RecAabb*>(a.rec)[s] = r;
-        } else {
-            RecSphere r;
-            r.c[0] = c4[0]; r.c[1] = c4[1]; r.c[2] = c4[2]; r.c[3] = c4[3];
-            r.orig = orig;
-            r.end = b;
-            r.pad[0] = r.pad[1] = 0u;
-            reinterpret_cast<RecSphere*>(a.rec)[s] = r;
+            r.aux = 0u;
+            reinterpret_cast<Rec*>(a.rec)[s] = r;
         }
+        s_end[tid] = end;
+        __syncthreads();
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < kWG; ++j) {
+            const uint32_t ej = s_end[j];
+            rank += (ej < end) | ((ej == end) & (j < tid));
+        }
+        if (valid) {
+            r.aux = s;
+            reinterpret_cast<Rec*>(a.grp)[sub + rank] = r;
+            a.grp_end[sub + rank] = end;
+        }
+        __syncthreads();
     }
     tests = wave_sum(tests);
     if (lane_id() == 0 && tests) atomicAdd(&a.ctrl->tests, tests);
 }
 
 // ---------------------------------------------------------------- sweep
-// One workgroup per 256 consecutive sorted objects of one scene.  Objects are
-// re-ordered inside the workgroup by their sweep end so each wave's 64 lanes
-// have similar candidate ranges; every lane is one object, and the wave walks
-// the union of its lanes' candidate ranges one candidate at a time.  The
-// candidate record is wave-uniform (scalar loads); each lane runs the exact
-// predicate, hits are compacted with ballot + mbcnt into a per-wave LDS
-// buffer, and flushed with one global atomic per batch.  Candidates past a
-// lane's own sweep end can never overlap it (the projected intervals are
-// conservative), so only the t > s condition needs a mask.
+// One wave per 64 objects of a 256-object sub-tile, taken in sweep-end order.
+// Lanes are candidates: the wave walks its objects' candidate range 64 sorted
+// positions at a time (coalesced vector loads of the candidate stream), and
+// for each still-active object (scalar loads; the wave's 64 objects, 2 KB,
+// stay in the scalar cache) every lane runs the exact predicate against its
+// candidate.  Objects whose sweep end precedes the chunk are retired, so the
+// work is the sweep-test count plus under one chunk per object.  Candidates
+// past an object's sweep end can never overlap it (the projected intervals
+// are conservative); only t > s needs a mask, and only near the sub-tile.
+// Hits go straight to global memory: each wave reserves 512-slot runs of the
+// pair buffer with one atomic, writes its hits contiguously (ballot + mbcnt),
+// and pads the unused tail of its last run with all-ones sentinels that the
+// pair sort moves past the real pairs.
+constexpr uint32_t kSlotRun = kSweepSlotRun;
+
 template <int KIND, bool GROUP, int MODE>  // MODE 0 canonical, 1 symmetric, 2 count
 __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
-    constexpr uint32_t BUF = 1024;
-    __shared__ uint32_t s_end[kSweepTile];
-    __shared__ uint32_t s_ord[kSweepTile];
-    __shared__ uint2 s_buf[MODE == 2 ? 1 : kWaves][MODE == 2 ? 1 : BUF];
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
-    // Records are read-only here; the constant address space lets uniform
-    // candidate loads go through the scalar cache (s_load_dwordx8).
+    // The object records are read-only here; the constant address space lets
+    // their wave-uniform loads go through the scalar cache.
 #if defined(__HIP_DEVICE_COMPILE__)
     typedef const __attribute__((address_space(4))) Rec CRec;
 #else
     typedef const Rec CRec;
 #endif
-    CRec* __restrict__ R = (CRec*)(a.rec);
+    const Rec* __restrict__ R = reinterpret_cast<const Rec*>(a.rec);
 
     const uint32_t subs = a.tile / kSweepTile;
     const uint32_t tile = blockIdx.x / subs, sub = blockIdx.x % subs;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const uint32_t lo = ti.lo + sub * kSweepTile;
     if (lo >= ti.hi) return;
-    const uint32_t hi = min(lo + kSweepTile, ti.hi), cnt = hi - lo;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const uint32_t hi = min(lo + kSweepTile, ti.hi);
+    const uint32_t lane = lane_id(), w = uniform(threadIdx.x >> 6);
+    const uint32_t g0 = lo + 64u * w;
+    if (g0 >= hi) return;  // no block-level synchronisation below
+    const uint32_t nobj = uniform(min(64u, hi - g0));
+    const uint32_t ge = lane < nobj ? a.grp_end[g0 + lane] : 0u;  // ascending in lane order
+    const uint32_t wend = uniform(wave_max_u32(ge));
+    CRec* __restrict__ O = (CRec*)(a.grp) + g0;
+    // directed pairs when symmetric (also in the pre-count)
+    const uint32_t mult = (MODE == 1 || (MODE == 2 && a.sym)) ? 2u : 1u;
+    const uint32_t cap = (uint32_t)a.slot_cap;  // < 2^32, checked on the host
 
-    const uint32_t e0 = tid < cnt ? reinterpret_cast<const Rec*>(a.rec)[lo + tid].end : 0u;
-    s_end[tid] = e0;
-    __syncthreads();
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < kSweepTile; ++j) {
-        const uint32_t ej = s_end[j];
-        rank += (ej < e0) | ((ej == e0) & (j < tid));
-    }
-    s_ord[rank] = tid;
-    __syncthreads();
-    const uint32_t my = s_ord[tid];
-    const bool act = my < cnt;
-    const uint32_t pos = lo + my;
-    Rec me = reinterpret_cast<const Rec*>(a.rec)[act ? pos : lo];
-    const uint32_t myend = act ? me.end : 0u;
-    const uint32_t wend = uniform(wave_max_u32(myend));
-    uint32_t gme = 0;
-    if (GROUP && act) gme = a.group[me.orig];
+    uint32_t npairs = 0;  // pairs (or directed pairs) found by this wave (< 64 * 2n)
+    uint32_t cur = 0, lim = 0;      // current slot run [cur, lim)
 
-    unsigned long long wcount = 0;  // MODE 2
-    uint32_t wcnt = 0;              // staged entries (uniform)
-    unsigned long long base_dummy = 0;
-    (void)base_dummy;
-
-    auto flush = [&]() {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&a.ctrl->num_pairs, (unsigned long long)wcnt);
-        base = __shfl(base, 0, 64);
-        for (uint32_t k = lane; k < wcnt; k += 64) {
-            if (base + k < a.cap) a.out[base + k] = s_buf[w][k];
-        }
-        wcnt = 0;
-    };
-
-    auto test = [&](const Rec& c) -> bool {
-        bool hit;
-        if constexpr (KIND == 0) {
-            hit = (c.lo[0] <= me.hi[0]) & (me.lo[0] <= c.hi[0]) & (c.lo[1] <= me.hi[1]) &
-                  (me.lo[1] <= c.hi[1]) & (c.lo[2] <= me.hi[2]) & (me.lo[2] <= c.hi[2]);
-        } else {
-#pragma clang fp contract(off)
-            const float x0 = me.c[0] - c.c[0];
-            const float y0 = me.c[1] - c.c[1];
-            const float z0 = me.c[2] - c.c[2];
-            const float r0 = me.c[3] + c.c[3];
-            const float t1 = (x0 * x0 + y0 * y0) + z0 * z0;
-            const float e1 = r0 * r0;
-            hit = t1 < e1;
-        }
-        return hit;
-    };
-
-    auto handle = [&](bool hit, uint32_t corig) {
-        if (GROUP) hit = hit && (a.group[corig] != gme);
-        const unsigned long long m = ballot(hit);
-        if (m) {
-            if constexpr (MODE == 2) {  // pre-count: directed pairs when symmetric
-                wcount += (unsigned long long)__popcll(m) << (a.sym ? 1 : 0);
+    uint32_t kfirst = 0;
+    for (uint32_t t0 = lo + 1; t0 < wend; t0 += 64u) {
+        // retire objects whose sweep end is at or before this chunk
+        kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
+        const uint32_t t = t0 + lane;
+        Rec c = R[t < wend ? t : wend - 1u];
+        if (t >= wend) {  // a candidate that overlaps nothing
+            if constexpr (KIND == 0) {
+                c.lo[0] = c.lo[1] = c.lo[2] = __builtin_inff();
+                c.hi[0] = c.hi[1] = c.hi[2] = -__builtin_inff();
             } else {
-                if (hit) {
-                    const uint32_t p = wcnt + (MODE == 1 ? 2u : 1u) * mbcnt(m);
-                    if constexpr (MODE == 0) {
-                        const uint32_t x = me.orig < corig ? me.orig : corig;
-                        const uint32_t y = me.orig < corig ? corig : me.orig;
-                        s_buf[w][p] = make_uint2(x, y);
-                    } else {
-                        s_buf[w][p] = make_uint2(me.orig, corig);
-                        s_buf[w][p + 1] = make_uint2(corig, me.orig);
-                    }
-                }
-                wcnt += (MODE == 1 ? 2u : 1u) * (uint32_t)__popcll(m);
-                if (wcnt > BUF - 128u) flush();
+                c.c[0] = __builtin_inff();
             }
         }
-    };
+        uint32_t gc = 0;
+        if (GROUP) gc = a.group[c.orig];
 
-    // phase A: candidates inside this workgroup's own range (need t > s)
-    const uint32_t a_end = min(hi, wend);
-    for (uint32_t t = lo + 1; t < a_end; ++t) {
-        const Rec c = R[t];
-        const bool h = test(c) & act & (t > pos);
-        handle(h, c.orig);
+        auto process = [&](const Rec& o, auto masked_tag) {
+            constexpr bool MASKED = decltype(masked_tag)::value;
+            bool hit;
+            if constexpr (KIND == 0) {
+                // Inclusive AABB test (aabb.h:47-57) on the vector ALU only:
+                // for finite floats with denormals preserved (the kernel's f32
+                // denorm mode), fl(x - y) <= 0 exactly when x <= y, so the six
+                // compares become six subtractions, a max tree and one compare
+                // -- no scalar-ALU mask arithmetic, which is the scarcer unit
+                // (one scalar ALU per CU vs four SIMDs).
+                const float d0 = c.lo[0] - o.hi[0], d1 = o.lo[0] - c.hi[0];
+                const float d2 = c.lo[1] - o.hi[1], d3 = o.lo[1] - c.hi[1];
+                const float d4 = c.lo[2] - o.hi[2], d5 = o.lo[2] - c.hi[2];
+                const float mx = fmaxf(fmaxf(fmaxf(d0, d1), d2), fmaxf(fmaxf(d3, d4), d5));
+                hit = mx <= 0.0f;
+            } else {
+#pragma clang fp contract(off)
+                const float x0 = o.c[0] - c.c[0];
+                const float y0 = o.c[1] - c.c[1];
+                const float z0 = o.c[2] - c.c[2];
+                const float r0 = o.c[3] + c.c[3];
+                const float t1 = (x0 * x0 + y0 * y0) + z0 * z0;
+                const float e1 = r0 * r0;
+                hit = t1 < e1;
+            }
+            if (MASKED) hit = hit & (t > o.aux);
+            if (GROUP) hit = hit & (a.group[o.orig] != gc);
+            const unsigned long long m = ballot(hit);
+            if (m == 0ull) return;
+            const uint32_t cnt = (uint32_t)__popcll(m) * mult;
+            npairs += cnt;
+            if constexpr (MODE != 2) {
+                const uint32_t r = mult * mbcnt(m);
+                const uint32_t rem = lim - cur;
+                uint32_t nb = 0;
+                const bool fresh = cnt > rem;
+                if (fresh) {  // reserve the next run (one atomic per wave)
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(&a.ctrl->slots, (unsigned long long)kSlotRun);
+                    nb = uniform((uint32_t)base);
+                }
+                if (hit) {
+                    const uint32_t p0 = r < rem ? cur + r : nb + (r - rem);
+                    if (MODE == 0) {
+                        const uint32_t x = o.orig < c.orig ? o.orig : c.orig;
+                        const uint32_t y = o.orig < c.orig ? c.orig : o.orig;
+                        if (p0 < cap) a.out[p0] = make_uint2(x, y);
+                    } else {
+                        const uint32_t p1 = r + 1 < rem ? cur + r + 1 : nb + (r + 1 - rem);
+                        if (p0 < cap) a.out[p0] = make_uint2(o.orig, c.orig);
+                        if (p1 < cap) a.out[p1] = make_uint2(c.orig, o.orig);
+                    }
+                }
+                if (fresh) {
+                    cur = nb + (cnt - rem);
+                    lim = nb + kSlotRun;
+                } else {
+                    cur += cnt;
+                }
+            }
+        };
+        auto objects = [&](auto masked_tag) {
+            uint32_t k = kfirst;
+            for (; k + 4 <= nobj; k += 4) {  // four scalar loads in flight
+                const Rec o0 = O[k], o1 = O[k + 1], o2 = O[k + 2], o3 = O[k + 3];
+                process(o0, masked_tag);
+                process(o1, masked_tag);
+                process(o2, masked_tag);
+                process(o3, masked_tag);
+            }
+            for (; k < nobj; ++k) process(O[k], masked_tag);
+        };
+        if (t0 < hi) objects(std::true_type{});  // an object may sit at or after t
+        else objects(std::false_type{});
     }
-    // phase B: candidates after the workgroup's range, four per step so the
-    // scalar loads of the next candidates overlap the tests of the current.
-    uint32_t t = hi;
-    for (; t + 4 <= wend; t += 4) {
-        const Rec c0 = R[t], c1 = R[t + 1], c2 = R[t + 2], c3 = R[t + 3];
-        const bool h0 = test(c0) & act, h1 = test(c1) & act;
-        const bool h2 = test(c2) & act, h3 = test(c3) & act;
-        handle(h0, c0.orig);
-        handle(h1, c1.orig);
-        handle(h2, c2.orig);
-        handle(h3, c3.orig);
+    if (lane == 0 && npairs) atomicAdd(&a.ctrl->num_pairs, (unsigned long long)npairs);
+    if constexpr (MODE != 2) {
+        for (uint32_t p = cur + lane; p < lim; p += 64)
+            if (p < cap) a.out[p] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
     }
-    for (; t < wend; ++t) {
-        const Rec c = R[t];
-        handle(test(c) & act, c.orig);
-    }
-    if constexpr (MODE == 2) {
-        if (lane == 0 && wcount) atomicAdd(&a.ctrl->num_pairs, wcount);
-    } else {
-        if (wcnt) flush();
-    }
-    if (lane == 0 && wend > lo + 1)
-        atomicAdd(&a.ctrl->exec_tests, (unsigned long long)(wend - lo - 1) * 64ull);
+    // candidate tests executed: every active object takes whole 64-lane chunks
+    unsigned long long ex = 0;
+    if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 63u) / 64u) * 64u;
+    ex = wave_sum(ex);
+    if (lane == 0 && ex) atomicAdd(&a.ctrl->exec_tests, ex);
 }
 
 // ---------------------------------------------------------------- CSR entries
@@ -471,6 +521,7 @@ __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
         ctrl->num_entries = 0;
         ctrl->tests = 0;
         ctrl->exec_tests = 0;
+        ctrl->slots = 0;
         ctrl->flags = 0;
     }
 }
@@ -503,7 +554,10 @@ hipError_t launch_project(const ObjPipeArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* K, const uint32_t* V, hipStream_t st) {
-    hipLaunchKernelGGL(k_gather, dim3(a.ntiles), dim3(kWG), 0, st, a, K, V);
+    if (a.kind == 0)
+        hipLaunchKernelGGL(k_gather<0>, dim3(a.ntiles), dim3(kWG), 0, st, a, K, V);
+    else
+        hipLaunchKernelGGL(k_gather<1>, dim3(a.ntiles), dim3(kWG), 0, st, a, K, V);
     return hipGetLastError();
 }
 

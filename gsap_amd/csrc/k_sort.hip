@@ -67,6 +67,13 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     const uint32_t tag = ((a.ctrl->epoch << 4) | ctr_idx) & 0x7FFFFFFFu;
     const unsigned long long ltm = lanemask_lt();
     const unsigned long long gtm = ~ltm & ~(1ull << lane);
+    // The final pass of a pair sort writes only the real pairs (the sentinel
+    // tail sorts last) and never past the caller's capacity.
+    unsigned long long out_lim = ~0ull;
+    if (a.last_n && (uint32_t)pass + 1u == a.npass) {
+        const unsigned long long ln = *a.last_n;
+        out_lim = ln < a.last_cap ? ln : a.last_cap;
+    }
 
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_ctr[ctr_idx], 1u);
@@ -165,8 +172,10 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
             const K kk = s_k[i];
             const uint32_t d = KT::digit(kk, shift, a.bbits, MASK);
             const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
-            kout[g] = kk;
-            if (KT::kHasVals) vout[g] = s_v[i];
+            if (g < out_lim) {
+                kout[g] = kk;
+                if (KT::kHasVals) vout[g] = s_v[i];
+            }
         }
         __syncthreads();
     }
@@ -217,7 +226,8 @@ uint32_t sort_tile(int items) { return (uint32_t)(kWG * items); }
 
 template <class KT, int BITS, int ITEMS>
 static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0,
-                             typename KT::T* k1, uint32_t* v1, int* which, hipStream_t st) {
+                             typename KT::T* k1, uint32_t* v1, int* which, hipStream_t st,
+                             typename KT::T* final_out = nullptr) {
     const uint32_t TILE = kWG * ITEMS;
     uint32_t grid = a.grid;
     if (!a.n_dev) {
@@ -229,7 +239,7 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
     int cur = 0;
     for (uint32_t p = 0; p < a.npass; ++p) {
         typename KT::T* ki = cur ? k1 : k0;
-        typename KT::T* ko = cur ? k0 : k1;
+        typename KT::T* ko = (final_out && p + 1 == a.npass) ? final_out : (cur ? k0 : k1);
         uint32_t* vi = cur ? v1 : v0;
         uint32_t* vo = cur ? v0 : v1;
         hipLaunchKernelGGL((k_onesweep<KT, BITS, ITEMS>), dim3(grid), dim3(kWG), 0, st, ki, ko, vi, vo,
@@ -254,11 +264,12 @@ hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1
     }
 }
 
-hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, int* which, hipStream_t st) {
+hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st) {
     if (a.bits != 8) return hipErrorInvalidValue;
+    int which = 0;
     switch (a.items) {
-        case 8: return run_passes<KPair, 8, 8>(a, p0, nullptr, p1, nullptr, which, st);
-        case 16: return run_passes<KPair, 8, 16>(a, p0, nullptr, p1, nullptr, which, st);
+        case 8: return run_passes<KPair, 8, 8>(a, p0, nullptr, p1, nullptr, &which, st, out);
+        case 16: return run_passes<KPair, 8, 16>(a, p0, nullptr, p1, nullptr, &which, st, out);
         default: return hipErrorInvalidValue;
     }
 }
