@@ -38,7 +38,12 @@ EXPORTS = [
     "gsap_prb_set_physics", "gsap_prb_update_particle_data", "gsap_prb_prepare",
     "gsap_prb_compute_force", "gsap_prb_update_velocity", "gsap_prb_integrate",
     "gsap_prb_collide", "gsap_prb_set_num_pairs", "gsap_prb_step", "gsap_prb_get_ptrs",
+    "gsap_mlp_create", "gsap_mlp_destroy", "gsap_mlp_num_params", "gsap_mlp_set_params",
+    "gsap_mlp_get_params", "gsap_mlp_forward", "gsap_mlp_loss_grad", "gsap_mlp_train",
+    "gsap_axis_candidates", "gsap_sap_set_learned_model", "gsap_sap_features_batched",
+    "gsap_sap_axis_tests_batched",
 ]
+N_FEATURES, N_PARAMS, N_CANDIDATES = 64, 25348, 64
 
 
 class GsapError(RuntimeError):
@@ -120,6 +125,18 @@ def lib():
         "gsap_prb_set_num_pairs": (i32, [P, i64, i64]),
         "gsap_prb_step": (i32, [P, P, P, f32]),
         "gsap_prb_get_ptrs": (i32, [P, P]),
+        "gsap_mlp_create": (i32, [u64, P]),
+        "gsap_mlp_destroy": (i32, [P]),
+        "gsap_mlp_num_params": (i32, []),
+        "gsap_mlp_set_params": (i32, [P, P, u32]),
+        "gsap_mlp_get_params": (i32, [P, P, u32]),
+        "gsap_mlp_forward": (i32, [P, P, P, u32, P]),
+        "gsap_mlp_loss_grad": (i32, [P, P, P, P, u32, f32, P, P]),
+        "gsap_mlp_train": (i32, [P, P, P, P, u32, u32, u32, f32, f32, P]),
+        "gsap_axis_candidates": (i32, [P]),
+        "gsap_sap_set_learned_model": (i32, [P, P, f32]),
+        "gsap_sap_features_batched": (i32, [P, P, P, P, u32, P]),
+        "gsap_sap_axis_tests_batched": (i32, [P, P, P, P, u32, P, u32, P, P]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -242,6 +259,33 @@ class Sap:
         P, ne, _ = self.collide_into(d_particle_data, pairs, pair_entries)
         return P, ne
 
+    def set_learned_model(self, model, conf_min=0.95):
+        check(lib().gsap_sap_set_learned_model(self.h, model.h, conf_min))
+
+    def features_batched(self, boxes, scene_offsets, stream=None):
+        import numpy as np
+        import torch
+        off = np.ascontiguousarray(scene_offsets, dtype=np.uint32)
+        ns = len(off) - 1
+        f = torch.empty((ns, N_FEATURES), dtype=torch.float32, device=boxes.device)
+        check(lib().gsap_sap_features_batched(self.h, _stream(stream), _ptr(boxes),
+                                              off.ctypes.data_as(ctypes.c_void_p), ns, _ptr(f)))
+        return f
+
+    def axis_tests_batched(self, boxes, scene_offsets, axes, with_pairs=True, stream=None):
+        import numpy as np
+        import torch
+        off = np.ascontiguousarray(scene_offsets, dtype=np.uint32)
+        ax = np.ascontiguousarray(axes, dtype=np.float32).reshape(-1, 3)
+        ns = len(off) - 1
+        tests = torch.empty((ns, len(ax)), dtype=torch.int64, device=boxes.device)
+        pairs = torch.empty(ns, dtype=torch.int64, device=boxes.device) if with_pairs else None
+        check(lib().gsap_sap_axis_tests_batched(self.h, _stream(stream), _ptr(boxes),
+                                                off.ctypes.data_as(ctypes.c_void_p), ns,
+                                                ax.ctypes.data_as(ctypes.c_void_p), len(ax),
+                                                _ptr(tests), _ptr(pairs)))
+        return tests, pairs
+
     def collide_batched(self, boxes, scene_offsets, pairs, scene_pair_offsets, stream=None):
         import numpy as np
         off = np.ascontiguousarray(scene_offsets, dtype=np.uint32)
@@ -256,6 +300,78 @@ class Sap:
             raise GsapError(rc, f"capacity {cap} < {npairs.value}")
         check(rc)
         return int(npairs.value), st
+
+
+def axis_candidates():
+    """The 64 fixed candidate sweep axes (64 x 3 float32), coordinate axes first."""
+    import numpy as np
+    a = np.zeros((N_CANDIDATES, 3), np.float32)
+    check(lib().gsap_axis_candidates(a.ctypes.data_as(ctypes.c_void_p)))
+    return a
+
+
+class AxisModel:
+    """Learned sweep-axis selector: 64-128-128-4 MLP (bf16 weights and
+    activations, fp32 accumulation) run and trained by libgsap's HIP kernels."""
+
+    def __init__(self, seed=0):
+        self.h = ctypes.c_void_p()
+        check(lib().gsap_mlp_create(seed, ctypes.byref(self.h)))
+
+    def params(self):
+        import numpy as np
+        p = np.zeros(N_PARAMS, np.float32)
+        check(lib().gsap_mlp_get_params(self.h, p.ctypes.data_as(ctypes.c_void_p), N_PARAMS))
+        return p
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().gsap_mlp_set_params(self.h, p.ctypes.data_as(ctypes.c_void_p), len(p)))
+
+    def save(self, path):
+        self.params().tofile(path)
+
+    @classmethod
+    def load(cls, path):
+        import numpy as np
+        m = cls()
+        m.set_params(np.fromfile(path, np.float32))
+        return m
+
+    def forward(self, x, stream=None):
+        import torch
+        y = torch.empty((x.shape[0], 4), dtype=torch.float32, device=x.device)
+        check(lib().gsap_mlp_forward(self.h, _stream(stream), _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def loss_grad(self, x, t, lambda_count=0.05, stream=None):
+        import numpy as np
+        g = np.zeros(N_PARAMS, np.float32)
+        loss = ctypes.c_float(0)
+        check(lib().gsap_mlp_loss_grad(self.h, _stream(stream), _ptr(x), _ptr(t), x.shape[0],
+                                       lambda_count, ctypes.byref(loss),
+                                       g.ctypes.data_as(ctypes.c_void_p)))
+        return float(loss.value), g
+
+    def train(self, x, t, epochs=10, batch=1000, lr=1e-3, lambda_count=0.05, stream=None):
+        import numpy as np
+        losses = np.zeros(epochs, np.float32)
+        check(lib().gsap_mlp_train(self.h, _stream(stream), _ptr(x), _ptr(t), x.shape[0], epochs,
+                                   batch, lr, lambda_count,
+                                   losses.ctypes.data_as(ctypes.c_void_p)))
+        return losses
+
+    def close(self):
+        if self.h:
+            lib().gsap_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class RadixSort:

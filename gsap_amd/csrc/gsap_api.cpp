@@ -68,6 +68,9 @@ struct gsap_sap {
     Buf keys[2], vals[2], rec, partials, axis, ticket, hist_obj, hist_pair, lb_obj, lb_pair;
     Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
+    Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
+    bool has_model = false;
+    float conf_min = 0.95f;
     unsigned long long* h_counts = nullptr;  // pinned
     uint32_t cap_n = 0, cap_tiles = 0, cap_seg = 0;
     uint64_t cap_pair_tiles = 0;
@@ -158,7 +161,8 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
                    &h->axis, &h->ticket, &h->hist_obj, &h->hist_pair, &h->lb_obj, &h->lb_pair,
                    &h->ctrl, &h->counts, &h->ptmp[0], &h->ptmp[1], &h->rowstart, &h->has,
                    &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
-                   &h->tile_seg, &h->grp, &h->grp_end};
+                   &h->tile_seg, &h->grp, &h->grp_end, &h->feats, &h->tile_sums, &h->seg_mom,
+                   &h->cand, &h->mlp_wb, &h->mlp_master, &h->ax_tests_axes};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
     for (int i = 0; i < 8; ++i)
@@ -191,6 +195,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         HIPCHK(h, grow(h->lb_obj, (size_t)ntiles * 256 * 8));
         HIPCHK(h, hipMemset(h->lb_obj.p, 0, (size_t)ntiles * 256 * 8));
         HIPCHK(h, grow(h->tile_seg, (size_t)ntiles * 4));
+        HIPCHK(h, grow(h->tile_sums, (size_t)ntiles * kFeatSums * 4));
         h->cap_tiles = ntiles;
     }
     if (nseg > h->cap_seg) {
@@ -201,6 +206,8 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         HIPCHK(h, grow(h->seg_start, (size_t)nseg * 4));
         HIPCHK(h, grow(h->seg_n, (size_t)nseg * 4));
         HIPCHK(h, grow(h->seg_tile0, (size_t)(nseg + 1) * 4));
+        HIPCHK(h, grow(h->seg_mom, (size_t)nseg * 9 * 8));
+        HIPCHK(h, grow(h->feats, (size_t)nseg * kFeat * 4));
         h->cap_seg = nseg;
     }
     if (!h->hist_pair.p) HIPCHK(h, grow(h->hist_pair, kMaxPasses * 256 * 4));
@@ -269,9 +276,28 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.rec = h->rec.p;
     a.grp = h->grp.p;
     a.grp_end = (uint32_t*)h->grp_end.p;
-    a.learned_axis = (h->desc.axis_mode == GSAP_AXIS_LEARNED) ? (const float*)h->learned_axis.p
-                                                              : nullptr;
+    a.learned_axis = nullptr;
+    a.seg_mom = (double*)h->seg_mom.p;
+    const bool learned = h->desc.axis_mode == GSAP_AXIS_LEARNED;
+    if (learned) {
+        if (!h->has_model)
+            return fail(h, GSAP_ERR_UNSUPPORTED, "GSAP_AXIS_LEARNED needs gsap_sap_set_learned_model");
+        a.axis_mode = GSAP_AXIS_PCA;  // the fallback; confident scenes are overwritten below
+    }
     HIPCHK(h, launch_moments(a, st));
+    if (learned) {
+        HIPCHK(h, launch_features(a, (float*)h->tile_sums.p, (float*)h->feats.p, st));
+        MlpWeights mw;
+        mw.w1 = (const uint16_t*)h->mlp_wb.p;
+        mw.w2 = mw.w1 + kFeat * kHid;
+        mw.w3 = mw.w2 + kHid * kHid;
+        const float* mb = (const float*)h->mlp_master.p;
+        mw.b1 = mb + kFeat * kHid;
+        mw.b2 = mb + kFeat * kHid + kHid + kHid * kHid;
+        mw.b3 = mb + kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut;
+        HIPCHK(h, launch_axis_infer((const float*)h->feats.p, seg.nseg, mw, (const float*)h->cand.p,
+                                    h->conf_min, a.axis, st));
+    }
     HIPCHK(h, launch_project(a, st));
     mark(h, st);
 
@@ -663,4 +689,98 @@ extern "C" int32_t gsap_radix_sort_f32(gsap_radix_sort* s, void* stream, float* 
     (void)key_bits;
     (void)negative_keys;  // the order-preserving map handles either sign
     return rs_sort(s, (hipStream_t)stream, (uint32_t*)d_keys, d_values, n, 32, true);
+}
+
+// ---------------------------------------------------------------- learned-axis support
+namespace gsap_internal_api {
+int32_t attach_model(gsap_sap* h, const uint16_t* d_wb, size_t wb_bytes, const float* d_master,
+                     size_t master_bytes, float conf_min) {
+    if (!h) return fail(nullptr, GSAP_ERR_INVALID_ARG, "null handle");
+    if (!(conf_min > 0.0f && conf_min <= 1.0f)) return fail(h, GSAP_ERR_INVALID_ARG, "conf_min in (0,1]");
+    HIPCHK(h, grow(h->mlp_wb, wb_bytes));
+    HIPCHK(h, grow(h->mlp_master, master_bytes));
+    HIPCHK(h, grow(h->cand, kCand * 3 * 4));
+    HIPCHK(h, hipDeviceSynchronize());
+    HIPCHK(h, hipMemcpy(h->mlp_wb.p, d_wb, wb_bytes, hipMemcpyDeviceToDevice));
+    HIPCHK(h, hipMemcpy(h->mlp_master.p, d_master, master_bytes, hipMemcpyDeviceToDevice));
+    float cand[kCand * 3];
+    gsap_axis_candidates(cand);
+    HIPCHK(h, hipMemcpy(h->cand.p, cand, sizeof cand, hipMemcpyHostToDevice));
+    h->conf_min = conf_min;
+    h->has_model = true;
+    return GSAP_OK;
+}
+}  // namespace gsap_internal_api
+
+static SegTable seg_table(gsap_sap* h, uint32_t ns, uint32_t nt) {
+    SegTable seg{};
+    seg.seg_start = (const uint32_t*)h->seg_start.p;
+    seg.seg_n = (const uint32_t*)h->seg_n.p;
+    seg.seg_tile0 = (const uint32_t*)h->seg_tile0.p;
+    seg.tile_seg = (const uint32_t*)h->tile_seg.p;
+    seg.nseg = ns;
+    seg.ntiles = nt;
+    return seg;
+}
+
+extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const void* d_in,
+                                             const uint32_t* h_off, uint32_t ns, float* d_features) {
+    if (!h || !d_in || !h_off || ns == 0 || !d_features)
+        return fail(h, GSAP_ERR_INVALID_ARG, "gsap_sap_features_batched: bad argument");
+    if (h->desc.input_kind != GSAP_INPUT_AABB)
+        return fail(h, GSAP_ERR_UNSUPPORTED, "features are defined for AABB scenes");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(h, hipStreamSynchronize(st));
+    uint32_t nt = 0;
+    if (int32_t rc = upload_scenes(h, st, h_off, ns, &nt)) return rc;
+    ObjPipeArgs a{};
+    a.in = d_in;
+    a.kind = h->desc.input_kind;
+    a.n = h_off[ns];
+    a.seg = seg_table(h, ns, nt);
+    a.tile = sort_tile(h->items);
+    a.ntiles = nt;
+    a.axis_mode = GSAP_AXIS_PCA;
+    a.axis = (float*)h->axis.p;
+    a.partials = (double*)h->partials.p;
+    a.seg_ticket = (uint32_t*)h->ticket.p;
+    a.ctrl = (Ctrl*)h->ctrl.p;
+    a.hist_obj = (uint32_t*)h->hist_obj.p;
+    a.npass_obj = 4;
+    a.hist_pair = (uint32_t*)h->hist_pair.p;
+    a.npass_pair = 0;
+    a.seg_mom = (double*)h->seg_mom.p;
+    if (nt == 0) return fail(h, GSAP_ERR_INVALID_ARG, "all scenes empty");
+    for (uint32_t s = 0; s < ns; ++s)
+        if (h_off[s + 1] == h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "empty scene");
+    HIPCHK(h, launch_moments(a, st));
+    HIPCHK(h, launch_features(a, (float*)h->tile_sums.p, d_features, st));
+    // the moments kernel bumped the epoch and cleared counters it owns; reset
+    // the accumulators the finish kernel normally resets
+    HIPCHK(h, launch_finish((Ctrl*)h->ctrl.p, ~0ull, (unsigned long long*)h->counts.p, 0, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const void* d_in,
+                                               const uint32_t* h_off, uint32_t ns, const float* h_axes,
+                                               uint32_t naxes, uint64_t* d_tests, uint64_t* d_pairs) {
+    if (!h || !d_in || !h_off || ns == 0 || !h_axes || naxes == 0 || !d_tests)
+        return fail(h, GSAP_ERR_INVALID_ARG, "gsap_sap_axis_tests_batched: bad argument");
+    for (uint32_t s = 0; s < ns; ++s) {
+        if (h_off[s + 1] < h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
+        if (h_off[s + 1] - h_off[s] > 2048)
+            return fail(h, GSAP_ERR_UNSUPPORTED, "axis tests: scenes of at most 2048 objects");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(h, hipStreamSynchronize(st));
+    uint32_t nt = 0;
+    if (int32_t rc = upload_scenes(h, st, h_off, ns, &nt)) return rc;
+    HIPCHK(h, grow(h->ax_tests_axes, (size_t)naxes * 12));
+    HIPCHK(h, hipMemcpy(h->ax_tests_axes.p, h_axes, (size_t)naxes * 12, hipMemcpyHostToDevice));
+    HIPCHK(h, launch_axis_tests(d_in, h->desc.input_kind, (const uint32_t*)h->seg_start.p,
+                                (const uint32_t*)h->seg_n.p, ns, (const float*)h->ax_tests_axes.p,
+                                naxes, (unsigned long long*)d_tests, (unsigned long long*)d_pairs, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    return GSAP_OK;
 }

@@ -84,6 +84,7 @@ struct ObjPipeArgs {
     void* grp;              // [n] records grouped by sweep end per 256-object sub-tile
     uint32_t* grp_end;      // [n] sweep end of each grouped record
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
+    double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by the last tile)
 };
 
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st);
@@ -122,6 +123,30 @@ hipError_t launch_scene_offsets(const uint2* P, Ctrl* ctrl, unsigned long long c
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
 hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts,
                          int sym, hipStream_t st);
+
+// ---- learned sweep axis (k_mlp.hip) ----
+constexpr int kFeat = 64, kHid = 128, kOut = 4, kCand = 64;
+constexpr int kTgt = kCand + 1;  // training target row: 64 candidate regrets, log(1 + pairs)
+constexpr int kFeatSums = 39;  // per-tile partial sums feeding the features
+struct MlpWeights {          // bf16 copies used by the kernels (row-major [in][out])
+    const uint16_t *w1, *w2, *w3;
+    const float *b1, *b2, *b3;
+};
+hipError_t launch_features(const ObjPipeArgs& a, float* tile_sums, float* feats, hipStream_t st);
+hipError_t launch_axis_infer(const float* feats, uint32_t nseg, MlpWeights w, const float* cand,
+                             float conf_min, float* axis, hipStream_t st);
+hipError_t launch_axis_tests(const void* in, int kind, const uint32_t* seg_start,
+                             const uint32_t* seg_n, uint32_t nseg, const float* axes, uint32_t naxes,
+                             unsigned long long* tests, unsigned long long* pairs, hipStream_t st);
+hipError_t mlp_forward(const float* x, uint32_t B, MlpWeights w, uint16_t* xb, uint16_t* h1,
+                       uint16_t* h2, float* y, hipStream_t st);
+hipError_t mlp_backward(const float* y, const float* target, uint32_t B, MlpWeights w,
+                        const uint16_t* xb, const uint16_t* h1, const uint16_t* h2, float* dh1,
+                        float* dh2, float* dy, float* grads, float lambda_count, float kappa,
+                        const float* cand, float* loss, hipStream_t st);
+hipError_t mlp_adam(float* master, const float* grads, float* m, float* v, uint32_t nparam,
+                    float lr, float b1, float b2, float eps, uint32_t step, uint16_t* w_bf16,
+                    const uint32_t* bf16_map, hipStream_t st);
 
 // ---- radix sort (k_sort.hip) ----
 struct SortArgs {

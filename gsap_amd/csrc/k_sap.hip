@@ -200,6 +200,8 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
             choose_axis(mm, ti.seg_end - ti.seg_start, a.axis_mode, a.fixed_axis, learned, u, &src);
             float* ax = a.axis + 4 * (size_t)ti.seg;
             ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
+            if (a.seg_mom)
+                for (int k = 0; k < 9; ++k) a.seg_mom[9 * (size_t)ti.seg + k] = mm[k];
             a.seg_ticket[ti.seg] = 0u;
         }
     }

@@ -136,6 +136,49 @@ int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
 int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages, int32_t* n_stages);
 int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable);
 
+/* ------------------------------------------------------------------ learned axis */
+/* No reference counterpart.  A 64-128-128-4 MLP (bf16 weights/activations,
+ * fp32 accumulation, hand-written HIP forward/backward/Adam) maps 64 per-scene
+ * features to a sweep axis (snapped to the nearest of 64 fixed candidates) and
+ * a log pair count.  GSAP_AXIS_LEARNED uses it per scene and keeps the PCA axis
+ * where the prediction is not confident.  The axis never changes the pair set. */
+typedef struct gsap_mlp gsap_mlp;
+int32_t gsap_mlp_create(uint64_t seed, gsap_mlp** out);
+int32_t gsap_mlp_destroy(gsap_mlp* m);
+int32_t gsap_mlp_num_params(void);
+/* fp32 parameters: W1[64][128] b1[128] W2[128][128] b2[128] W3[128][4] b3[4] */
+int32_t gsap_mlp_set_params(gsap_mlp* m, const float* h_params, uint32_t n);
+int32_t gsap_mlp_get_params(gsap_mlp* m, float* h_params, uint32_t n);
+/* d_x: n x 64 fp32 features (rounded to bf16 on input); d_y: n x 4 */
+int32_t gsap_mlp_forward(gsap_mlp* m, void* stream, const float* d_x, uint32_t n, float* d_y);
+/* d_t: n x 65 targets per scene -- the regret tests_c / min_c tests - 1 of each
+ * of the 64 candidate axes, then log(1 + pairs).  Loss = mean over rows of
+ * sum_c softmax_c(30 (u.a_c)^2) regret_c + lambda (y3 - log(1+pairs))^2 with
+ * u = y[0:3]/|y[0:3]| (expected regret of the axes the prediction points at).
+ * Gradient only. */
+int32_t gsap_mlp_loss_grad(gsap_mlp* m, void* stream, const float* d_x, const float* d_t,
+                           uint32_t n, float lambda_count, float* h_loss, float* h_grads);
+/* Adam (0.9, 0.999, 1e-8) over contiguous minibatches, `epochs` passes. */
+int32_t gsap_mlp_train(gsap_mlp* m, void* stream, const float* d_x, const float* d_t, uint32_t n,
+                       uint32_t epochs, uint32_t batch, float lr, float lambda_count,
+                       float* h_epoch_loss);
+/* The 64 fixed candidate axes (64 x 3 floats): coordinate axes first. */
+int32_t gsap_axis_candidates(float* h_axes);
+/* Attach a trained model to a handle (weights are copied); conf_min in (0,1]
+ * is the |cos| between prediction and snapped candidate required to use it. */
+int32_t gsap_sap_set_learned_model(gsap_sap* h, const gsap_mlp* m, float conf_min);
+/* Per-scene features (n_scenes x 64 fp32, device) of packed AABB scenes. */
+int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const void* d_in,
+                                  const uint32_t* h_scene_offsets, uint32_t n_scenes,
+                                  float* d_features);
+/* The judge: exact sweep-test count of every scene on every axis
+ * (d_tests: n_scenes x n_axes) and each scene's pair count (d_pairs, may be
+ * NULL).  Scenes of at most 2048 objects. */
+int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const void* d_in,
+                                    const uint32_t* h_scene_offsets, uint32_t n_scenes,
+                                    const float* h_axes, uint32_t n_axes, uint64_t* d_tests,
+                                    uint64_t* d_pairs);
+
 /* ------------------------------------------------------------------ radix sort */
 typedef struct gsap_radix_sort gsap_radix_sort;
 
