@@ -26,6 +26,143 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _time_ms(torch, fn, steps, warmup):
+    """Mean ms per call of fn over `steps` calls after `warmup`, from events on
+    torch's current stream (the stream every library call here launches on)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_batched(G, W, torch, dev, ns, steps, warmup):
+    """512 packed scenes (sizes log-uniform in [1024, 65536], seed 4) in one
+    graph-replayed update, against the same kernels run scene by scene."""
+    import numpy as np
+    boxes, off = W.scenes(ns, seed=4)
+    ntot = int(off[-1])
+    bd = torch.from_numpy(boxes).to(dev)
+    sb = G.Sap(max_n=ntot)
+    P, st = sb.collide_batched(bd, off, None, None)
+    pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
+    poff = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+    ms_b = _time_ms(torch, lambda: sb.collide_batched(bd, off, pairs, poff), steps, warmup)
+    po = poff.cpu().numpy()
+    sp = G.Sap(max_n=int(np.max(np.diff(off))))
+    lp = torch.empty_like(pairs)
+    counts = torch.zeros((ns, 8), dtype=torch.int64, device=dev)
+    views = [(bd[off[s]:off[s + 1]], lp[po[s]:po[s + 1]]) for s in range(ns)]
+
+    def loop_async():
+        for s in range(ns):
+            sp.collide_async(views[s][0], views[s][1], counts[s])
+
+    def loop_sync():
+        for s in range(ns):
+            sp.collide_into(views[s][0], views[s][1])
+
+    ms_la = _time_ms(torch, loop_async, max(1, steps // 4), 1)
+    ms_ls = _time_ms(torch, loop_sync, max(1, steps // 4), 1)
+    loop_async()
+    torch.cuda.synchronize()
+    shift = torch.repeat_interleave(torch.from_numpy(off[:-1].astype(np.int32)).to(dev),
+                                    torch.from_numpy(np.diff(po)).to(dev))
+    same = bool(torch.equal(counts[:, 0].cpu(), torch.from_numpy(np.diff(po))) and
+                torch.equal(lp[:P] + shift[:, None], pairs[:P]))
+    return {"workload": f"{ns} scenes, n log-U[1024,65536] seed 4, uniform or 8-cluster centres, "
+                        "half-extents U[0.1,0.6]",
+            "boxes": ntot, "pairs": P, "sweep_tests": int(st.sweep_tests),
+            "ms_batched": round(ms_b, 4), "updates_per_s": round(ntot / (ms_b / 1e3), 1),
+            "ms_per_scene_loop_async": round(ms_la, 4),
+            "ms_per_scene_loop_sync": round(ms_ls, 4),
+            "speedup_vs_loop_async": round(ms_la / ms_b, 2),
+            "speedup_vs_loop_sync": round(ms_ls / ms_b, 2),
+            "pairs_identical": same}
+
+
+def bench_learned(G, W, torch, dev, ns):
+    """2048 packed scenes (seed 5): sweep tests and update time with the PCA
+    axis against the trained selector; the pair lists must be identical."""
+    from gsap_amd import learn as L
+    if not os.path.exists(L.DEFAULT_MODEL):
+        return {"skipped": "no trained model at " + os.path.relpath(L.DEFAULT_MODEL, ROOT)}
+    model = G.AxisModel.load(L.DEFAULT_MODEL)
+    boxes, off = W.scenes(ns, seed=5)
+    bd = torch.from_numpy(boxes).to(dev)
+    ev = L.evaluate(model, bd, off, conf_min=0.95)
+    ms = {}
+    for name, mode in (("pca", G.AXIS_PCA), ("learned", G.AXIS_LEARNED)):
+        s = G.Sap(max_n=int(off[-1]), axis_mode=mode)
+        if mode == G.AXIS_LEARNED:
+            s.set_learned_model(model, 0.95)
+        P, _ = s.collide_batched(bd, off, None, None)
+        pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
+        po = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+        ms[name] = _time_ms(torch, lambda: s.collide_batched(bd, off, pairs, po), 5, 2)
+        s.close()
+    return {"workload": f"{ns} scenes, n log-U[1024,65536] seed 5", "boxes": int(off[-1]),
+            "pairs": ev["pairs"], "sweep_tests_pca": ev["tests_pca"],
+            "sweep_tests_learned": ev["tests_learned"], "reduction": round(ev["reduction"], 4),
+            "pairs_identical": ev["pairs_identical"], "ms_batched_pca": round(ms["pca"], 4),
+            "ms_batched_learned": round(ms["learned"], 4)}
+
+
+def bench_particles(G, W, torch, dev, nb, steps, with_cpu):
+    """Rigid cubes (3x3x3 particles, radius 0.5, spacing 1) dropped into a
+    256x256 walled box from heights U[0,200]: spring-dashpot k=1e4, c=50,
+    g=-9.81, dt=1e-4, per-step SaP on the particle AABBs (same-body pairs
+    skipped), symmetric CSR pairs feeding the force kernel."""
+    import numpy as np
+    sc = W.bodies(nb, seed=3)
+    npart = nb * 27
+    flags = G.PRB_ABS_SPRING | G.PRB_WALLS | G.PRB_SKIP_SAME_BODY
+    kw = dict(gravity=(0, -9.81, 0), k_abs=1e4, c_abs=50.0, floor_scale=1.0, wall_lo=(0, 0),
+              wall_hi=(256.0, 256.0), flags=flags)
+    prb = G.ParticleRigidBody(nb, npart, npart * 8)
+    prb.set_rb_data(sc["pos"], sc["quat"], sc["linvel"], sc["angvel"], sc["shape"], sc["pstart"],
+                    sc["npart"], sc["inv_mass"], sc["inv_inertia"])
+    prb.set_particle_data(sc["ref_pos"], sc["ref_mass"], sc["max_per_shape"])
+    prb.set_particle_shape_inv_inertia(sc["shape_inv_inertia"])
+    prb.set_physics(broadphase_kind=G.INPUT_AABB, **kw)
+    sap = G.Sap(max_n=npart, input_kind=G.INPUT_AABB, output_kind=G.OUT_SYM_CSR,
+                flags=G.FLAG_SKIP_SAME_GROUP)
+    dt = 1e-4
+    ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
+    p = prb.ptrs()  # raises on pair-buffer overflow
+    out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase",
+           "particles": npart, "steps": steps, "ms_per_step": round(ms, 4),
+           "steps_per_s": round(1e3 / ms, 1), "particle_updates_per_s": round(npart * 1e3 / ms, 1),
+           "directed_pairs_last_step": int(p.num_pairs)}
+    if with_cpu:
+        # Bounded sample: one full-size oracle step whose sweep visits every
+        # 64th sorted particle; the sweep and canonical-sort time is scaled
+        # back up (a full oracle step takes minutes on one core).
+        from oracle import oracle as O
+        stride = 64
+        ref = O.PrbState(sc["pos"], sc["quat"] / np.linalg.norm(sc["quat"], axis=1, keepdims=True),
+                         sc["linvel"], sc["angvel"], sc["shape"], sc["pstart"], sc["npart"],
+                         sc["inv_mass"], sc["ref_pos"], sc["ref_mass"], sc["max_per_shape"],
+                         sc["shape_inv_inertia"])
+        params = O.prb_params(**kw)
+        t = time.perf_counter()
+        ref.step(dt, params, broadphase="aabb", sample_stride=stride)
+        meas = time.perf_counter() - t
+        _, sweep, canon = O.last_times()
+        est = meas + (stride - 1) * (sweep + canon)
+        out["cpu_baseline"] = {"value": round(npart / est, 1), "unit": "particle updates/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"1 full-size oracle step sweeping every {stride}th sorted "
+                                         f"particle ({meas:.1f}s), sweep+canonical sort scaled "
+                                         f"x{stride}: {est:.1f}s per step"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -35,6 +172,12 @@ def main():
     ap.add_argument("--axis", default="pca", choices=["pca", "coord", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=64)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched, learned-axis and particle-step sections")
+    ap.add_argument("--scenes", type=int, default=512, help="batched workload size")
+    ap.add_argument("--learned-scenes", type=int, default=2048)
+    ap.add_argument("--prb-bodies", type=int, default=65536)
+    ap.add_argument("--prb-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -135,6 +278,11 @@ def main():
         "wall_s": round(wall, 3),
         "cpu_baseline": cpu,
     }
+    if not args.no_extras:
+        out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup)
+        out["learned_axis"] = bench_learned(G, W, torch, dev, args.learned_scenes)
+        out["particle_step"] = bench_particles(G, W, torch, dev, args.prb_bodies, args.prb_steps,
+                                               not args.no_cpu_baseline)
     print(json.dumps(out))
 
 

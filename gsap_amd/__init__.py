@@ -236,6 +236,15 @@ class Sap:
         check(rc)
         return int(npairs.value), int(nent.value), st
 
+    def collide_async(self, boxes, pairs, counts, entries=None, group=None, stream=None):
+        """One update with no host synchronisation: the five counts (pairs,
+        entries, sweep tests, flags, executed tests) land in the uint64 device
+        tensor `counts` (a view of >= 5 int64 elements)."""
+        n = boxes.shape[0]
+        cap = pairs.shape[0] if pairs is not None else 0
+        check(lib().gsap_sap_collide_async(self.h, _stream(stream), _ptr(boxes), n, _ptr(group),
+                                           _ptr(pairs), cap, _ptr(entries), _ptr(counts)))
+
     def collide(self, boxes, group=None, stream=None):
         """Pre-count, allocate, update.  Returns (pairs, entries, stats)."""
         import torch

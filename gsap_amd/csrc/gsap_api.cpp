@@ -65,7 +65,7 @@ struct gsap_sap {
     std::string err;
     int items = 8;       // object sort: keys per thread (tile = 256 * items)
     int pair_items = 16; // pair sort
-    Buf keys[2], vals[2], rec, partials, axis, ticket, hist_obj, hist_pair, lb_obj, lb_pair;
+    Buf keys[2], vals[2], rec, partials, axis, hist_obj, hist_pair, lb_obj, lb_pair;
     Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
@@ -158,7 +158,7 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
     if (!h) return GSAP_OK;
     (void)hipDeviceSynchronize();
     Buf* bufs[] = {&h->keys[0], &h->keys[1], &h->vals[0], &h->vals[1], &h->rec, &h->partials,
-                   &h->axis, &h->ticket, &h->hist_obj, &h->hist_pair, &h->lb_obj, &h->lb_pair,
+                   &h->axis, &h->hist_obj, &h->hist_pair, &h->lb_obj, &h->lb_pair,
                    &h->ctrl, &h->counts, &h->ptmp[0], &h->ptmp[1], &h->rowstart, &h->has,
                    &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
                    &h->tile_seg, &h->grp, &h->grp_end, &h->feats, &h->tile_sums, &h->seg_mom,
@@ -200,8 +200,6 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     }
     if (nseg > h->cap_seg) {
         HIPCHK(h, grow(h->axis, (size_t)nseg * 16));
-        HIPCHK(h, grow(h->ticket, (size_t)nseg * 4));
-        HIPCHK(h, hipMemset(h->ticket.p, 0, (size_t)nseg * 4));
         HIPCHK(h, grow(h->hist_obj, (size_t)nseg * 4 * 256 * 4));
         HIPCHK(h, grow(h->seg_start, (size_t)nseg * 4));
         HIPCHK(h, grow(h->seg_n, (size_t)nseg * 4));
@@ -214,13 +212,14 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     return GSAP_OK;
 }
 
-// Internal pair buffers: the sweep reserves slots in runs, so they hold the
-// caller's capacity plus one run per sweep wave (the sentinel-padded tails).
+// Internal pair buffers: the sweep writes exactly the pairs it finds (dense),
+// so they hold the caller's capacity.
 static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg) {
-    const uint64_t waves = (uint64_t)n / 64 + 8ull * nseg + 64;
-    const uint64_t slots = cap + waves * kSweepSlotRun;
+    (void)n;
+    (void)nseg;
+    const uint64_t slots = cap > 0 ? cap : 1;
     if (slots >= (1ull << 32) - (1ull << 20))
-        return fail(h, GSAP_ERR_UNSUPPORTED, "pair capacity too large (slot indices are 32-bit)");
+        return fail(h, GSAP_ERR_UNSUPPORTED, "pair capacity too large (pair indices are 32-bit)");
     if (slots > h->slot_cap) {
         for (int i = 0; i < 2; ++i) HIPCHK(h, grow(h->ptmp[i], (size_t)slots * 8));
         h->slot_cap = slots;
@@ -263,7 +262,6 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     for (int k = 0; k < 3; ++k) a.fixed_axis[k] = h->desc.fixed_axis[k];
     a.axis = (float*)h->axis.p;
     a.partials = (double*)h->partials.p;
-    a.seg_ticket = (uint32_t*)h->ticket.p;
     a.ctrl = ctrl;
     a.hist_obj = (uint32_t*)h->hist_obj.p;
     a.npass_obj = 4;
@@ -743,7 +741,6 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     a.axis_mode = GSAP_AXIS_PCA;
     a.axis = (float*)h->axis.p;
     a.partials = (double*)h->partials.p;
-    a.seg_ticket = (uint32_t*)h->ticket.p;
     a.ctrl = (Ctrl*)h->ctrl.p;
     a.hist_obj = (uint32_t*)h->hist_obj.p;
     a.npass_obj = 4;

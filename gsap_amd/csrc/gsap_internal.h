@@ -52,8 +52,7 @@ struct Ctrl {
     uint32_t flags;                  // 1 = capacity overflow, 2 = non-finite input
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
-    uint32_t ticket;                 // last-block ticket (single scene)
-    uint32_t pad[3];
+    uint32_t pad[4];
 };
 
 enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u };
@@ -70,7 +69,6 @@ struct ObjPipeArgs {
     float fixed_axis[3];
     float* axis;            // [nseg][4] device: axis used (w = source)
     double* partials;       // [ntiles][10] moment partials
-    uint32_t* seg_ticket;   // [nseg] last-tile tickets (self-resetting)
     Ctrl* ctrl;
     uint32_t* hist_obj;     // [nseg][npass][256]
     uint32_t npass_obj;
@@ -84,7 +82,7 @@ struct ObjPipeArgs {
     void* grp;              // [n] records grouped by sweep end per 256-object sub-tile
     uint32_t* grp_end;      // [n] sweep end of each grouped record
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
-    double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by the last tile)
+    double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by k_axis)
 };
 
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st);
@@ -101,7 +99,7 @@ struct SweepArgs {
     uint32_t tile;
     uint32_t ntiles;
     const uint32_t* group;   // per original index, or null
-    uint2* out;              // slot buffer (sentinel padded)
+    uint2* out;              // pair buffer (dense, unsorted)
     unsigned long long slot_cap;
     Ctrl* ctrl;
     int kind;
@@ -165,7 +163,7 @@ struct SortArgs {
     int bbits;                          // pairs: bits of the second index
     uint32_t grid;                      // persistent grid size
     const unsigned long long* last_n;   // final pass writes only positions < min(*last_n,
-    unsigned long long last_cap;        //   last_cap) (pairs: drop the sentinel tail)
+    unsigned long long last_cap;        //   last_cap) (pairs: the caller's capacity)
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
@@ -173,7 +171,7 @@ hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1
                      int* which, hipStream_t st);
 // Pairs: passes ping-pong between p0 and p1; the final pass writes `out`.
 hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st);
-constexpr uint32_t kSweepSlotRun = 512;  // pair-buffer slots reserved per sweep atomic
+constexpr uint32_t kSweepStage = 512;  // pairs staged in LDS per sweep wave
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
                            int flip_float, hipStream_t st);

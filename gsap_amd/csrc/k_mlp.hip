@@ -86,10 +86,8 @@ __device__ static SceneStats scene_stats(const double* m, uint32_t n) {
     return S;
 }
 
-__global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sums, float* feats) {
+__global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sums, float*) {
     __shared__ float s_red[kWaves][kFeatSums];
-    __shared__ uint32_t s_last;
-    __shared__ double s_fin[kFeatSums];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x, l = lane_id(), w = tid >> 6;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const uint32_t n = ti.seg_end - ti.seg_start;
@@ -132,28 +130,25 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
         for (int ww = 0; ww < kWaves; ++ww) v += s_red[ww][tid];
         tile_sums[(size_t)tile * kFeatSums + tid] = v;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t nt = a.seg.tile_seg ? (a.seg.seg_tile0[ti.seg + 1] - a.seg.seg_tile0[ti.seg])
-                                           : a.ntiles;
-        s_last = (atomicAdd(&a.seg_ticket[ti.seg], 1u) == nt - 1u);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg] : 0u;
-    const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg + 1] : a.ntiles;
+}
+
+// One wave per scene: the tile sums in tile order (fp64), then the features.
+// A kernel of its own so that no per-tile device-scope fence is needed.
+__global__ __launch_bounds__(64) void k_features_fin(ObjPipeArgs a, const float* tile_sums,
+                                                     float* feats) {
+    __shared__ double s_fin[kFeatSums];
+    const uint32_t seg = blockIdx.x, tid = threadIdx.x;
+    const uint32_t n = a.seg.tile_seg ? a.seg.seg_n[seg] : a.n;
+    const SceneStats S = scene_stats(a.seg_mom + 9 * (size_t)seg, n);
+    const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[seg] : 0u;
+    const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[seg + 1] : a.ntiles;
     if (tid < kFeatSums) {
         double v = 0.0;
-        for (uint32_t t = t0; t < t1; ++t)
-            v += (double)__hip_atomic_load(&tile_sums[(size_t)t * kFeatSums + tid], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t t = t0; t < t1; ++t) v += (double)tile_sums[(size_t)t * kFeatSums + tid];
         s_fin[tid] = v;
     }
     __syncthreads();
     if (tid != 0) return;
-    a.seg_ticket[ti.seg] = 0u;
     const double nn = n ? (double)n : 1.0, s2 = S.s2, e = S.ell;
     float f[kFeat];
     f[0] = (float)(log2(nn) / 16.0);
@@ -192,7 +187,7 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
     f[55] = (float)(s_fin[36] / nn / (e * e));
     f[56] = (float)(s_fin[37] / nn / (e * e));
     f[57] = (float)(s_fin[38] / nn / (e * e));
-    float* out = feats + (size_t)kFeat * ti.seg;
+    float* out = feats + (size_t)kFeat * seg;
     for (int k = 0; k < kFeat; ++k) out[k] = isfinite(f[k]) ? f[k] : 0.0f;
 }
 
@@ -487,6 +482,8 @@ __global__ __launch_bounds__(kWG) void k_axis_tests(const void* in, int kind, co
 // ------------------------------------------------------------------ launchers
 hipError_t launch_features(const ObjPipeArgs& a, float* tile_sums, float* feats, hipStream_t st) {
     hipLaunchKernelGGL(k_features, dim3(a.ntiles), dim3(kWG), 0, st, a, tile_sums, feats);
+    const uint32_t nseg = a.seg.tile_seg ? a.seg.nseg : 1u;
+    hipLaunchKernelGGL(k_features_fin, dim3(nseg), dim3(64), 0, st, a, tile_sums, feats);
     return hipGetLastError();
 }
 hipError_t launch_axis_infer(const float* feats, uint32_t nseg, MlpWeights w, const float* cand,

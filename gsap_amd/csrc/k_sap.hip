@@ -106,7 +106,6 @@ __device__ static void choose_axis(const double m[9], uint32_t n, int mode, cons
 
 __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
     __shared__ double s_red[kWaves][9];
-    __shared__ uint32_t s_last;
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
 
@@ -159,51 +158,44 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
         for (int ww = 0; ww < kWaves; ++ww) v += s_red[ww][tid];
         a.partials[(size_t)tile * 10 + tid] = v;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t nt = a.seg.tile_seg ? (a.seg.seg_tile0[ti.seg + 1] - a.seg.seg_tile0[ti.seg])
-                                           : a.ntiles;
-        const uint32_t t = atomicAdd(&a.seg_ticket[ti.seg], 1u);
-        s_last = (t == nt - 1u);
+}
+
+// One workgroup per scene: the scene's tile partials reduced in a fixed order
+// (per-thread strided sums, then the wave/LDS tree), then the axis.  A kernel
+// of its own rather than a last-tile ticket: the kernel boundary orders the
+// partials, where a ticket needs a device-scope fence per tile (an L2
+// write-back on a multi-XCD part).
+__global__ __launch_bounds__(kWG) void k_axis(ObjPipeArgs a) {
+    __shared__ double s_red[kWaves][9];
+    const uint32_t seg = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[seg] : 0u;
+    const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[seg + 1] : a.ntiles;
+    const uint32_t nseg_obj = a.seg.tile_seg ? a.seg.seg_n[seg] : a.n;
+    double m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = 0.0;
+    for (uint32_t t = t0 + tid; t < t1; t += kWG)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) m[k] += a.partials[(size_t)t * 10 + k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        double v = wave_sum(m[k]);
+        if (l == 0) s_red[w][k] = v;
     }
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    {
-        // Last tile of the scene: reduce every tile's partials in a fixed
-        // order (per-thread strided sums, then the wave/LDS tree).
-        const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg] : 0u;
-        const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg + 1] : a.ntiles;
-        double m[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) m[k] = 0.0;
-        for (uint32_t t = t0 + tid; t < t1; t += kWG)
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                m[k] += __hip_atomic_load(&a.partials[(size_t)t * 10 + k], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
+    if (tid == 0) {
+        double mm[9];
         for (int k = 0; k < 9; ++k) {
-            double v = wave_sum(m[k]);
-            if (l == 0) s_red[w][k] = v;
+            mm[k] = 0.0;
+            for (int ww = 0; ww < kWaves; ++ww) mm[k] += s_red[ww][k];
         }
-        __syncthreads();
-        if (tid == 0) {
-            double mm[9];
-            for (int k = 0; k < 9; ++k) {
-                mm[k] = 0.0;
-                for (int ww = 0; ww < kWaves; ++ww) mm[k] += s_red[ww][k];
-            }
-            float u[3], src;
-            const float* learned = a.learned_axis ? a.learned_axis + 4 * (size_t)ti.seg : nullptr;
-            choose_axis(mm, ti.seg_end - ti.seg_start, a.axis_mode, a.fixed_axis, learned, u, &src);
-            float* ax = a.axis + 4 * (size_t)ti.seg;
-            ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
-            if (a.seg_mom)
-                for (int k = 0; k < 9; ++k) a.seg_mom[9 * (size_t)ti.seg + k] = mm[k];
-            a.seg_ticket[ti.seg] = 0u;
-        }
+        float u[3], src;
+        const float* learned = a.learned_axis ? a.learned_axis + 4 * (size_t)seg : nullptr;
+        choose_axis(mm, nseg_obj, a.axis_mode, a.fixed_axis, learned, u, &src);
+        float* ax = a.axis + 4 * (size_t)seg;
+        ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
+        if (a.seg_mom)
+            for (int k = 0; k < 9; ++k) a.seg_mom[9 * (size_t)seg + k] = mm[k];
     }
 }
 
@@ -298,8 +290,16 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
         }
         __syncthreads();
     }
+    // one atomic per workgroup (the counter is shared by the whole grid)
+    __shared__ unsigned long long s_t[kWaves];
     tests = wave_sum(tests);
-    if (lane_id() == 0 && tests) atomicAdd(&a.ctrl->tests, tests);
+    if (lane_id() == 0) s_t[tid >> 6] = tests;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int v = 0; v < kWaves; ++v) t += s_t[v];
+        if (t) atomicAdd(&a.ctrl->tests, t);
+    }
 }
 
 // ---------------------------------------------------------------- sweep
@@ -312,11 +312,11 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
 // work is the sweep-test count plus under one chunk per object.  Candidates
 // past an object's sweep end can never overlap it (the projected intervals
 // are conservative); only t > s needs a mask, and only near the sub-tile.
-// Hits go straight to global memory: each wave reserves 512-slot runs of the
-// pair buffer with one atomic, writes its hits contiguously (ballot + mbcnt),
-// and pads the unused tail of its last run with all-ones sentinels that the
-// pair sort moves past the real pairs.
-constexpr uint32_t kSlotRun = kSweepSlotRun;
+// Hits are staged per wave in LDS (ballot + mbcnt compaction) and flushed to
+// the pair buffer with one atomic claim of exactly the staged count when the
+// stage fills; the last wave of the workgroup to finish flushes what every
+// wave has left, so the pair buffer holds exactly the pairs, densely.
+constexpr uint32_t kStage = kSweepStage;
 
 template <int KIND, bool GROUP, int MODE>  // MODE 0 canonical, 1 symmetric, 2 count
 __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
@@ -328,130 +328,174 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 #else
     typedef const Rec CRec;
 #endif
+    constexpr uint32_t kSt = MODE == 2 ? 1u : kStage;
+    __shared__ uint2 s_stage[kWaves][kSt];
+    __shared__ uint32_t s_fill[kWaves];
+    __shared__ uint32_t s_np[kWaves];
+    __shared__ unsigned long long s_ex[kWaves];
+    __shared__ uint32_t s_done;
     const Rec* __restrict__ R = reinterpret_cast<const Rec*>(a.rec);
 
     const uint32_t subs = a.tile / kSweepTile;
     const uint32_t tile = blockIdx.x / subs, sub = blockIdx.x % subs;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const uint32_t lo = ti.lo + sub * kSweepTile;
-    if (lo >= ti.hi) return;
+    if (lo >= ti.hi) return;  // the whole workgroup
+    if (threadIdx.x == 0) s_done = 0u;
+    __syncthreads();  // the only block-level barrier
     const uint32_t hi = min(lo + kSweepTile, ti.hi);
     const uint32_t lane = lane_id(), w = uniform(threadIdx.x >> 6);
     const uint32_t g0 = lo + 64u * w;
-    if (g0 >= hi) return;  // no block-level synchronisation below
-    const uint32_t nobj = uniform(min(64u, hi - g0));
-    const uint32_t ge = lane < nobj ? a.grp_end[g0 + lane] : 0u;  // ascending in lane order
-    const uint32_t wend = uniform(wave_max_u32(ge));
-    CRec* __restrict__ O = (CRec*)(a.grp) + g0;
     // directed pairs when symmetric (also in the pre-count)
     const uint32_t mult = (MODE == 1 || (MODE == 2 && a.sym)) ? 2u : 1u;
-    const uint32_t cap = (uint32_t)a.slot_cap;  // < 2^32, checked on the host
+    const unsigned long long cap = a.slot_cap;
+    uint2* stage = s_stage[w];
 
     uint32_t npairs = 0;  // pairs (or directed pairs) found by this wave (< 64 * 2n)
-    uint32_t cur = 0, lim = 0;      // current slot run [cur, lim)
+    uint32_t fill = 0;    // staged pairs of this wave
+    unsigned long long ex = 0;
 
-    uint32_t kfirst = 0;
-    for (uint32_t t0 = lo + 1; t0 < wend; t0 += 64u) {
-        // retire objects whose sweep end is at or before this chunk
-        kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
-        const uint32_t t = t0 + lane;
-        Rec c = R[t < wend ? t : wend - 1u];
-        if (t >= wend) {  // a candidate that overlaps nothing
-            if constexpr (KIND == 0) {
-                c.lo[0] = c.lo[1] = c.lo[2] = __builtin_inff();
-                c.hi[0] = c.hi[1] = c.hi[2] = -__builtin_inff();
-            } else {
-                c.c[0] = __builtin_inff();
+    auto flush = [&](uint32_t cnt) {  // cnt > 0, wave-uniform
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&a.ctrl->slots, (unsigned long long)cnt);
+        base = __builtin_amdgcn_readfirstlane((uint32_t)base) |
+               ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32);
+        return base;
+    };
+
+    if (g0 < hi) {
+        const uint32_t nobj = uniform(min(64u, hi - g0));
+        const uint32_t ge = lane < nobj ? a.grp_end[g0 + lane] : 0u;  // ascending in lane order
+        const uint32_t wend = uniform(wave_max_u32(ge));
+        CRec* __restrict__ O = (CRec*)(a.grp) + g0;
+
+        uint32_t kfirst = 0;
+        for (uint32_t t0 = lo + 1; t0 < wend; t0 += 64u) {
+            // retire objects whose sweep end is at or before this chunk
+            kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
+            const uint32_t t = t0 + lane;
+            Rec c = R[t < wend ? t : wend - 1u];
+            if (t >= wend) {  // a candidate that overlaps nothing
+                if constexpr (KIND == 0) {
+                    c.lo[0] = c.lo[1] = c.lo[2] = __builtin_inff();
+                    c.hi[0] = c.hi[1] = c.hi[2] = -__builtin_inff();
+                } else {
+                    c.c[0] = __builtin_inff();
+                }
+            }
+            uint32_t gc = 0;
+            if (GROUP) gc = a.group[c.orig];
+
+            auto process = [&](const Rec& o, auto masked_tag) {
+                constexpr bool MASKED = decltype(masked_tag)::value;
+                bool hit;
+                if constexpr (KIND == 0) {
+                    // Inclusive AABB test (aabb.h:47-57) on the vector ALU only:
+                    // for finite floats with denormals preserved (the kernel's f32
+                    // denorm mode), fl(x - y) <= 0 exactly when x <= y, so the six
+                    // compares become six subtractions, a max tree and one compare
+                    // -- no scalar-ALU mask arithmetic, which is the scarcer unit
+                    // (one scalar ALU per CU vs four SIMDs).
+                    const float d0 = c.lo[0] - o.hi[0], d1 = o.lo[0] - c.hi[0];
+                    const float d2 = c.lo[1] - o.hi[1], d3 = o.lo[1] - c.hi[1];
+                    const float d4 = c.lo[2] - o.hi[2], d5 = o.lo[2] - c.hi[2];
+                    const float mx = fmaxf(fmaxf(fmaxf(d0, d1), d2), fmaxf(fmaxf(d3, d4), d5));
+                    hit = mx <= 0.0f;
+                } else {
+#pragma clang fp contract(off)
+                    const float x0 = o.c[0] - c.c[0];
+                    const float y0 = o.c[1] - c.c[1];
+                    const float z0 = o.c[2] - c.c[2];
+                    const float r0 = o.c[3] + c.c[3];
+                    const float t1 = (x0 * x0 + y0 * y0) + z0 * z0;
+                    const float e1 = r0 * r0;
+                    hit = t1 < e1;
+                }
+                if (MASKED) hit = hit & (t > o.aux);
+                if (GROUP) hit = hit & (a.group[o.orig] != gc);
+                const unsigned long long m = ballot(hit);
+                if (m == 0ull) return;
+                const uint32_t cnt = (uint32_t)__popcll(m) * mult;
+                npairs += cnt;
+                if constexpr (MODE != 2) {
+                    if (fill + cnt > kStage) {  // stage full: write it out
+                        const unsigned long long base = flush(fill);
+                        for (uint32_t i = lane; i < fill; i += 64u)
+                            if (base + i < cap) a.out[base + i] = stage[i];
+                        fill = 0;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    if (hit) {
+                        const uint32_t r = fill + mult * mbcnt(m);
+                        if (MODE == 0) {
+                            const uint32_t x = o.orig < c.orig ? o.orig : c.orig;
+                            const uint32_t y = o.orig < c.orig ? c.orig : o.orig;
+                            stage[r] = make_uint2(x, y);
+                        } else {
+                            stage[r] = make_uint2(o.orig, c.orig);
+                            stage[r + 1] = make_uint2(c.orig, o.orig);
+                        }
+                    }
+                    fill += cnt;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            };
+            auto objects = [&](auto masked_tag) {
+                uint32_t k = kfirst;
+                for (; k + 4 <= nobj; k += 4) {  // four scalar loads in flight
+                    const Rec o0 = O[k], o1 = O[k + 1], o2 = O[k + 2], o3 = O[k + 3];
+                    process(o0, masked_tag);
+                    process(o1, masked_tag);
+                    process(o2, masked_tag);
+                    process(o3, masked_tag);
+                }
+                for (; k < nobj; ++k) process(O[k], masked_tag);
+            };
+            if (t0 < hi) objects(std::true_type{});  // an object may sit at or after t
+            else objects(std::false_type{});
+        }
+        // candidate tests executed: every active object takes whole 64-lane chunks
+        if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 63u) / 64u) * 64u;
+        ex = wave_sum(ex);
+    }
+
+    // Hand this wave's remainder and counters to the workgroup; the last wave
+    // to arrive writes them all out with one atomic per counter.
+    if (lane == 0) {
+        s_fill[w] = fill;
+        s_np[w] = npairs;
+        s_ex[w] = ex;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    uint32_t old = 0;
+    if (lane == 0) old = atomicAdd(&s_done, 1u);
+    old = __builtin_amdgcn_readfirstlane(old);
+    if (old != kWaves - 1u) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    unsigned long long np = 0, ext = 0;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+        np += s_np[v];
+        ext += s_ex[v];
+        tot += s_fill[v];
+    }
+    if (lane == 0) {
+        if (np) atomicAdd(&a.ctrl->num_pairs, np);
+        if (ext) atomicAdd(&a.ctrl->exec_tests, ext);
+    }
+    if constexpr (MODE != 2) {
+        if (tot) {
+            unsigned long long base = flush(tot);
+#pragma unroll
+            for (int v = 0; v < kWaves; ++v) {
+                const uint32_t f = s_fill[v];
+                for (uint32_t i = lane; i < f; i += 64u)
+                    if (base + i < cap) a.out[base + i] = s_stage[v][i];
+                base += f;
             }
         }
-        uint32_t gc = 0;
-        if (GROUP) gc = a.group[c.orig];
-
-        auto process = [&](const Rec& o, auto masked_tag) {
-            constexpr bool MASKED = decltype(masked_tag)::value;
-            bool hit;
-            if constexpr (KIND == 0) {
-                // Inclusive AABB test (aabb.h:47-57) on the vector ALU only:
-                // for finite floats with denormals preserved (the kernel's f32
-                // denorm mode), fl(x - y) <= 0 exactly when x <= y, so the six
-                // compares become six subtractions, a max tree and one compare
-                // -- no scalar-ALU mask arithmetic, which is the scarcer unit
-                // (one scalar ALU per CU vs four SIMDs).
-                const float d0 = c.lo[0] - o.hi[0], d1 = o.lo[0] - c.hi[0];
-                const float d2 = c.lo[1] - o.hi[1], d3 = o.lo[1] - c.hi[1];
-                const float d4 = c.lo[2] - o.hi[2], d5 = o.lo[2] - c.hi[2];
-                const float mx = fmaxf(fmaxf(fmaxf(d0, d1), d2), fmaxf(fmaxf(d3, d4), d5));
-                hit = mx <= 0.0f;
-            } else {
-#pragma clang fp contract(off)
-                const float x0 = o.c[0] - c.c[0];
-                const float y0 = o.c[1] - c.c[1];
-                const float z0 = o.c[2] - c.c[2];
-                const float r0 = o.c[3] + c.c[3];
-                const float t1 = (x0 * x0 + y0 * y0) + z0 * z0;
-                const float e1 = r0 * r0;
-                hit = t1 < e1;
-            }
-            if (MASKED) hit = hit & (t > o.aux);
-            if (GROUP) hit = hit & (a.group[o.orig] != gc);
-            const unsigned long long m = ballot(hit);
-            if (m == 0ull) return;
-            const uint32_t cnt = (uint32_t)__popcll(m) * mult;
-            npairs += cnt;
-            if constexpr (MODE != 2) {
-                const uint32_t r = mult * mbcnt(m);
-                const uint32_t rem = lim - cur;
-                uint32_t nb = 0;
-                const bool fresh = cnt > rem;
-                if (fresh) {  // reserve the next run (one atomic per wave)
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(&a.ctrl->slots, (unsigned long long)kSlotRun);
-                    nb = uniform((uint32_t)base);
-                }
-                if (hit) {
-                    const uint32_t p0 = r < rem ? cur + r : nb + (r - rem);
-                    if (MODE == 0) {
-                        const uint32_t x = o.orig < c.orig ? o.orig : c.orig;
-                        const uint32_t y = o.orig < c.orig ? c.orig : o.orig;
-                        if (p0 < cap) a.out[p0] = make_uint2(x, y);
-                    } else {
-                        const uint32_t p1 = r + 1 < rem ? cur + r + 1 : nb + (r + 1 - rem);
-                        if (p0 < cap) a.out[p0] = make_uint2(o.orig, c.orig);
-                        if (p1 < cap) a.out[p1] = make_uint2(c.orig, o.orig);
-                    }
-                }
-                if (fresh) {
-                    cur = nb + (cnt - rem);
-                    lim = nb + kSlotRun;
-                } else {
-                    cur += cnt;
-                }
-            }
-        };
-        auto objects = [&](auto masked_tag) {
-            uint32_t k = kfirst;
-            for (; k + 4 <= nobj; k += 4) {  // four scalar loads in flight
-                const Rec o0 = O[k], o1 = O[k + 1], o2 = O[k + 2], o3 = O[k + 3];
-                process(o0, masked_tag);
-                process(o1, masked_tag);
-                process(o2, masked_tag);
-                process(o3, masked_tag);
-            }
-            for (; k < nobj; ++k) process(O[k], masked_tag);
-        };
-        if (t0 < hi) objects(std::true_type{});  // an object may sit at or after t
-        else objects(std::false_type{});
     }
-    if (lane == 0 && npairs) atomicAdd(&a.ctrl->num_pairs, (unsigned long long)npairs);
-    if constexpr (MODE != 2) {
-        for (uint32_t p = cur + lane; p < lim; p += 64)
-            if (p < cap) a.out[p] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-    }
-    // candidate tests executed: every active object takes whole 64-lane chunks
-    unsigned long long ex = 0;
-    if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 63u) / 64u) * 64u;
-    ex = wave_sum(ex);
-    if (lane == 0 && ex) atomicAdd(&a.ctrl->exec_tests, ex);
 }
 
 // ---------------------------------------------------------------- CSR entries
@@ -549,6 +593,8 @@ __global__ void k_scene_offsets(const uint2* __restrict__ P, const Ctrl* ctrl, u
 // ---------------------------------------------------------------- launchers
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_moments, dim3(a.ntiles), dim3(kWG), 0, st, a);
+    const uint32_t nseg = a.seg.tile_seg ? a.seg.nseg : 1u;
+    hipLaunchKernelGGL(k_axis, dim3(nseg), dim3(kWG), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_project(const ObjPipeArgs& a, hipStream_t st) {

@@ -67,8 +67,7 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     const uint32_t tag = ((a.ctrl->epoch << 4) | ctr_idx) & 0x7FFFFFFFu;
     const unsigned long long ltm = lanemask_lt();
     const unsigned long long gtm = ~ltm & ~(1ull << lane);
-    // The final pass of a pair sort writes only the real pairs (the sentinel
-    // tail sorts last) and never past the caller's capacity.
+    // The final pass of a pair sort never writes past the caller's capacity.
     unsigned long long out_lim = ~0ull;
     if (a.last_n && (uint32_t)pass + 1u == a.npass) {
         const unsigned long long ln = *a.last_n;
@@ -181,7 +180,10 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     }
 }
 
-// Histogram of every digit position (plain sorts and the pair sort).
+// Histogram of every digit position (plain sorts and the pair sort).  Lanes
+// hold consecutive keys; equal digits in neighbouring lanes are counted as one
+// run (one LDS atomic per run), so the high digits of nearly sorted or narrow
+// keys -- the same value across a whole wave -- do not serialise on one bin.
 template <class KT, int BITS>
 __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__ keys, uint32_t n_host,
                                               const unsigned long long* __restrict__ n_dev,
@@ -196,12 +198,27 @@ __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__
         unsigned long long nd = *n_dev;
         n = (uint32_t)(nd > cap ? cap : nd);
     }
-    for (uint32_t i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-        typename KT::T k = keys[i];
+    const uint32_t lane = lane_id();
+    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
+    for (uint32_t b0 = blockIdx.x * kWG + (threadIdx.x & ~63u); b0 < n; b0 += gridDim.x * kWG) {
+        const uint32_t i = b0 + lane;
+        const bool valid = i < n;
+        typename KT::T k{};
+        if (valid) k = keys[i];
         if constexpr (KT::kHasVals) {
             if (flip_float) k = flip_f32(__uint_as_float(k));
         }
-        for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * RADIX + KT::digit(k, p * BITS, bbits, MASK)], 1u);
+        const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+        for (int p = 0; p < npass; ++p) {
+            const uint32_t d = KT::digit(k, p * BITS, bbits, MASK);
+            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+            const bool head = valid && (lane == 0u || prev != d);
+            const unsigned long long nxt = ballot(head) & above;
+            if (head) {
+                const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+                atomicAdd(&s_h[p * RADIX + d], end - lane);
+            }
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < (uint32_t)npass * RADIX; i += kWG)

@@ -269,15 +269,17 @@ class PrbState:
         r = self.ppos[:, 3:4]
         return np.concatenate([self.ppos[:, :3] - r, self.ppos[:, :3] + r], axis=1)
 
-    def step(self, dt, params, broadphase="sphere"):
-        """DemoParticles::step order (PB/cudaPhysics/demoParticles.h:856-928)."""
+    def step(self, dt, params, broadphase="sphere", sample_stride=1):
+        """DemoParticles::step order (PB/cudaPhysics/demoParticles.h:856-928).
+        sample_stride > 1 sweeps only every k-th sorted object (a timing
+        sample: the pair list and so the forces are then incomplete)."""
         self.integrate(dt, params)
         self.update_particles()
         grp = self.pbody.astype(np.uint32) if params.flags & PRB_SKIP_SAME_BODY else None
         if broadphase == "sphere":
-            canon = brute_or_sap(self.ppos, KIND_SPHERE, grp)
+            canon = brute_or_sap(self.ppos, KIND_SPHERE, grp, sample_stride)
         else:
-            canon = brute_or_sap(self.particle_aabbs(), KIND_AABB, grp)
+            canon = brute_or_sap(self.particle_aabbs(), KIND_AABB, grp, sample_stride)
         sym, _ = sym_csr(canon)
         self.prepare()
         self.forces(sym, dt, params)
@@ -285,6 +287,8 @@ class PrbState:
         return canon
 
 
-def brute_or_sap(boxes, kind, group):
-    pairs, _, _ = sap(boxes, kind=kind, axis_mode=AXIS_COORD_MAXVAR, group=group)
+def brute_or_sap(boxes, kind, group, sample_stride=1):
+    cap = None if sample_stride == 1 else max(16, 64 * len(boxes) // sample_stride)
+    pairs, _, _ = sap(boxes, kind=kind, axis_mode=AXIS_COORD_MAXVAR, group=group, cap=cap,
+                      sample_stride=sample_stride)
     return pairs
