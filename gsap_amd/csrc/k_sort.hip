@@ -6,10 +6,11 @@
 // scatter (4 launches + scan per digit).  Here one histogram of every digit
 // position is taken up front (fused into the projection kernel for the object
 // sort), then each digit is ONE kernel:
-//   tile load (striped, coalesced) -> wave64 ballot match ranking (stable)
-//   -> per-tile digit counts published to a decoupled look-back array ->
-//   look-back for the exclusive prefix -> keys staged in LDS in digit order
-//   -> coalesced scatter.
+//   tile load (striped, coalesced) -> per-tile digit counts (LDS atomics,
+//   equal neighbours aggregated) published at once to a decoupled look-back
+//   array -> wave64 ballot match ranking (stable), while predecessors publish
+//   -> look-back for the exclusive prefix, four predecessors per load round
+//   -> keys staged in LDS in digit order -> coalesced scatter.
 // Tiles are claimed through an atomic counter so a workgroup only ever waits
 // on tiles that were claimed before it (no deadlock with a persistent grid).
 // Segments (batched scenes) restart the look-back chain at their first tile.
@@ -51,6 +52,7 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     __shared__ uint32_t s_tstart[RADIX];
     __shared__ uint32_t s_gbase[RADIX];
     __shared__ uint32_t s_tmp[2 * kWaves];
+    __shared__ uint32_t s_cnt[RADIX];
     __shared__ K s_k[TILE];
     __shared__ uint32_t s_v[KT::kHasVals ? TILE : 1];
     __shared__ uint32_t s_tile;
@@ -67,11 +69,15 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     const uint32_t tag = ((a.ctrl->epoch << 4) | ctr_idx) & 0x7FFFFFFFu;
     const unsigned long long ltm = lanemask_lt();
     const unsigned long long gtm = ~ltm & ~(1ull << lane);
-    // The final pass of a pair sort never writes past the caller's capacity.
-    unsigned long long out_lim = ~0ull;
+    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
+    for (uint32_t i = tid; i < RADIX; i += kWG) s_cnt[i] = 0u;
+    // No pass writes past the sorted range; the final pass of a pair sort
+    // also stops at the caller's capacity.
+    unsigned long long out_lim = n;
     if (a.last_n && (uint32_t)pass + 1u == a.npass) {
         const unsigned long long ln = *a.last_n;
-        out_lim = ln < a.last_cap ? ln : a.last_cap;
+        out_lim = ln < out_lim ? ln : out_lim;
+        out_lim = a.last_cap < out_lim ? a.last_cap : out_lim;
     }
 
     for (;;) {
@@ -94,13 +100,39 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
                 if (KT::kHasVals) val[k] = vin[idx];
             }
         }
-        // Stable ranking: items in tile order (wave, item, lane); peers with
-        // the same digit found by BITS ballots.
+        // Tile digit counts first, so that the tile's aggregate is published
+        // before the (much longer) stable ranking.  Valid lanes are a prefix
+        // of each wave; a run of equal digits in neighbouring lanes is one
+        // LDS atomic.
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const bool valid = (base + 64u * k) < ti.hi;
             const uint32_t d = valid ? KT::digit(key[k], shift, a.bbits, MASK) : 0u;
             dig[k] = d;
+            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+            const bool head = valid && (lane == 0u || prev != d);
+            // both ballots in converged control flow (all lanes active)
+            const unsigned long long hm = ballot(head);
+            const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+            if (head) {
+                const unsigned long long nxt = hm & above;
+                const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+                atomicAdd(&s_cnt[d], end - lane);
+            }
+        }
+        __syncthreads();
+        uint32_t tcount = 0;
+        if (tid < RADIX) {
+            tcount = s_cnt[tid];
+            s_cnt[tid] = 0u;
+            st_rel(a.lookback + (size_t)tile * RADIX + tid, lb_word(tag, first, tcount));
+        }
+        // Stable ranking: items in tile order (wave, item, lane); peers with
+        // the same digit found by BITS ballots.
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const bool valid = (base + 64u * k) < ti.hi;
+            const uint32_t d = dig[k];
             unsigned long long m = ballot(valid);
 #pragma unroll
             for (int b = 0; b < BITS; ++b) {
@@ -114,7 +146,7 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
         }
         __syncthreads();
 
-        uint32_t tcount = 0, excl = 0;
+        uint32_t excl = 0;
         if (tid < RADIX) {
             uint32_t run = 0;
 #pragma unroll
@@ -123,13 +155,32 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
                 s_wh[w][tid] = run;
                 run += c;
             }
-            tcount = run;
-            st_rel(a.lookback + (size_t)tile * RADIX + tid, lb_word(tag, first, tcount));
             if (!first) {
-                uint32_t p = tile - 1u, spins = 0;
+                // Predecessors' words four at a time: sum the ready ones up to
+                // the nearest inclusive prefix; stop at a not-yet-published one
+                // and reload from there.  The scene's first tile is inclusive,
+                // so the walk never leaves the scene.
+                constexpr int WIN = 4;
+                int p = (int)tile - 1;
+                uint32_t spins = 0;
                 for (;;) {
-                    const unsigned long long v = ld_acq(a.lookback + (size_t)p * RADIX + tid);
-                    if ((uint32_t)(v >> 33) != tag) {
+                    unsigned long long v[WIN];
+#pragma unroll
+                    for (int k = 0; k < WIN; ++k)
+                        v[k] = p - k >= 0 ? ld_acq(a.lookback + (size_t)(p - k) * RADIX + tid) : 0ull;
+                    uint32_t add = 0;
+                    int used = 0;
+                    bool done = false;
+#pragma unroll
+                    for (int k = 0; k < WIN; ++k) {
+                        if (done || used != k || (uint32_t)(v[k] >> 33) != tag) continue;
+                        add += (uint32_t)v[k];
+                        used = k + 1;
+                        done = (v[k] >> 32) & 1ull;
+                    }
+                    excl += add;
+                    if (done) break;
+                    if (used == 0) {
                         // Bounded wait: a predecessor publishes within microseconds;
                         // if it never does, flag the update and let the kernel end.
                         if (++spins > (1u << 22)) {
@@ -137,11 +188,8 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
-                        continue;
                     }
-                    excl += (uint32_t)v;
-                    if (v & (1ull << 32)) break;
-                    --p;
+                    p -= used;
                 }
                 st_rel(a.lookback + (size_t)tile * RADIX + tid, lb_word(tag, true, excl + tcount));
             }
