@@ -131,11 +131,12 @@ def bench_particles(G, W, torch, dev, nb, steps, with_cpu):
     prb.set_particle_shape_inv_inertia(sc["shape_inv_inertia"])
     prb.set_physics(broadphase_kind=G.INPUT_AABB, **kw)
     sap = G.Sap(max_n=npart, input_kind=G.INPUT_AABB, output_kind=G.OUT_SYM_CSR,
-                flags=G.FLAG_SKIP_SAME_GROUP)
+                flags=G.FLAG_SKIP_SAME_GROUP | G.FLAG_SLABS)
     dt = 1e-4
     ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
     p = prb.ptrs()  # raises on pair-buffer overflow
-    out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase",
+    out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase "
+                       "(slab sweep)",
            "particles": npart, "steps": steps, "ms_per_step": round(ms, 4),
            "steps_per_s": round(1e3 / ms, 1), "particle_updates_per_s": round(npart * 1e3 / ms, 1),
            "directed_pairs_last_step": int(p.num_pairs)}
@@ -170,6 +171,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--axis", default="pca", choices=["pca", "coord", "learned"])
+    ap.add_argument("--no-slabs", action="store_true",
+                    help="plain sweep along one axis (no slab split on a second axis)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--no-extras", action="store_true",
@@ -195,7 +198,8 @@ def main():
     n = len(boxes)
     bd = torch.from_numpy(boxes).to(dev)
     mode = {"pca": G.AXIS_PCA, "coord": G.AXIS_COORD_MAXVAR, "learned": G.AXIS_LEARNED}[args.axis]
-    sap = G.Sap(max_n=n, axis_mode=mode)
+    slabs = not args.no_slabs
+    sap = G.Sap(max_n=n, axis_mode=mode, flags=G.FLAG_SLABS if slabs else 0)
     P, st0 = sap.count(bd)  # pre-count pass sizes the 64-bit pair buffer
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     sap.set_timing(True)
@@ -271,6 +275,7 @@ def main():
         "config": {"workload": f"aniso-{n}: AABBs, centres R diag(20,5,1) z, half-extents "
                                "log-U[0.05,0.8], seed 1, canonical uint32 pair list",
                    "n": n, "pairs": P, "axis": args.axis,
+                   "sweep": "slabs" if slabs else "plain",
                    "sweep_tests": int(st.sweep_tests), "axis_used": [round(x, 6) for x in st.axis]},
         "roofline": roof,
         "sort_roofline": sort_roof,
@@ -278,6 +283,17 @@ def main():
         "wall_s": round(wall, 3),
         "cpu_baseline": cpu,
     }
+    if not args.no_extras and slabs:
+        plain = G.Sap(max_n=n, axis_mode=mode)
+        plain.set_timing(True)
+        ms_plain = _time_ms(torch, lambda: plain.collide_into(bd, pairs), max(1, args.steps // 2), 1)
+        _, _, stp = plain.collide_into(bd, pairs)
+        out["plain_sweep"] = {"ms_per_step": round(ms_plain, 4),
+                              "updates_per_s": round(n / (ms_plain / 1e3), 1),
+                              "sweep_tests": int(stp.sweep_tests),
+                              "stages_ms": {k: round(v, 4) for k, v in
+                                            zip(names, plain.stage_times())}}
+        plain.close()
     if not args.no_extras:
         out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup)
         out["learned_axis"] = bench_learned(G, W, torch, dev, args.learned_scenes)

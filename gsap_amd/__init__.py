@@ -23,6 +23,7 @@ INPUT_AABB, INPUT_SPHERE = 0, 1
 AXIS_PCA, AXIS_COORD_MAXVAR, AXIS_FIXED, AXIS_LEARNED = 0, 1, 2, 3
 OUT_CANON, OUT_SYM_CSR = 0, 1
 FLAG_SKIP_SAME_GROUP = 1
+FLAG_SLABS = 2
 PRB_ABS_SPRING, PRB_WALLS, PRB_FLOOR_CLAMP, PRB_CAP_VEL, PRB_SKIP_SAME_BODY, PRB_INERTIA = (
     1, 2, 4, 8, 16, 32)
 

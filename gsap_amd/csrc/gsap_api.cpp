@@ -69,6 +69,7 @@ struct gsap_sap {
     Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
+    Buf axis2, vr_partials, slabp;  // slab mode
     bool has_model = false;
     float conf_min = 0.95f;
     unsigned long long* h_counts = nullptr;  // pinned
@@ -162,7 +163,8 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
                    &h->ctrl, &h->counts, &h->ptmp[0], &h->ptmp[1], &h->rowstart, &h->has,
                    &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
                    &h->tile_seg, &h->grp, &h->grp_end, &h->feats, &h->tile_sums, &h->seg_mom,
-                   &h->cand, &h->mlp_wb, &h->mlp_master, &h->ax_tests_axes};
+                   &h->cand, &h->mlp_wb, &h->mlp_master, &h->ax_tests_axes, &h->axis2,
+                   &h->vr_partials, &h->slabp};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
     for (int i = 0; i < 8; ++i)
@@ -173,17 +175,21 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
     return GSAP_OK;
 }
 
-// Object workspace for n objects, nseg scenes, ntiles tiles.
+static bool slab_mode(const gsap_sap* h) { return (h->desc.flags & GSAP_FLAG_SLABS) != 0; }
+
+// Object workspace for n objects, nseg scenes, ntiles tiles.  In slab mode
+// the sort, gather and sweep buffers hold two entries per object.
 static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntiles) {
+    const size_t em = slab_mode(h) ? 2 : 1;
     if (n > h->cap_n) {
         uint32_t want = std::max(n, h->desc.max_n);
         for (int i = 0; i < 2; ++i) {
-            HIPCHK(h, grow(h->keys[i], (size_t)want * 4));
-            HIPCHK(h, grow(h->vals[i], (size_t)want * 4));
+            HIPCHK(h, grow(h->keys[i], (size_t)want * 4 * em));
+            HIPCHK(h, grow(h->vals[i], (size_t)want * 4 * em));
         }
-        HIPCHK(h, grow(h->rec, (size_t)want * 32));
-        HIPCHK(h, grow(h->grp, (size_t)want * 32));
-        HIPCHK(h, grow(h->grp_end, (size_t)want * 4));
+        HIPCHK(h, grow(h->rec, (size_t)want * 32 * em));
+        HIPCHK(h, grow(h->grp, (size_t)want * 32 * em));
+        HIPCHK(h, grow(h->grp_end, (size_t)want * 4 * em));
         HIPCHK(h, grow(h->rowstart, (size_t)want * 4));
         HIPCHK(h, grow(h->has, (size_t)want * 4));
         HIPCHK(h, hipMemset(h->has.p, 0, (size_t)want * 4));
@@ -192,8 +198,9 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     }
     if (ntiles > h->cap_tiles) {
         HIPCHK(h, grow(h->partials, (size_t)ntiles * 10 * sizeof(double)));
-        HIPCHK(h, grow(h->lb_obj, (size_t)ntiles * 256 * 8));
-        HIPCHK(h, hipMemset(h->lb_obj.p, 0, (size_t)ntiles * 256 * 8));
+        HIPCHK(h, grow(h->lb_obj, ((size_t)ntiles * em + 1) * 256 * 8));
+        HIPCHK(h, hipMemset(h->lb_obj.p, 0, ((size_t)ntiles * em + 1) * 256 * 8));
+        if (slab_mode(h)) HIPCHK(h, grow(h->vr_partials, (size_t)ntiles * kVr * 4));
         HIPCHK(h, grow(h->tile_seg, (size_t)ntiles * 4));
         HIPCHK(h, grow(h->tile_sums, (size_t)ntiles * kFeatSums * 4));
         h->cap_tiles = ntiles;
@@ -209,6 +216,10 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         h->cap_seg = nseg;
     }
     if (!h->hist_pair.p) HIPCHK(h, grow(h->hist_pair, kMaxPasses * 256 * 4));
+    if (slab_mode(h) && !h->slabp.p) {
+        HIPCHK(h, grow(h->axis2, 4 * sizeof(float)));
+        HIPCHK(h, grow(h->slabp, kSlabParams * sizeof(float)));
+    }
     return GSAP_OK;
 }
 
@@ -276,6 +287,14 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.grp_end = (uint32_t*)h->grp_end.p;
     a.learned_axis = nullptr;
     a.seg_mom = (double*)h->seg_mom.p;
+    // slab mode: one scene only (the batched entry point refuses the flag)
+    const bool slabs = slab_mode(h) && seg.tile_seg == nullptr;
+    a.slabs = slabs ? 1 : 0;
+    a.axis2 = (float*)h->axis2.p;
+    a.vr_partials = (float*)h->vr_partials.p;
+    a.slab = (float*)h->slabp.p;
+    const uint32_t n_e = slabs ? 2 * n : n;  // sorted entries
+    const uint32_t ntiles_e = slabs ? (n_e + tile - 1) / tile : ntiles;
     const bool learned = h->desc.axis_mode == GSAP_AXIS_LEARNED;
     if (learned) {
         if (!h->has_model)
@@ -296,13 +315,14 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         HIPCHK(h, launch_axis_infer((const float*)h->feats.p, seg.nseg, mw, (const float*)h->cand.p,
                                     h->conf_min, a.axis, st));
     }
+    if (slabs) HIPCHK(h, launch_slab_params(a, st));
     HIPCHK(h, launch_project(a, st));
     mark(h, st);
 
     SortArgs s{};
-    s.n = n;
+    s.n = n_e;
     s.seg = seg;
-    s.ntiles = ntiles;
+    s.ntiles = ntiles_e;
     s.hist = a.hist_obj;
     s.npass = 4;
     s.lookback = (unsigned long long*)h->lb_obj.p;
@@ -314,17 +334,21 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     int which = 0;
     HIPCHK(h, sort_kv32(s, a.keys, a.vals, a.keys2, a.vals2, &which, st));
     mark(h, st);
-    HIPCHK(h, launch_gather(a, which ? a.keys2 : a.keys, which ? a.vals2 : a.vals, st));
+    ObjPipeArgs ae = a;  // entry space
+    ae.n = n_e;
+    ae.ntiles = ntiles_e;
+    HIPCHK(h, launch_gather(ae, which ? a.keys2 : a.keys, which ? a.vals2 : a.vals, st));
     mark(h, st);
 
     SweepArgs w{};
     w.rec = h->rec.p;
     w.grp = h->grp.p;
     w.grp_end = (const uint32_t*)h->grp_end.p;
-    w.n = n;
+    w.n = n_e;
     w.seg = seg;
     w.tile = tile;
-    w.ntiles = ntiles;
+    w.ntiles = ntiles_e;
+    w.slabs = slabs ? 1 : 0;
     w.group = (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP) ? group : nullptr;
     w.ctrl = ctrl;
     w.kind = kind;
@@ -373,6 +397,8 @@ static int32_t check_common(gsap_sap* h, const void* in, uint32_t n, const uint3
     if (n > 0x7FFFFFFFu) return fail(h, GSAP_ERR_INVALID_ARG, "n exceeds 2^31-1");
     if ((h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP) && n > 0 && !group)
         return fail(h, GSAP_ERR_INVALID_ARG, "GSAP_FLAG_SKIP_SAME_GROUP needs d_group");
+    if (slab_mode(h) && n > (1u << 30))
+        return fail(h, GSAP_ERR_UNSUPPORTED, "GSAP_FLAG_SLABS supports n <= 2^30");
     return GSAP_OK;
 }
 
@@ -433,6 +459,7 @@ static int32_t read_counts(gsap_sap* h, hipStream_t st, uint64_t cap, bool count
         if (h->timing && h->nev >= 2) (void)hipEventElapsedTime(&stats->ms_total, h->ev[0], h->ev[h->nev - 1]);
     }
     if (c[3] & kFlagTimeout) return fail(h, GSAP_ERR_HIP, "internal: sort look-back wait expired");
+    if (c[3] & kFlagSlab) return fail(h, GSAP_ERR_HIP, "internal: a box spans more than two slabs");
     if (c[3] & kFlagNonFinite) return fail(h, GSAP_ERR_NONFINITE, "non-finite value in input boxes");
     if (!count_only && c[0] > cap) {
         char buf[160];
@@ -501,6 +528,7 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
         return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode emits canonical pairs only");
     if (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP)
         return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no group filter");
+    if (slab_mode(h)) return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no slab split");
     const uint32_t n = h_scene_offsets[n_scenes];
     if (h_scene_offsets[0] != 0) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets must start at 0");
     int32_t rc = check_common(h, d_in, n, nullptr);

@@ -55,7 +55,7 @@ struct Ctrl {
     uint32_t pad[4];
 };
 
-enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u };
+enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u };
 
 // ---- launch wrappers (k_sap.hip) ----
 struct ObjPipeArgs {
@@ -83,9 +83,20 @@ struct ObjPipeArgs {
     uint32_t* grp_end;      // [n] sweep end of each grouped record
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
     double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by k_axis)
+    // slab mode (GSAP_FLAG_SLABS, one scene): objects i -> entries 2i, 2i+1
+    int slabs;
+    float* axis2;           // [4] slab axis v (largest spread orthogonal to the sweep axis)
+    float* vr_partials;     // [ntiles][kVr] per-tile ranges for the slab parameters
+    float* slab;            // [kSlabParams] slab and key-quantisation parameters
 };
+constexpr int kVr = 5;           // max v extent, min lo_v, max hi_v, min lo_u, max hi_u
+constexpr int kSlabParams = 8;   // v0, 1/W, S, u0, 1/du, W, -, -
+constexpr uint32_t kSlabNull = 0xFFFFFFFFu;  // key of an unused second entry
+constexpr uint32_t kFirstBit = 0x80000000u;  // entry is its box's first slab
 
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st);
+// slab mode: per-tile projection ranges and the slab parameters (one scene)
+hipError_t launch_slab_params(const ObjPipeArgs& a, hipStream_t st);
 hipError_t launch_project(const ObjPipeArgs& a, hipStream_t st);
 hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* sorted_keys,
                          const uint32_t* perm, hipStream_t st);
@@ -105,6 +116,7 @@ struct SweepArgs {
     int kind;
     int sym;                 // emit both orientations
     int count_only;
+    int slabs;               // entries carry kFirstBit in orig and their slab's first position in aux
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 

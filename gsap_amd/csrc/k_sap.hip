@@ -160,6 +160,51 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
     }
 }
 
+// Slab axis: the direction of largest spread orthogonal to the sweep axis u,
+// the top eigenvector of P C P with P = I - u u^T (u is in its null space).
+__device__ static void slab_axis(const double m[9], uint32_t n, const float uf[3], float v[3]) {
+    const double inv = n ? 1.0 / (double)n : 0.0;
+    const double mu[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+    double C[3][3];
+    C[0][0] = m[3] * inv - mu[0] * mu[0];
+    C[0][1] = C[1][0] = m[4] * inv - mu[0] * mu[1];
+    C[0][2] = C[2][0] = m[5] * inv - mu[0] * mu[2];
+    C[1][1] = m[6] * inv - mu[1] * mu[1];
+    C[1][2] = C[2][1] = m[7] * inv - mu[1] * mu[2];
+    C[2][2] = m[8] * inv - mu[2] * mu[2];
+    const double u[3] = {uf[0], uf[1], uf[2]};
+    double P[3][3], PC[3][3], A[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = (i == j ? 1.0 : 0.0) - u[i] * u[j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) PC[i][j] = P[i][0] * C[0][j] + P[i][1] * C[1][j] + P[i][2] * C[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = PC[i][0] * P[0][j] + PC[i][1] * P[1][j] + PC[i][2] * P[2][j];
+    double V[3][3], ev[3];
+    jacobi3(A, V, ev);
+    int best = 0;
+    for (int k = 1; k < 3; ++k)
+        if (ev[k] > ev[best]) best = k;
+    double w[3] = {V[0][best], V[1][best], V[2][best]};
+    // remove any component along u left by rounding
+    const double d = w[0] * u[0] + w[1] * u[1] + w[2] * u[2];
+    for (int k = 0; k < 3; ++k) w[k] -= d * u[k];
+    double nn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (!(nn > 1e-6)) {  // no spread across u: any unit vector orthogonal to u
+        int k = 0;
+        for (int j = 1; j < 3; ++j)
+            if (fabs(u[j]) < fabs(u[k])) k = j;
+        double e[3] = {0.0, 0.0, 0.0};
+        e[k] = 1.0;
+        w[0] = u[1] * e[2] - u[2] * e[1];
+        w[1] = u[2] * e[0] - u[0] * e[2];
+        w[2] = u[0] * e[1] - u[1] * e[0];
+        nn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (!(nn > 0)) { w[0] = 0; w[1] = 1; w[2] = 0; nn = 1; }
+    }
+    for (int k = 0; k < 3; ++k) v[k] = (float)(w[k] / nn);
+}
+
 // One workgroup per scene: the scene's tile partials reduced in a fixed order
 // (per-thread strided sums, then the wave/LDS tree), then the axis.  A kernel
 // of its own rather than a last-tile ticket: the kernel boundary orders the
@@ -196,7 +241,127 @@ __global__ __launch_bounds__(kWG) void k_axis(ObjPipeArgs a) {
         ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
         if (a.seg_mom)
             for (int k = 0; k < 9; ++k) a.seg_mom[9 * (size_t)seg + k] = mm[k];
+        if (a.slabs) {
+            float v[3];
+            slab_axis(mm, nseg_obj, u, v);
+            a.axis2[0] = v[0]; a.axis2[1] = v[1]; a.axis2[2] = v[2]; a.axis2[3] = 0.0f;
+        }
     }
+}
+
+// ---------------------------------------------------------------- slabs
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Per tile: the largest slab-axis extent and the ranges of both projections.
+__global__ __launch_bounds__(kWG) void k_vrange(ObjPipeArgs a) {
+    __shared__ float s_r[kWaves][kVr];
+    const uint32_t tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
+    const float u[3] = {a.axis[0], a.axis[1], a.axis[2]};
+    const float v[3] = {a.axis2[0], a.axis2[1], a.axis2[2]};
+    const bool exact = axis_exact(a.kind, u);
+    float r[kVr] = {0.0f, __builtin_inff(), -__builtin_inff(), __builtin_inff(), -__builtin_inff()};
+    for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
+        float lo[3], hi[3], c4[4], pl, ph, ql, qh;
+        load_lohi(a.in, a.kind, i, lo, hi, c4);
+        project_lohi(a.kind, lo, hi, c4, u, exact, &pl, &ph);
+        project_lohi(a.kind, lo, hi, c4, v, false, &ql, &qh);
+        r[0] = fmaxf(r[0], qh - ql);
+        r[1] = fminf(r[1], ql);
+        r[2] = fmaxf(r[2], qh);
+        r[3] = fminf(r[3], pl);
+        r[4] = fmaxf(r[4], ph);
+    }
+    r[0] = wave_maxf(r[0]);
+    r[1] = wave_minf(r[1]);
+    r[2] = wave_maxf(r[2]);
+    r[3] = wave_minf(r[3]);
+    r[4] = wave_maxf(r[4]);
+    if (l == 0)
+        for (int k = 0; k < kVr; ++k) s_r[w][k] = r[k];
+    __syncthreads();
+    if (tid == 0) {
+        for (int ww = 1; ww < kWaves; ++ww) {
+            r[0] = fmaxf(r[0], s_r[ww][0]);
+            r[1] = fminf(r[1], s_r[ww][1]);
+            r[2] = fmaxf(r[2], s_r[ww][2]);
+            r[3] = fminf(r[3], s_r[ww][3]);
+            r[4] = fmaxf(r[4], s_r[ww][4]);
+        }
+        for (int k = 0; k < kVr; ++k) a.vr_partials[(size_t)tile * kVr + k] = r[k];
+    }
+}
+
+// Slab width W >= (1 + 2^-10) x the largest extent, so a box touches at most
+// two slabs (the margin covers the rounding of the slab index), and >= the
+// range / 254, so there are at most 255 slabs; keys quantise the sweep axis to
+// 24 bits over its range (any monotone map keeps the sweep exact).
+__global__ __launch_bounds__(kWG) void k_slab_params(ObjPipeArgs a) {
+    __shared__ float s_r[kWaves][kVr];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    float r[kVr] = {0.0f, __builtin_inff(), -__builtin_inff(), __builtin_inff(), -__builtin_inff()};
+    for (uint32_t t = tid; t < a.ntiles; t += kWG) {
+        const float* p = a.vr_partials + (size_t)t * kVr;
+        r[0] = fmaxf(r[0], p[0]);
+        r[1] = fminf(r[1], p[1]);
+        r[2] = fmaxf(r[2], p[2]);
+        r[3] = fminf(r[3], p[3]);
+        r[4] = fmaxf(r[4], p[4]);
+    }
+    r[0] = wave_maxf(r[0]);
+    r[1] = wave_minf(r[1]);
+    r[2] = wave_maxf(r[2]);
+    r[3] = wave_minf(r[3]);
+    r[4] = wave_maxf(r[4]);
+    if (l == 0)
+        for (int k = 0; k < kVr; ++k) s_r[w][k] = r[k];
+    __syncthreads();
+    if (tid != 0) return;
+    for (int ww = 1; ww < kWaves; ++ww) {
+        r[0] = fmaxf(r[0], s_r[ww][0]);
+        r[1] = fminf(r[1], s_r[ww][1]);
+        r[2] = fmaxf(r[2], s_r[ww][2]);
+        r[3] = fminf(r[3], s_r[ww][3]);
+        r[4] = fmaxf(r[4], s_r[ww][4]);
+    }
+    float v0 = 0.0f, invW = 0.0f, S = 1.0f, W = 0.0f;
+    const float span = r[2] - r[1];
+    if (isfinite(r[0]) && isfinite(span) && span > 0.0f && r[0] > 0.0f) {
+        W = fmaxf(r[0] * (1.0f + 0x1p-10f), span * (1.0f / 254.0f) * (1.0f + 0x1p-10f));
+        invW = 1.0f / W;
+        if (isfinite(invW) && invW > 0.0f) {
+            v0 = r[1];
+            S = fminf(floorf(span * invW) + 1.0f, 255.0f);
+        } else {
+            invW = 0.0f;
+        }
+    }
+    float u0 = 0.0f, idu = 0.0f;
+    const float uspan = r[4] - r[3];
+    if (isfinite(uspan) && uspan > 0.0f) {
+        idu = 16777215.0f / uspan;
+        if (isfinite(idu)) u0 = r[3]; else idu = 0.0f;
+    }
+    float* sp = a.slab;
+    sp[0] = v0; sp[1] = invW; sp[2] = S; sp[3] = u0; sp[4] = idu; sp[5] = W; sp[6] = 0.0f; sp[7] = 0.0f;
+}
+
+__device__ __forceinline__ uint32_t slab_of(float x, const float* sp) {
+    const float q = floorf((x - sp[0]) * sp[1]);
+    return (uint32_t)fminf(fmaxf(q, 0.0f), sp[2] - 1.0f);  // NaN -> 0
+}
+__device__ __forceinline__ uint32_t qkey(float x, const float* sp) {
+    const float q = floorf((x - sp[3]) * sp[4]);
+    return (uint32_t)fminf(fmaxf(q, 0.0f), 16777215.0f);
 }
 
 // ---------------------------------------------------------------- projection
@@ -209,18 +374,51 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
     const float u[3] = {ax[0], ax[1], ax[2]};
     const bool exact = axis_exact(a.kind, u);
     __syncthreads();
-    for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
-        float lo[3], hi[3], c4[4];
-        load_lohi(a.in, a.kind, i, lo, hi, c4);
-        float pl, ph;
-        project_lohi(a.kind, lo, hi, c4, u, exact, &pl, &ph);
-        const uint32_t key = flip_f32(pl);
-        a.keys[i] = key;
-        a.vals[i] = i;
-        atomicAdd(&s_h[0][key & 255u], 1u);
-        atomicAdd(&s_h[1][(key >> 8) & 255u], 1u);
-        atomicAdd(&s_h[2][(key >> 16) & 255u], 1u);
-        atomicAdd(&s_h[3][key >> 24], 1u);
+    if (!a.slabs) {
+        for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
+            float lo[3], hi[3], c4[4];
+            load_lohi(a.in, a.kind, i, lo, hi, c4);
+            float pl, ph;
+            project_lohi(a.kind, lo, hi, c4, u, exact, &pl, &ph);
+            const uint32_t key = flip_f32(pl);
+            a.keys[i] = key;
+            a.vals[i] = i;
+            atomicAdd(&s_h[0][key & 255u], 1u);
+            atomicAdd(&s_h[1][(key >> 8) & 255u], 1u);
+            atomicAdd(&s_h[2][(key >> 16) & 255u], 1u);
+            atomicAdd(&s_h[3][key >> 24], 1u);
+        }
+    } else {
+        // Box i -> entries 2i (its first slab, flagged) and 2i+1 (the next
+        // slab when it straddles a boundary, else an unused entry that sorts
+        // last).  Key = slab << 24 | 24-bit quantised lower bound on u.
+        const float v[3] = {a.axis2[0], a.axis2[1], a.axis2[2]};
+        const float* sp = a.slab;
+        for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
+            float lo[3], hi[3], c4[4], pl, ph, ql, qh;
+            load_lohi(a.in, a.kind, i, lo, hi, c4);
+            project_lohi(a.kind, lo, hi, c4, u, exact, &pl, &ph);
+            project_lohi(a.kind, lo, hi, c4, v, false, &ql, &qh);
+            const uint32_t s0 = slab_of(ql, sp);
+            uint32_t s1 = slab_of(qh, sp);
+            if (s1 > s0 + 1u) {  // cannot happen with W > max extent; fail loudly
+                atomicOr(&a.ctrl->flags, kFlagSlab);
+                s1 = s0 + 1u;
+            }
+            const uint32_t q = qkey(pl, sp);
+            const uint32_t k0 = (s0 << 24) | q;
+            const uint32_t k1 = s1 > s0 ? ((s1 << 24) | q) : kSlabNull;
+            reinterpret_cast<uint2*>(a.keys)[i] = make_uint2(k0, k1);
+            reinterpret_cast<uint2*>(a.vals)[i] = make_uint2(i | kFirstBit, i);
+            atomicAdd(&s_h[0][k0 & 255u], 1u);
+            atomicAdd(&s_h[1][(k0 >> 8) & 255u], 1u);
+            atomicAdd(&s_h[2][(k0 >> 16) & 255u], 1u);
+            atomicAdd(&s_h[3][k0 >> 24], 1u);
+            atomicAdd(&s_h[0][k1 & 255u], 1u);
+            atomicAdd(&s_h[1][(k1 >> 8) & 255u], 1u);
+            atomicAdd(&s_h[2][(k1 >> 16) & 255u], 1u);
+            atomicAdd(&s_h[3][k1 >> 24], 1u);
+        }
     }
     __syncthreads();
     uint32_t* h = a.hist_obj + (size_t)ti.seg * a.npass_obj * 256u;
@@ -252,13 +450,17 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
         Rec r;
         uint32_t end = 0xFFFFFFFFu;  // invalid lanes rank last
         if (valid) {
-            const uint32_t orig = V[s];
+            const uint32_t vs = V[s];
+            const uint32_t key = a.slabs ? K[s] : 0u;
+            const uint32_t orig = a.slabs ? (vs & ~kFirstBit) : vs;
             float lo[3], hi[3], c4[4];
             load_lohi(a.in, KIND, orig, lo, hi, c4);
             float pl, ph;
             project_lohi(KIND, lo, hi, c4, u, exact, &pl, &ph);
-            const uint32_t hk = flip_f32(ph);
-            uint32_t b = s + 1, e = ti.seg_end;  // first t in [s+1, seg_end) with K[t] > hk
+            // first t in [s+1, seg_end) with K[t] > hk
+            const uint32_t hk = a.slabs ? ((key & 0xFF000000u) | qkey(ph, a.slab)) : flip_f32(ph);
+            uint32_t b = s + 1, e = ti.seg_end;
+            if (a.slabs && key == kSlabNull) e = b;  // unused entry: no candidates
             while (b < e) {
                 const uint32_t mid = b + ((e - b) >> 1);
                 if (K[mid] <= hk) b = mid + 1; else e = mid;
@@ -272,8 +474,30 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
                 r.c[0] = c4[0]; r.c[1] = c4[1]; r.c[2] = c4[2]; r.c[3] = c4[3];
                 r.pad[0] = r.pad[1] = 0u;
             }
-            r.orig = orig;
+            r.orig = vs;  // slab mode: with kFirstBit
             r.aux = 0u;
+            if (a.slabs) {
+                if (key == kSlabNull) {  // a box that overlaps nothing
+                    if constexpr (KIND == 0) {
+                        r.lo[0] = r.lo[1] = r.lo[2] = __builtin_inff();
+                        r.hi[0] = r.hi[1] = r.hi[2] = -__builtin_inff();
+                    } else {
+                        r.c[0] = __builtin_inff();
+                    }
+                    r.orig = orig;
+                    r.aux = s;
+                } else {
+                    // first position of this entry's slab: candidates of another
+                    // slab are told apart by it in the sweep
+                    const uint32_t lk = key & 0xFF000000u;
+                    uint32_t lb = ti.seg_start, le = s;
+                    while (lb < le) {
+                        const uint32_t mid = lb + ((le - lb) >> 1);
+                        if (K[mid] < lk) lb = mid + 1; else le = mid;
+                    }
+                    r.aux = lb;
+                }
+            }
             reinterpret_cast<Rec*>(a.rec)[s] = r;
         }
         s_end[tid] = end;
@@ -318,7 +542,11 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
 // wave has left, so the pair buffer holds exactly the pairs, densely.
 constexpr uint32_t kStage = kSweepStage;
 
-template <int KIND, bool GROUP, int MODE>  // MODE 0 canonical, 1 symmetric, 2 count
+// SLAB: records are slab entries; a hit also needs the candidate in the
+// object's slab (its slab starts at or before the object) and one of the two
+// entries to be its box's first slab (the pair's slab: that of the larger
+// lower bound), so each pair is found exactly once.
+template <int KIND, bool GROUP, int MODE, bool SLAB>  // MODE 0 canonical, 1 symmetric, 2 count
 __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
     // The object records are read-only here; the constant address space lets
@@ -383,8 +611,9 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     c.c[0] = __builtin_inff();
                 }
             }
+            const uint32_t corig = SLAB ? (c.orig & ~kFirstBit) : c.orig;
             uint32_t gc = 0;
-            if (GROUP) gc = a.group[c.orig];
+            if (GROUP) gc = a.group[corig];
 
             auto process = [&](const Rec& o, auto masked_tag) {
                 constexpr bool MASKED = decltype(masked_tag)::value;
@@ -411,8 +640,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     const float e1 = r0 * r0;
                     hit = t1 < e1;
                 }
+                const uint32_t oorig = SLAB ? (o.orig & ~kFirstBit) : o.orig;
                 if (MASKED) hit = hit & (t > o.aux);
-                if (GROUP) hit = hit & (a.group[o.orig] != gc);
+                if (SLAB) hit = hit & (c.aux <= o.aux) & (((o.orig | c.orig) & kFirstBit) != 0u);
+                if (GROUP) hit = hit & (a.group[oorig] != gc);
                 const unsigned long long m = ballot(hit);
                 if (m == 0ull) return;
                 const uint32_t cnt = (uint32_t)__popcll(m) * mult;
@@ -428,12 +659,12 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     if (hit) {
                         const uint32_t r = fill + mult * mbcnt(m);
                         if (MODE == 0) {
-                            const uint32_t x = o.orig < c.orig ? o.orig : c.orig;
-                            const uint32_t y = o.orig < c.orig ? c.orig : o.orig;
+                            const uint32_t x = oorig < corig ? oorig : corig;
+                            const uint32_t y = oorig < corig ? corig : oorig;
                             stage[r] = make_uint2(x, y);
                         } else {
-                            stage[r] = make_uint2(o.orig, c.orig);
-                            stage[r + 1] = make_uint2(c.orig, o.orig);
+                            stage[r] = make_uint2(oorig, corig);
+                            stage[r + 1] = make_uint2(corig, oorig);
                         }
                     }
                     fill += cnt;
@@ -591,6 +822,11 @@ __global__ void k_scene_offsets(const uint2* __restrict__ P, const Ctrl* ctrl, u
 }
 
 // ---------------------------------------------------------------- launchers
+hipError_t launch_slab_params(const ObjPipeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_vrange, dim3(a.ntiles), dim3(kWG), 0, st, a);
+    hipLaunchKernelGGL(k_slab_params, dim3(1), dim3(kWG), 0, st, a);
+    return hipGetLastError();
+}
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_moments, dim3(a.ntiles), dim3(kWG), 0, st, a);
     const uint32_t nseg = a.seg.tile_seg ? a.seg.nseg : 1u;
@@ -612,7 +848,10 @@ hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* K, const uint32_t
 template <int KIND, bool GROUP, int MODE>
 static void sweep_launch(const SweepArgs& a, hipStream_t st) {
     const uint32_t blocks = a.ntiles * (a.tile / kSweepTile);
-    hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE>), dim3(blocks), dim3(kWG), 0, st, a);
+    if (a.slabs)
+        hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, true>), dim3(blocks), dim3(kWG), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, false>), dim3(blocks), dim3(kWG), 0, st, a);
 }
 
 template <int KIND, bool GROUP>

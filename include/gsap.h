@@ -62,7 +62,16 @@ enum { GSAP_OUT_CANON = 0,  /* uint32x2 (i<j), lexicographic, each pair once    
        GSAP_OUT_SYM_CSR = 1 /* reference format: int32x2 both orientations,
                                sorted by (i,j), plus pairEntries row starts     */ };
 
-enum { GSAP_FLAG_SKIP_SAME_GROUP = 1u /* drop pairs whose d_group ids are equal */ };
+enum {
+    GSAP_FLAG_SKIP_SAME_GROUP = 1u, /* drop pairs whose d_group ids are equal              */
+    /* Sweep inside slabs of the second principal axis (the reference's cell split,
+     * SapKernel.cu:150-213 and 2634-2678, generalised to up to 255 slabs with no
+     * cross pass): each box is keyed into the one or two slabs its projection
+     * touches and a pair is reported only in the slab of the larger lower bound.
+     * Same pair set, far fewer sweep tests on spread-out scenes.  Single-scene
+     * updates only; n <= 2^30. */
+    GSAP_FLAG_SLABS = 2u
+};
 
 typedef struct gsap_sap gsap_sap;
 
