@@ -49,6 +49,7 @@ def bench_batched(G, W, torch, dev, ns, steps, warmup):
     ntot = int(off[-1])
     bd = torch.from_numpy(boxes).to(dev)
     sb = G.Sap(max_n=ntot)
+    tune = sb.autotune(bd, scene_offsets=off)
     P, st = sb.collide_batched(bd, off, None, None)
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     poff = torch.empty(ns + 1, dtype=torch.int64, device=dev)
@@ -83,7 +84,8 @@ def bench_batched(G, W, torch, dev, ns, steps, warmup):
             "ms_per_scene_loop_sync": round(ms_ls, 4),
             "speedup_vs_loop_async": round(ms_la / ms_b, 2),
             "speedup_vs_loop_sync": round(ms_ls / ms_b, 2),
-            "pairs_identical": same}
+            "pairs_identical": same,
+            "tune": {k: v for k, v in tune.items() if k != "from_cache"}}
 
 
 def bench_learned(G, W, torch, dev, ns):
@@ -113,6 +115,24 @@ def bench_learned(G, W, torch, dev, ns):
             "ms_batched_learned": round(ms["learned"], 4)}
 
 
+def _tune_particles(G, torch, sap, pp, npart):
+    """Autotune the particle broadphase on the step's own particle AABBs and
+    body ids (device buffers owned by the rigid-body handle)."""
+    import ctypes
+    aabb = torch.empty((npart, 6), dtype=torch.float32, device="cuda")
+    pos = torch.empty((npart, 4), dtype=torch.float32, device="cuda")
+    body = torch.empty(npart, dtype=torch.int32, device="cuda")
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    torch.cuda.synchronize()
+    assert hip.hipMemcpy(pos.data_ptr(), pp.p_pos, npart * 16, 3) == 0
+    assert hip.hipMemcpy(body.data_ptr(), pp.p_body, npart * 4, 3) == 0
+    r = pos[:, 3:4]
+    aabb[:, :3] = pos[:, :3] - r
+    aabb[:, 3:] = pos[:, :3] + r
+    return sap.autotune(aabb, group=body)
+
+
 def bench_particles(G, W, torch, dev, nb, steps, with_cpu):
     """Rigid cubes (3x3x3 particles, radius 0.5, spacing 1) dropped into a
     256x256 walled box from heights U[0,200]: spring-dashpot k=1e4, c=50,
@@ -133,13 +153,17 @@ def bench_particles(G, W, torch, dev, nb, steps, with_cpu):
     sap = G.Sap(max_n=npart, input_kind=G.INPUT_AABB, output_kind=G.OUT_SYM_CSR,
                 flags=G.FLAG_SKIP_SAME_GROUP | G.FLAG_SLABS)
     dt = 1e-4
+    prb.step(sap, dt)  # particle AABBs of the first step, to tune the broadphase on
+    pp = prb.ptrs()
+    tune = _tune_particles(G, torch, sap, pp, npart)
     ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
     p = prb.ptrs()  # raises on pair-buffer overflow
     out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase "
                        "(slab sweep)",
            "particles": npart, "steps": steps, "ms_per_step": round(ms, 4),
            "steps_per_s": round(1e3 / ms, 1), "particle_updates_per_s": round(npart * 1e3 / ms, 1),
-           "directed_pairs_last_step": int(p.num_pairs)}
+           "directed_pairs_last_step": int(p.num_pairs),
+           "tune": {k: v for k, v in tune.items() if k != "from_cache"}}
     if with_cpu:
         # Bounded sample: one full-size oracle step whose sweep visits every
         # 64th sorted particle; the sweep and canonical-sort time is scaled
@@ -200,6 +224,7 @@ def main():
     mode = {"pca": G.AXIS_PCA, "coord": G.AXIS_COORD_MAXVAR, "learned": G.AXIS_LEARNED}[args.axis]
     slabs = not args.no_slabs
     sap = G.Sap(max_n=n, axis_mode=mode, flags=G.FLAG_SLABS if slabs else 0)
+    tune = sap.autotune(bd)  # measured launch configuration (tiles, digit widths, sweep kind)
     P, st0 = sap.count(bd)  # pre-count pass sizes the 64-bit pair buffer
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     sap.set_timing(True)
@@ -275,7 +300,8 @@ def main():
         "config": {"workload": f"aniso-{n}: AABBs, centres R diag(20,5,1) z, half-extents "
                                "log-U[0.05,0.8], seed 1, canonical uint32 pair list",
                    "n": n, "pairs": P, "axis": args.axis,
-                   "sweep": "slabs" if slabs else "plain",
+                   "sweep": "slabs" if sap.tune["slabs"] else "plain",
+                   "tune": {k: v for k, v in tune.items() if k != "from_cache"},
                    "sweep_tests": int(st.sweep_tests), "axis_used": [round(x, 6) for x in st.axis]},
         "roofline": roof,
         "sort_roofline": sort_roof,

@@ -42,7 +42,7 @@ EXPORTS = [
     "gsap_mlp_create", "gsap_mlp_destroy", "gsap_mlp_num_params", "gsap_mlp_set_params",
     "gsap_mlp_get_params", "gsap_mlp_forward", "gsap_mlp_loss_grad", "gsap_mlp_train",
     "gsap_axis_candidates", "gsap_sap_set_learned_model", "gsap_sap_features_batched",
-    "gsap_sap_axis_tests_batched",
+    "gsap_sap_axis_tests_batched", "gsap_sap_get_tune", "gsap_sap_set_tune", "gsap_sap_autotune",
 ]
 N_FEATURES, N_PARAMS, N_CANDIDATES = 64, 25348, 64
 
@@ -64,6 +64,16 @@ class Stats(ctypes.Structure):
     _fields_ = [("num_pairs", ctypes.c_uint64), ("sweep_tests", ctypes.c_uint64),
                 ("axis", ctypes.c_float * 3), ("axis_source", ctypes.c_int32),
                 ("ms_total", ctypes.c_float), ("n", ctypes.c_uint32)]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("sort_items", ctypes.c_int32), ("sort_bits", ctypes.c_int32),
+                ("pair_items", ctypes.c_int32), ("pair_bits", ctypes.c_int32),
+                ("pair_grid", ctypes.c_int32), ("slabs", ctypes.c_int32),
+                ("ms", ctypes.c_float), ("from_cache", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PrbPhysics(ctypes.Structure):
@@ -138,6 +148,9 @@ def lib():
         "gsap_sap_set_learned_model": (i32, [P, P, f32]),
         "gsap_sap_features_batched": (i32, [P, P, P, P, u32, P]),
         "gsap_sap_axis_tests_batched": (i32, [P, P, P, P, u32, P, u32, P, P]),
+        "gsap_sap_get_tune": (i32, [P, P]),
+        "gsap_sap_set_tune": (i32, [P, P]),
+        "gsap_sap_autotune": (i32, [P, P, P, u32, P, P, u32, ctypes.c_char_p, P]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -268,6 +281,36 @@ class Sap:
         del spacing
         P, ne, _ = self.collide_into(d_particle_data, pairs, pair_entries)
         return P, ne
+
+    @property
+    def tune(self):
+        """The launch configuration (gsap_tune) as a dict."""
+        t = Tune()
+        check(lib().gsap_sap_get_tune(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, **kw):
+        t = Tune()
+        check(lib().gsap_sap_get_tune(self.h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().gsap_sap_set_tune(self.h, ctypes.byref(t)))
+
+    def autotune(self, boxes, scene_offsets=None, group=None, cache=None, stream=None):
+        """Measure candidate configurations on this input and keep the fastest
+        (gsap_sap_autotune); `cache` is an optional text file path."""
+        import numpy as np
+        t = Tune()
+        off = None
+        ns = 0
+        if scene_offsets is not None:
+            off = np.ascontiguousarray(scene_offsets, dtype=np.uint32)
+            ns = len(off) - 1
+        check(lib().gsap_sap_autotune(
+            self.h, _stream(stream), _ptr(boxes), boxes.shape[0], _ptr(group),
+            None if off is None else off.ctypes.data_as(ctypes.c_void_p), ns,
+            None if cache is None else os.fsencode(cache), ctypes.byref(t)))
+        return t.as_dict()
 
     def set_learned_model(self, model, conf_min=0.95):
         check(lib().gsap_sap_set_learned_model(self.h, model.h, conf_min))

@@ -63,8 +63,16 @@ uint32_t bits_for(uint64_t n) {
 struct gsap_sap {
     gsap_sap_desc desc{};
     std::string err;
-    int items = 8;       // object sort: keys per thread (tile = 256 * items)
-    int pair_items = 16; // pair sort
+    // launch configuration (gsap_tune)
+    int items = 8;        // object sort: keys per thread (tile = 256 * items)
+    int obj_bits = 8;     // object sort digit width
+    int pair_items = 16;  // pair sort
+    int pair_bits = 8;    // pair sort digit width
+    int pair_grid = 2048; // pair sort persistent workgroups
+    int slabs = 0;        // slab sweep for single-scene updates
+    float tune_ms = 0.0f;
+    int tune_cached = 0;
+    uint32_t cap_em = 1;  // entries per object the sort buffers are sized for
     Buf keys[2], vals[2], rec, partials, axis, hist_obj, hist_pair, lb_obj, lb_pair;
     Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
@@ -127,8 +135,8 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
         return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad axis_mode");
     if (desc->output_kind != GSAP_OUT_CANON && desc->output_kind != GSAP_OUT_SYM_CSR)
         return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad output_kind");
-    if (desc->digit_bits != 0 && desc->digit_bits != 8)
-        return fail(nullptr, GSAP_ERR_UNSUPPORTED, "gsap_sap_create: digit_bits must be 0 or 8");
+    if (desc->digit_bits != 0 && (desc->digit_bits < 6 || desc->digit_bits > 8))
+        return fail(nullptr, GSAP_ERR_UNSUPPORTED, "gsap_sap_create: digit_bits must be 0 or 6..8");
     if (desc->sort_items != 0 && desc->sort_items != 4 && desc->sort_items != 8 &&
         desc->sort_items != 12 && desc->sort_items != 16)
         return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: sort_items in {4,8,12,16}");
@@ -140,6 +148,8 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
     gsap_sap* h = new gsap_sap();
     h->desc = *desc;
     if (desc->sort_items) h->items = desc->sort_items;
+    if (desc->digit_bits) h->obj_bits = h->pair_bits = desc->digit_bits;
+    h->slabs = (desc->flags & GSAP_FLAG_SLABS) ? 1 : 0;
     hipError_t e = grow(h->ctrl, sizeof(Ctrl));
     if (e == hipSuccess) e = hipMemset(h->ctrl.p, 0, sizeof(Ctrl));
     if (e == hipSuccess) e = grow(h->counts, 8 * sizeof(unsigned long long));
@@ -175,12 +185,16 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
     return GSAP_OK;
 }
 
-static bool slab_mode(const gsap_sap* h) { return (h->desc.flags & GSAP_FLAG_SLABS) != 0; }
+static bool slab_mode(const gsap_sap* h) { return h->slabs != 0; }
 
 // Object workspace for n objects, nseg scenes, ntiles tiles.  In slab mode
 // the sort, gather and sweep buffers hold two entries per object.
 static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntiles) {
-    const size_t em = slab_mode(h) ? 2 : 1;
+    const uint32_t em = slab_mode(h) ? 2 : 1;
+    if (em > h->cap_em) {  // sort buffers sized for one entry per object: regrow
+        h->cap_n = 0;
+        h->cap_tiles = 0;
+    }
     if (n > h->cap_n) {
         uint32_t want = std::max(n, h->desc.max_n);
         for (int i = 0; i < 2; ++i) {
@@ -195,6 +209,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         HIPCHK(h, hipMemset(h->has.p, 0, (size_t)want * 4));
         HIPCHK(h, grow(h->block_sums, ((size_t)want / 2048 + 2) * 4));
         h->cap_n = want;
+        h->cap_em = std::max(h->cap_em, em);
     }
     if (ntiles > h->cap_tiles) {
         HIPCHK(h, grow(h->partials, (size_t)ntiles * 10 * sizeof(double)));
@@ -207,7 +222,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     }
     if (nseg > h->cap_seg) {
         HIPCHK(h, grow(h->axis, (size_t)nseg * 16));
-        HIPCHK(h, grow(h->hist_obj, (size_t)nseg * 4 * 256 * 4));
+        HIPCHK(h, grow(h->hist_obj, (size_t)nseg * kMaxPasses * 256 * 4));
         HIPCHK(h, grow(h->seg_start, (size_t)nseg * 4));
         HIPCHK(h, grow(h->seg_n, (size_t)nseg * 4));
         HIPCHK(h, grow(h->seg_tile0, (size_t)(nseg + 1) * 4));
@@ -257,7 +272,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     const bool sym = h->desc.output_kind == GSAP_OUT_SYM_CSR;
     const uint32_t tile = sort_tile(h->items);
     const uint32_t bbits = bits_for(n);
-    const uint32_t npass_pair = (2 * bbits + 7) / 8;
+    const uint32_t npass_pair = (2 * bbits + h->pair_bits - 1) / h->pair_bits;
+    const uint32_t npass_obj = (32 + h->obj_bits - 1) / h->obj_bits;
     Ctrl* ctrl = (Ctrl*)h->ctrl.p;
     h->nev = 0;
     mark(h, st);
@@ -275,7 +291,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.partials = (double*)h->partials.p;
     a.ctrl = ctrl;
     a.hist_obj = (uint32_t*)h->hist_obj.p;
-    a.npass_obj = 4;
+    a.npass_obj = npass_obj;
+    a.bits_obj = h->obj_bits;
     a.hist_pair = (uint32_t*)h->hist_pair.p;
     a.npass_pair = count_only ? 0 : npass_pair;
     a.keys = (uint32_t*)h->keys[0].p;
@@ -324,11 +341,11 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     s.seg = seg;
     s.ntiles = ntiles_e;
     s.hist = a.hist_obj;
-    s.npass = 4;
+    s.npass = npass_obj;
     s.lookback = (unsigned long long*)h->lb_obj.p;
     s.ctrl = ctrl;
     s.ctr_base = 0;
-    s.bits = 8;
+    s.bits = h->obj_bits;
     s.items = h->items;
     s.grid = 0;
     int which = 0;
@@ -362,7 +379,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     HIPCHK(h, launch_sweep(w, st));
     mark(h, st);
     if (!count_only) {
-        HIPCHK(h, launch_pair_hist(t0p, &ctrl->slots, h->slot_cap, a.hist_pair, npass_pair, bbits, st));
+        HIPCHK(h, launch_pair_hist(t0p, &ctrl->slots, h->slot_cap, a.hist_pair, npass_pair, bbits,
+                                   h->pair_bits, st));
         SortArgs p{};
         p.n_dev = &ctrl->slots;
         p.cap = h->slot_cap;
@@ -371,10 +389,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;
         p.ctr_base = kMaxPasses;
-        p.bits = 8;
+        p.bits = h->pair_bits;
         p.items = h->pair_items;
         p.bbits = (int)bbits;
-        p.grid = 2048;
+        p.grid = (uint32_t)h->pair_grid;
         p.last_n = &ctrl->num_pairs;
         p.last_cap = cap;
         HIPCHK(h, sort_pairs(p, t0p, t1p, outp, st));
@@ -528,7 +546,8 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
         return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode emits canonical pairs only");
     if (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP)
         return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no group filter");
-    if (slab_mode(h)) return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no slab split");
+    if (h->desc.flags & GSAP_FLAG_SLABS)
+        return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no slab split");
     const uint32_t n = h_scene_offsets[n_scenes];
     if (h_scene_offsets[0] != 0) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets must start at 0");
     int32_t rc = check_common(h, d_in, n, nullptr);
@@ -772,6 +791,7 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     a.ctrl = (Ctrl*)h->ctrl.p;
     a.hist_obj = (uint32_t*)h->hist_obj.p;
     a.npass_obj = 4;
+    a.bits_obj = 8;
     a.hist_pair = (uint32_t*)h->hist_pair.p;
     a.npass_pair = 0;
     a.seg_mom = (double*)h->seg_mom.p;
@@ -807,5 +827,236 @@ extern "C" int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const 
                                 (const uint32_t*)h->seg_n.p, ns, (const float*)h->ax_tests_axes.p,
                                 naxes, (unsigned long long*)d_tests, (unsigned long long*)d_pairs, st));
     HIPCHK(h, hipStreamSynchronize(st));
+    return GSAP_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+static bool valid_tune(const gsap_tune* t) {
+    const bool si = t->sort_items == 4 || t->sort_items == 8 || t->sort_items == 12 ||
+                    t->sort_items == 16;
+    const bool pi = t->pair_items == 8 || t->pair_items == 16;
+    return si && pi && t->sort_bits >= 6 && t->sort_bits <= 8 && t->pair_bits >= 6 &&
+           t->pair_bits <= 8 && t->pair_grid >= 256 && t->pair_grid <= 16384 &&
+           (t->slabs == 0 || t->slabs == 1);
+}
+
+static gsap_tune tune_of(const gsap_sap* h) {
+    gsap_tune t{};
+    t.sort_items = h->items;
+    t.sort_bits = h->obj_bits;
+    t.pair_items = h->pair_items;
+    t.pair_bits = h->pair_bits;
+    t.pair_grid = h->pair_grid;
+    t.slabs = h->slabs;
+    t.ms = h->tune_ms;
+    t.from_cache = h->tune_cached;
+    return t;
+}
+
+static void apply_tune(gsap_sap* h, const gsap_tune& t) {
+    h->items = t.sort_items;
+    h->obj_bits = t.sort_bits;
+    h->pair_items = t.pair_items;
+    h->pair_bits = t.pair_bits;
+    h->pair_grid = t.pair_grid;
+    h->slabs = t.slabs;
+    if (h->gexec) {  // the captured batched update used the old configuration
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    h->g_offsets.clear();
+}
+
+extern "C" int32_t gsap_sap_get_tune(gsap_sap* h, gsap_tune* out) {
+    if (!h || !out) return fail(h, GSAP_ERR_INVALID_ARG, "null argument");
+    *out = tune_of(h);
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_set_tune(gsap_sap* h, const gsap_tune* t) {
+    if (!h || !t) return fail(h, GSAP_ERR_INVALID_ARG, "null argument");
+    if (!valid_tune(t))
+        return fail(h, GSAP_ERR_INVALID_ARG,
+                    "gsap_sap_set_tune: items 4/8/12/16 and 8/16, bits 6..8, grid 256..16384, "
+                    "slabs 0/1");
+    apply_tune(h, *t);
+    h->tune_ms = 0.0f;
+    h->tune_cached = 0;
+    return GSAP_OK;
+}
+
+// Cache key: device, what the update computes, and the size bucket (half
+// octaves of the object count; batched: also of the scene count).
+static std::string tune_key(gsap_sap* h, uint32_t n, uint32_t n_scenes, bool batched) {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    (void)hipGetDevice(&dev);
+    (void)hipGetDeviceProperties(&prop, dev);
+    std::string arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    const int nb = n ? (int)std::lround(2.0 * std::log2((double)n)) : 0;
+    const int sb = batched && n_scenes ? (int)std::lround(2.0 * std::log2((double)n_scenes)) : 0;
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s|cu%d|in%d|out%d|axis%d|flags%u|%s|n%d|s%d", arch.c_str(),
+             prop.multiProcessorCount, h->desc.input_kind, h->desc.output_kind, h->desc.axis_mode,
+             h->desc.flags, batched ? "batched" : "single", nb, sb);
+    return buf;
+}
+
+static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t) {
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    char line[512];
+    bool hit = false;
+    while (fgets(line, sizeof line, f)) {
+        char k[300];
+        gsap_tune c{};
+        if (sscanf(line, "%299s %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
+                   &c.pair_items, &c.pair_bits, &c.pair_grid, &c.slabs, &c.ms) == 8 &&
+            key == k && valid_tune(&c)) {
+            *t = c;  // the last matching line wins
+            hit = true;
+        }
+    }
+    fclose(f);
+    return hit;
+}
+
+static void cache_append(const char* path, const std::string& key, const gsap_tune& t) {
+    FILE* f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
+            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.ms);
+    fclose(f);
+}
+
+extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                                     const uint32_t* d_group, const uint32_t* offs,
+                                     uint32_t n_scenes, const char* cache_path, gsap_tune* out) {
+    if (!h || !out) return fail(h, GSAP_ERR_INVALID_ARG, "null argument");
+    const bool batched = offs != nullptr;
+    if (batched) {
+        if (n_scenes == 0) return fail(h, GSAP_ERR_INVALID_ARG, "no scenes");
+        n = offs[n_scenes];
+    }
+    if (n == 0 || !d_in) return fail(h, GSAP_ERR_INVALID_ARG, "nothing to tune on");
+    hipStream_t st = (hipStream_t)stream;
+    const std::string key = tune_key(h, n, n_scenes, batched);
+    gsap_tune best = tune_of(h);
+    if (cache_path && cache_lookup(cache_path, key, &best)) {
+        if (batched) best.slabs = h->slabs;
+        apply_tune(h, best);
+        h->tune_ms = best.ms;
+        h->tune_cached = 1;
+        *out = tune_of(h);
+        return GSAP_OK;
+    }
+    // Pair buffers for the measured updates (the pair count does not depend
+    // on the configuration).
+    const bool sym = h->desc.output_kind == GSAP_OUT_SYM_CSR;
+    int32_t rc;
+    uint64_t P = 0;
+    if (batched)
+        rc = gsap_sap_collide_batched(h, stream, d_in, offs, n_scenes, nullptr, 0, nullptr, &P,
+                                      nullptr);
+    else
+        rc = gsap_sap_collide(h, stream, d_in, n, d_group, nullptr, 0, nullptr, &P, nullptr,
+                              nullptr);
+    if (rc) return rc;
+    void* d_pairs = nullptr;
+    void* d_extra = nullptr;
+    const size_t extra = batched ? ((size_t)n_scenes + 1) * 8 : (sym ? (size_t)n * 4 : 0);
+    HIPCHK(h, hipMalloc(&d_pairs, std::max<uint64_t>(P, 1) * 8));
+    if (extra && hipMalloc(&d_extra, extra) != hipSuccess) {
+        (void)hipFree(d_pairs);
+        return fail(h, GSAP_ERR_OOM, "autotune: no memory for the scene offsets / entries");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const bool timing = h->timing;
+    h->timing = false;
+    auto run = [&]() -> int32_t {
+        if (batched)
+            return gsap_sap_collide_batched(h, stream, d_in, offs, n_scenes, d_pairs, P,
+                                            (uint64_t*)d_extra, nullptr, nullptr);
+        return gsap_sap_collide(h, stream, d_in, n, d_group, d_pairs, P, (int32_t*)d_extra,
+                                nullptr, nullptr, nullptr);
+    };
+    // Median of three timed updates after one untimed (graph capture, first
+    // touch of newly grown buffers).
+    auto measure = [&](const gsap_tune& t, float* ms) -> int32_t {
+        apply_tune(h, t);
+        int32_t r = run();
+        if (r) return r;
+        float v[3];
+        for (int k = 0; k < 3; ++k) {
+            if (hipEventRecord(e0, st) != hipSuccess) return GSAP_ERR_HIP;
+            r = run();
+            if (r) return r;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+                return GSAP_ERR_HIP;
+            (void)hipEventElapsedTime(&v[k], e0, e1);
+        }
+        std::sort(v, v + 3);
+        *ms = v[1];
+        return GSAP_OK;
+    };
+    float best_ms = 0.0f;
+    rc = measure(best, &best_ms);
+    auto consider = [&](gsap_tune t) {
+        if (rc) return;
+        float ms = 0.0f;
+        rc = measure(t, &ms);
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    };
+    if (!batched) {  // stage 1: plain or slab sweep
+        gsap_tune t = best;
+        t.slabs = 1 - t.slabs;
+        consider(t);
+    }
+    {  // stage 2: object sort tile and digit width
+        const gsap_tune base = best;
+        for (int items : {4, 8, 12, 16})
+            for (int bits : {6, 7, 8}) {
+                gsap_tune t = base;
+                t.sort_items = items;
+                t.sort_bits = bits;
+                if (items != base.sort_items || bits != base.sort_bits) consider(t);
+            }
+    }
+    {  // stage 3: pair sort tile, digit width and grid
+        const gsap_tune base = best;
+        for (int items : {8, 16})
+            for (int bits : {6, 7, 8})
+                for (int grid : {512, 1024, 2048, 4096}) {
+                    gsap_tune t = base;
+                    t.pair_items = items;
+                    t.pair_bits = bits;
+                    t.pair_grid = grid;
+                    if (items != base.pair_items || bits != base.pair_bits || grid != base.pair_grid)
+                        consider(t);
+                }
+    }
+    h->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_pairs);
+    if (d_extra) (void)hipFree(d_extra);
+    if (rc) {
+        const std::string msg = "autotune: " + h->err;
+        return fail(h, rc, msg);
+    }
+    best.ms = best_ms;
+    apply_tune(h, best);
+    h->tune_ms = best_ms;
+    h->tune_cached = 0;
+    if (cache_path) cache_append(cache_path, key, tune_of(h));
+    *out = tune_of(h);
     return GSAP_OK;
 }

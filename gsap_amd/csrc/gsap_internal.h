@@ -72,6 +72,7 @@ struct ObjPipeArgs {
     Ctrl* ctrl;
     uint32_t* hist_obj;     // [nseg][npass][256]
     uint32_t npass_obj;
+    uint32_t bits_obj;      // object-sort digit width (6..8)
     uint32_t* hist_pair;    // [npass_pair][256]
     uint32_t npass_pair;
     uint32_t* keys;         // [n] keys (ping)
@@ -121,7 +122,7 @@ struct SweepArgs {
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
 hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
-                            unsigned long long cap, uint32_t* hist, int npass, int bbits,
+                            unsigned long long cap, uint32_t* hist, int npass, int bbits, int bits,
                             hipStream_t st);
 hipError_t launch_entries(const uint2* sorted_pairs, const unsigned long long* n_dev,
                           unsigned long long cap, uint32_t nobj, uint32_t* rowstart,

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
 
     // Per-update housekeeping for the later stages (which run after this kernel).
     {
-        const uint32_t total = a.seg.nseg * a.npass_obj * 256u;
+        const uint32_t total = a.seg.nseg * a.npass_obj * (1u << a.bits_obj);
         const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
         const uint32_t b0 = tile * chunk, b1 = min(b0 + chunk, total);
         for (uint32_t i = b0 + tid; i < b1; i += kWG) a.hist_obj[i] = 0u;
@@ -365,11 +365,19 @@ __device__ __forceinline__ uint32_t qkey(float x, const float* sp) {
 }
 
 // ---------------------------------------------------------------- projection
+// Digit histograms of one key for every pass of the object sort.
+__device__ __forceinline__ void hist_key(uint32_t (*s_h)[256], uint32_t key, uint32_t npass,
+                                         uint32_t bits) {
+    const uint32_t mask = (1u << bits) - 1u;
+    for (uint32_t p = 0; p < npass; ++p) atomicAdd(&s_h[p][(key >> (p * bits)) & mask], 1u);
+}
+
 __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
-    __shared__ uint32_t s_h[4][256];
+    __shared__ uint32_t s_h[kMaxPasses][256];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
-    for (uint32_t i = tid; i < 4 * 256; i += kWG) (&s_h[0][0])[i] = 0u;
+    const uint32_t npass = a.npass_obj, bits = a.bits_obj, radix = 1u << bits;
+    for (uint32_t i = tid; i < kMaxPasses * 256; i += kWG) (&s_h[0][0])[i] = 0u;
     const float* ax = a.axis + 4 * (size_t)ti.seg;
     const float u[3] = {ax[0], ax[1], ax[2]};
     const bool exact = axis_exact(a.kind, u);
@@ -383,10 +391,7 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
             const uint32_t key = flip_f32(pl);
             a.keys[i] = key;
             a.vals[i] = i;
-            atomicAdd(&s_h[0][key & 255u], 1u);
-            atomicAdd(&s_h[1][(key >> 8) & 255u], 1u);
-            atomicAdd(&s_h[2][(key >> 16) & 255u], 1u);
-            atomicAdd(&s_h[3][key >> 24], 1u);
+            hist_key(s_h, key, npass, bits);
         }
     } else {
         // Box i -> entries 2i (its first slab, flagged) and 2i+1 (the next
@@ -410,20 +415,14 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
             const uint32_t k1 = s1 > s0 ? ((s1 << 24) | q) : kSlabNull;
             reinterpret_cast<uint2*>(a.keys)[i] = make_uint2(k0, k1);
             reinterpret_cast<uint2*>(a.vals)[i] = make_uint2(i | kFirstBit, i);
-            atomicAdd(&s_h[0][k0 & 255u], 1u);
-            atomicAdd(&s_h[1][(k0 >> 8) & 255u], 1u);
-            atomicAdd(&s_h[2][(k0 >> 16) & 255u], 1u);
-            atomicAdd(&s_h[3][k0 >> 24], 1u);
-            atomicAdd(&s_h[0][k1 & 255u], 1u);
-            atomicAdd(&s_h[1][(k1 >> 8) & 255u], 1u);
-            atomicAdd(&s_h[2][(k1 >> 16) & 255u], 1u);
-            atomicAdd(&s_h[3][k1 >> 24], 1u);
+            hist_key(s_h, k0, npass, bits);
+            hist_key(s_h, k1, npass, bits);
         }
     }
     __syncthreads();
-    uint32_t* h = a.hist_obj + (size_t)ti.seg * a.npass_obj * 256u;
-    for (uint32_t i = tid; i < 4 * 256; i += kWG) {
-        const uint32_t v = (&s_h[0][0])[i];
+    uint32_t* h = a.hist_obj + (size_t)ti.seg * npass * radix;
+    for (uint32_t i = tid; i < npass * radix; i += kWG) {
+        const uint32_t v = s_h[i / radix][i % radix];
         if (v) atomicAdd(&h[i], v);
     }
 }

@@ -317,46 +317,72 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
     return hipSuccess;
 }
 
+template <int BITS>
+static hipError_t kv32_items(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1,
+                             uint32_t* v1, int* which, hipStream_t st) {
+    switch (a.items) {
+        case 4: return run_passes<KU32, BITS, 4>(a, k0, v0, k1, v1, which, st);
+        case 8: return run_passes<KU32, BITS, 8>(a, k0, v0, k1, v1, which, st);
+        case 12: return run_passes<KU32, BITS, 12>(a, k0, v0, k1, v1, which, st);
+        case 16: return run_passes<KU32, BITS, 16>(a, k0, v0, k1, v1, which, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      int* which, hipStream_t st) {
-    if (a.bits != 8) return hipErrorInvalidValue;
+    switch (a.bits) {
+        case 6: return kv32_items<6>(a, k0, v0, k1, v1, which, st);
+        case 7: return kv32_items<7>(a, k0, v0, k1, v1, which, st);
+        case 8: return kv32_items<8>(a, k0, v0, k1, v1, which, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int BITS>
+static hipError_t pair_items(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st) {
+    int which = 0;
     switch (a.items) {
-        case 4: return run_passes<KU32, 8, 4>(a, k0, v0, k1, v1, which, st);
-        case 8: return run_passes<KU32, 8, 8>(a, k0, v0, k1, v1, which, st);
-        case 12: return run_passes<KU32, 8, 12>(a, k0, v0, k1, v1, which, st);
-        case 16: return run_passes<KU32, 8, 16>(a, k0, v0, k1, v1, which, st);
+        case 8: return run_passes<KPair, BITS, 8>(a, p0, nullptr, p1, nullptr, &which, st, out);
+        case 16: return run_passes<KPair, BITS, 16>(a, p0, nullptr, p1, nullptr, &which, st, out);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st) {
-    if (a.bits != 8) return hipErrorInvalidValue;
-    int which = 0;
-    switch (a.items) {
-        case 8: return run_passes<KPair, 8, 8>(a, p0, nullptr, p1, nullptr, &which, st, out);
-        case 16: return run_passes<KPair, 8, 16>(a, p0, nullptr, p1, nullptr, &which, st, out);
+    switch (a.bits) {
+        case 6: return pair_items<6>(a, p0, p1, out, st);
+        case 7: return pair_items<7>(a, p0, p1, out, st);
+        case 8: return pair_items<8>(a, p0, p1, out, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
                            int flip_float, hipStream_t st) {
-    if (bits != 8) return hipErrorInvalidValue;
     uint32_t grid = (n + kWG * 16 - 1) / (kWG * 16);
     if (grid == 0) grid = 1;
     if (grid > 2048) grid = 2048;
-    hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist,
-                       npass, 0, flip_float);
+    switch (bits) {
+        case 6: hipLaunchKernelGGL((k_hist<KU32, 6>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float); break;
+        case 7: hipLaunchKernelGGL((k_hist<KU32, 7>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float); break;
+        case 8: hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
-                            unsigned long long cap, uint32_t* hist, int npass, int bbits,
+                            unsigned long long cap, uint32_t* hist, int npass, int bbits, int bits,
                             hipStream_t st) {
     unsigned long long g = (cap + kWG * 16 - 1) / (kWG * 16);
     uint32_t grid = (uint32_t)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
-    hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist,
-                       npass, bbits, 0);
+    switch (bits) {
+        case 6: hipLaunchKernelGGL((k_hist<KPair, 6>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0); break;
+        case 7: hipLaunchKernelGGL((k_hist<KPair, 7>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0); break;
+        case 8: hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -82,8 +82,8 @@ typedef struct gsap_sap_desc {
     int32_t output_kind; /* GSAP_OUT_*                                            */
     float fixed_axis[3]; /* GSAP_AXIS_FIXED (need not be normalised)              */
     uint32_t flags;      /* GSAP_FLAG_*                                           */
-    int32_t digit_bits;  /* radix digit width (0 = tuned default; 4..8 allowed)   */
-    int32_t sort_items;  /* keys per thread in a sort tile (0 = tuned default)    */
+    int32_t digit_bits;  /* radix digit width of both sorts (0 = default 8; 6..8) */
+    int32_t sort_items;  /* keys per thread in a sort tile (0 = default 8)        */
 } gsap_sap_desc;
 
 typedef struct gsap_stats {
@@ -144,6 +144,37 @@ int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
  * Stage names: moments, project, sort, gather, sweep, pair_sort, finish. */
 int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages, int32_t* n_stages);
 int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable);
+
+/* ------------------------------------------------------------------ autotuner */
+/* Launch configuration of a SaP handle.  The reference's analogue is
+ * computeNumCTAs (sapDetection/radixsort.cpp:49-117), which sized the
+ * persistent grids of its pre-Fermi sort from the device; here every knob is
+ * measured on the actual update. */
+typedef struct gsap_tune {
+    int32_t sort_items;  /* object sort: keys per thread, 4/8/12/16 (tile = 256 x items) */
+    int32_t sort_bits;   /* object sort: digit width, 6/7/8                               */
+    int32_t pair_items;  /* pair sort: keys per thread, 8/16                              */
+    int32_t pair_bits;   /* pair sort: digit width, 6/7/8                                 */
+    int32_t pair_grid;   /* pair sort: persistent workgroups per pass, 256..16384         */
+    int32_t slabs;       /* 1: slab sweep (GSAP_FLAG_SLABS) for single-scene updates      */
+    float ms;            /* out: measured update time of this configuration (autotune)    */
+    int32_t from_cache;  /* out: 1 if the configuration came from the cache file          */
+} gsap_tune;
+int32_t gsap_sap_get_tune(gsap_sap* h, gsap_tune* out);
+int32_t gsap_sap_set_tune(gsap_sap* h, const gsap_tune* t);
+/* Time the update of this input under candidate configurations (a staged
+ * search: sweep kind, then object sort items x digit width, then pair sort
+ * items x digit width x grid), keep the fastest in the handle and return it.
+ * h_scene_offsets == NULL tunes the single-scene update of d_in[0..n)
+ * (d_group as in gsap_sap_collide); otherwise the batched update of n_scenes
+ * scenes (n and d_group are ignored).
+ * cache_path (optional) is a text file keyed by device, input and output
+ * kind, axis mode, flags and size bucket: a hit sets the stored configuration
+ * without measuring, a miss appends the measured one.  Allocates temporary
+ * device memory for the pairs it produces while measuring. */
+int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                          const uint32_t* d_group, const uint32_t* h_scene_offsets,
+                          uint32_t n_scenes, const char* cache_path, gsap_tune* out);
 
 /* ------------------------------------------------------------------ learned axis */
 /* No reference counterpart.  A 64-128-128-4 MLP (bf16 weights/activations,

@@ -1,0 +1,101 @@
+"""Autotuner and launch configurations: every digit width / tile / grid /
+sweep variant gives the identical pair list; the tuner returns a valid
+configuration, keeps it, and round-trips through its cache file."""
+import numpy as np
+import pytest
+
+import gsap_amd as G
+from gsap_amd import workloads as W
+from oracle import oracle as O
+from tests.gpu_util import pairs_np, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("sort_bits,pair_bits", [(6, 6), (7, 7), (8, 8), (6, 8), (8, 7)])
+@pytest.mark.parametrize("slabs", [0, 1])
+def test_digit_widths_same_pairs(dev, sort_bits, pair_bits, slabs):
+    b = W.small(20000, seed=11)
+    ref = O.brute(b)
+    for sort_items, pair_items, grid in ((4, 8, 512), (12, 16, 4096), (16, 8, 256)):
+        s = G.Sap()
+        s.set_tune(sort_items=sort_items, sort_bits=sort_bits, pair_items=pair_items,
+                   pair_bits=pair_bits, pair_grid=grid, slabs=slabs)
+        pairs, _, _ = s.collide(to_dev(b, dev))
+        assert np.array_equal(pairs_np(pairs), ref), (sort_items, pair_items, grid)
+
+
+def test_digit_widths_batched(dev):
+    import torch
+    boxes, off = W.scenes(24, seed=3, lo=500, hi=6000)
+    bd = to_dev(boxes, dev)
+    outs = []
+    for bits in (6, 7, 8):
+        s = G.Sap()
+        s.set_tune(sort_bits=bits, pair_bits=bits, sort_items=12, pair_items=8)
+        P, _ = s.collide_batched(bd, off, None, None)
+        pairs = torch.empty((P, 2), dtype=torch.int32, device=dev)
+        po = torch.empty(len(off), dtype=torch.int64, device=dev)
+        s.collide_batched(bd, off, pairs, po)
+        outs.append((pairs.cpu(), po.cpu()))
+    for p, o in outs[1:]:
+        assert torch.equal(p, outs[0][0]) and torch.equal(o, outs[0][1])
+
+
+def test_set_tune_rejects_invalid(dev):
+    s = G.Sap()
+    for bad in (dict(sort_items=5), dict(sort_bits=5), dict(pair_bits=9), dict(pair_items=12),
+                dict(pair_grid=100), dict(slabs=2)):
+        with pytest.raises(G.GsapError) as e:
+            s.set_tune(**bad)
+        assert e.value.code == G.ERR_INVALID_ARG
+    assert s.tune["sort_items"] == 8  # unchanged
+
+
+def test_autotune_single_and_cache(dev, tmp_path):
+    import torch
+    b = W.aniso(300_000, seed=2)
+    bd = to_dev(b, dev)
+    ref, _, _ = G.Sap().collide(bd)
+    cache = str(tmp_path / "tune.txt")
+    s = G.Sap()
+    t = s.autotune(bd, cache=cache)
+    assert t["from_cache"] == 0 and t["ms"] > 0
+    assert t["sort_items"] in (4, 8, 12, 16) and 6 <= t["sort_bits"] <= 8
+    assert s.tune["pair_grid"] == t["pair_grid"]
+    got, _, _ = s.collide(bd)
+    assert torch.equal(got, ref)
+    s2 = G.Sap()
+    t2 = s2.autotune(bd, cache=cache)
+    assert t2["from_cache"] == 1
+    for k in ("sort_items", "sort_bits", "pair_items", "pair_bits", "pair_grid", "slabs"):
+        assert t2[k] == t[k]
+    with open(cache) as f:
+        assert len(f.read().splitlines()) == 1
+
+
+def test_autotune_batched_and_grouped(dev):
+    import torch
+    boxes, off = W.scenes(48, seed=8, lo=1000, hi=20000)
+    bd = to_dev(boxes, dev)
+    base = G.Sap()
+    P, _ = base.collide_batched(bd, off, None, None)
+    p0 = torch.empty((P, 2), dtype=torch.int32, device=dev)
+    o0 = torch.empty(len(off), dtype=torch.int64, device=dev)
+    base.collide_batched(bd, off, p0, o0)
+    s = G.Sap()
+    t = s.autotune(bd, scene_offsets=off)
+    assert t["slabs"] == 0 and t["from_cache"] == 0
+    p1 = torch.empty_like(p0)
+    o1 = torch.empty_like(o0)
+    s.collide_batched(bd, off, p1, o1)
+    assert torch.equal(p0, p1) and torch.equal(o0, o1)
+    # a grouped, symmetric (particle-style) handle tunes with its group array
+    sph = np.concatenate([np.random.default_rng(1).uniform(0, 30, (20000, 3)),
+                          np.full((20000, 1), 0.5)], 1).astype(np.float32)
+    g = to_dev((np.arange(20000) // 27).astype(np.int32), dev)
+    h = G.Sap(input_kind=G.INPUT_SPHERE, output_kind=G.OUT_SYM_CSR, flags=G.FLAG_SKIP_SAME_GROUP)
+    ref, ent, _ = h.collide(to_dev(sph, dev), group=g)
+    h.autotune(to_dev(sph, dev), group=g)
+    got, ent2, _ = h.collide(to_dev(sph, dev), group=g)
+    assert torch.equal(got, ref) and torch.equal(ent, ent2)
