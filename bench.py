@@ -26,6 +26,33 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def stage_dict(t):
+    """Library stage times [moments+project, object_sort, gather, sweep,
+    pair_hist, one per pair-sort pass..., pair_tail, finish] -> (named
+    stages, per-pass array)."""
+    import numpy as np
+    t = np.asarray(t, np.float64)
+    passes = t[5:len(t) - 2]
+    st = {"moments+project": t[0], "object_sort": t[1], "gather": t[2], "sweep": t[3],
+          "pair_hist": t[4], "pair_passes": float(passes.sum()), "pair_tail": t[-2],
+          "finish": t[-1]}
+    st["pair_sort"] = st["pair_hist"] + st["pair_passes"] + st["pair_tail"]
+    return {k: float(v) for k, v in st.items()}, passes
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a
+    copy of known size), or None when none is committed."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
 def _time_ms(torch, fn, steps, warmup):
     """Mean ms per call of fn over `steps` calls after `warmup`, from events on
     torch's current stream (the stream every library call here launches on)."""
@@ -41,7 +68,7 @@ def _time_ms(torch, fn, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def bench_batched(G, W, torch, dev, ns, steps, warmup):
+def bench_batched(G, W, torch, dev, ns, steps, warmup, cache=None):
     """512 packed scenes (sizes log-uniform in [1024, 65536], seed 4) in one
     graph-replayed update, against the same kernels run scene by scene."""
     import numpy as np
@@ -49,7 +76,7 @@ def bench_batched(G, W, torch, dev, ns, steps, warmup):
     ntot = int(off[-1])
     bd = torch.from_numpy(boxes).to(dev)
     sb = G.Sap(max_n=ntot)
-    tune = sb.autotune(bd, scene_offsets=off)
+    tune = sb.autotune(bd, scene_offsets=off, cache=cache)
     P, st = sb.collide_batched(bd, off, None, None)
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     poff = torch.empty(ns + 1, dtype=torch.int64, device=dev)
@@ -115,7 +142,7 @@ def bench_learned(G, W, torch, dev, ns):
             "ms_batched_learned": round(ms["learned"], 4)}
 
 
-def _tune_particles(G, torch, sap, pp, npart):
+def _tune_particles(G, torch, sap, pp, npart, cache=None):
     """Autotune the particle broadphase on the step's own particle AABBs and
     body ids (device buffers owned by the rigid-body handle)."""
     import ctypes
@@ -130,10 +157,10 @@ def _tune_particles(G, torch, sap, pp, npart):
     r = pos[:, 3:4]
     aabb[:, :3] = pos[:, :3] - r
     aabb[:, 3:] = pos[:, :3] + r
-    return sap.autotune(aabb, group=body)
+    return sap.autotune(aabb, group=body, cache=cache)
 
 
-def bench_particles(G, W, torch, dev, nb, steps, with_cpu):
+def bench_particles(G, W, torch, dev, nb, steps, with_cpu, cache=None):
     """Rigid cubes (3x3x3 particles, radius 0.5, spacing 1) dropped into a
     256x256 walled box from heights U[0,200]: spring-dashpot k=1e4, c=50,
     g=-9.81, dt=1e-4, per-step SaP on the particle AABBs (same-body pairs
@@ -155,7 +182,7 @@ def bench_particles(G, W, torch, dev, nb, steps, with_cpu):
     dt = 1e-4
     prb.step(sap, dt)  # particle AABBs of the first step, to tune the broadphase on
     pp = prb.ptrs()
-    tune = _tune_particles(G, torch, sap, pp, npart)
+    tune = _tune_particles(G, torch, sap, pp, npart, cache)
     ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
     p = prb.ptrs()  # raises on pair-buffer overflow
     out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase "
@@ -205,6 +232,10 @@ def main():
     ap.add_argument("--learned-scenes", type=int, default=2048)
     ap.add_argument("--prb-bodies", type=int, default=65536)
     ap.add_argument("--prb-steps", type=int, default=1000)
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file (a hit skips the measurement of configurations)")
+    ap.add_argument("--only", choices=["batched", "learned", "particles"], default=None,
+                    help="run one extra section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -218,13 +249,23 @@ def main():
     from gsap_amd import workloads as W
 
     dev = torch.device("cuda:0")
+    if args.only:
+        if args.only == "batched":
+            r = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup, args.tune_cache)
+        elif args.only == "learned":
+            r = bench_learned(G, W, torch, dev, args.learned_scenes)
+        else:
+            r = bench_particles(G, W, torch, dev, args.prb_bodies, args.prb_steps,
+                                not args.no_cpu_baseline, args.tune_cache)
+        print(json.dumps({"section": args.only, **r}))
+        return
     boxes = W.aniso(args.n, seed=1)
     n = len(boxes)
     bd = torch.from_numpy(boxes).to(dev)
     mode = {"pca": G.AXIS_PCA, "coord": G.AXIS_COORD_MAXVAR, "learned": G.AXIS_LEARNED}[args.axis]
     slabs = not args.no_slabs
     sap = G.Sap(max_n=n, axis_mode=mode, flags=G.FLAG_SLABS if slabs else 0)
-    tune = sap.autotune(bd)  # measured launch configuration (tiles, digit widths, sweep kind)
+    tune = sap.autotune(bd, cache=args.tune_cache)  # measured tiles, digit widths, sweep kind
     P, st0 = sap.count(bd)  # pre-count pass sizes the 64-bit pair buffer
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     sap.set_timing(True)
@@ -246,28 +287,33 @@ def main():
     ms_per_step = ev0.elapsed_time(ev1) / args.steps
     value = n / (ms_per_step / 1e3)
 
-    names = ["moments+project", "object_sort", "gather", "sweep", "pair_sort", "finish"]
-    stages = dict(zip(names, (stage_sum / args.steps).tolist()))
-    # algorithmic bytes per launch (per-unit figures x units)
-    sort_passes = 4
-    alg = {
-        "sweep": 24.0 * n + 8.0 * P,                  # boxes streamed + pairs written
-        "object_sort": 16.0 * n * sort_passes,         # key+value read+write per pass
-        "pair_sort": 16.0 * P * 5 + 8.0 * P,           # 5 passes + histogram read
-    }
-    dom = max(("sweep", "pair_sort"), key=lambda k: stages[k])
-    launches = {"sweep": 1, "pair_sort": 6}[dom]
-    dom_ms = stages[dom] / launches
-    dom_bytes = alg[dom] / launches
-    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    stages, pass_ms = stage_dict(stage_sum / args.steps)
+    # Algorithmic bytes per launch: per-unit figures x the units one launch
+    # processes.  Object sort: 8 B key+value read and written per entry and
+    # pass (entries = 2n in slab mode); pair-sort pass: 8 B pair read and
+    # written, P pairs; sweep: 24 B per box streamed + 8 B per pair written.
+    tn = sap.tune
+    entries = 2 * n if tn["slabs"] else n
+    obj_passes = -(-32 // tn["sort_bits"])
+    alg_sweep = 24.0 * n + 8.0 * P
+    alg_pass = 16.0 * P
+    if stages["sweep"] >= pass_ms.mean() * len(pass_ms):
+        roof_k, roof_ms, roof_b = "k_sweep", stages["sweep"], alg_sweep
+    else:
+        roof_k, roof_ms, roof_b = "k_onesweep<pair>", float(pass_ms.mean()), alg_pass
+    achieved = roof_b / (roof_ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_sweep" if dom == "sweep" else "k_onesweep<pair>",
-            "bytes_per_launch": dom_bytes, "ms_per_launch": round(dom_ms, 4)}
-    sort_gbs = alg["object_sort"] / (stages["object_sort"] / 1e3) / 1e9
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(roof_k),
+            "kernel": roof_k, "bytes_per_launch": roof_b, "ms_per_launch": round(roof_ms, 4)}
+    obj_bytes = 16.0 * entries * obj_passes
+    sort_gbs = obj_bytes / (stages["object_sort"] / 1e3) / 1e9
     sort_roof = {"achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "ms": round(stages["object_sort"], 4),
-                 "bytes": alg["object_sort"], "passes": sort_passes}
+                 "bytes": obj_bytes, "passes": obj_passes, "entries": entries}
+    pair_gbs = alg_pass / (float(pass_ms.mean()) / 1e3) / 1e9
+    pair_roof = {"achieved": round(pair_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(pair_gbs / HBM_PEAK_GBS, 4), "ms_per_pass": round(float(pass_ms.mean()), 4),
+                 "passes": len(pass_ms), "bytes_per_pass": alg_pass}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -305,6 +351,7 @@ def main():
                    "sweep_tests": int(st.sweep_tests), "axis_used": [round(x, 6) for x in st.axis]},
         "roofline": roof,
         "sort_roofline": sort_roof,
+        "pair_sort_roofline": pair_roof,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "wall_s": round(wall, 3),
         "cpu_baseline": cpu,
@@ -318,13 +365,14 @@ def main():
                               "updates_per_s": round(n / (ms_plain / 1e3), 1),
                               "sweep_tests": int(stp.sweep_tests),
                               "stages_ms": {k: round(v, 4) for k, v in
-                                            zip(names, plain.stage_times())}}
+                                            stage_dict(np.array(plain.stage_times()))[0].items()}}
         plain.close()
     if not args.no_extras:
-        out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup)
+        out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup,
+                                       args.tune_cache)
         out["learned_axis"] = bench_learned(G, W, torch, dev, args.learned_scenes)
         out["particle_step"] = bench_particles(G, W, torch, dev, args.prb_bodies, args.prb_steps,
-                                               not args.no_cpu_baseline)
+                                               not args.no_cpu_baseline, args.tune_cache)
     print(json.dumps(out))
 
 

@@ -70,7 +70,8 @@ class Tune(ctypes.Structure):
     _fields_ = [("sort_items", ctypes.c_int32), ("sort_bits", ctypes.c_int32),
                 ("pair_items", ctypes.c_int32), ("pair_bits", ctypes.c_int32),
                 ("pair_grid", ctypes.c_int32), ("slabs", ctypes.c_int32),
-                ("ms", ctypes.c_float), ("from_cache", ctypes.c_int32)]
+                ("sweep_chunk", ctypes.c_int32), ("ms", ctypes.c_float),
+                ("from_cache", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -222,9 +223,9 @@ class Sap:
         check(lib().gsap_sap_set_timing(self.h, 1 if on else 0))
 
     def stage_times(self):
-        buf = (ctypes.c_float * 8)()
+        buf = (ctypes.c_float * 32)()
         n = ctypes.c_int32(0)
-        check(lib().gsap_sap_stage_times(self.h, buf, 8, ctypes.byref(n)))
+        check(lib().gsap_sap_stage_times(self.h, buf, 32, ctypes.byref(n)))
         return list(buf[:n.value])
 
     def count(self, boxes, group=None, stream=None):

@@ -70,6 +70,7 @@ struct gsap_sap {
     int pair_bits = 8;    // pair sort digit width
     int pair_grid = 2048; // pair sort persistent workgroups
     int slabs = 0;        // slab sweep for single-scene updates
+    int sweep_chunk = 0;  // sweep sub-tiles per XCD chunk
     float tune_ms = 0.0f;
     int tune_cached = 0;
     uint32_t cap_em = 1;  // entries per object the sort buffers are sized for
@@ -85,9 +86,10 @@ struct gsap_sap {
     uint64_t cap_pair_tiles = 0;
     uint64_t slot_cap = 0;  // capacity of each internal pair buffer
     bool timing = false;
-    hipEvent_t ev[8] = {};
+    static constexpr int kEv = 20;
+    hipEvent_t ev[kEv] = {};
     int nev = 0;
-    float stage_ms[8] = {};
+    float stage_ms[kEv] = {};
     // batched graph cache
     hipGraphExec_t gexec = nullptr;
     hipStream_t cap_stream = nullptr;
@@ -154,7 +156,7 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
     if (e == hipSuccess) e = hipMemset(h->ctrl.p, 0, sizeof(Ctrl));
     if (e == hipSuccess) e = grow(h->counts, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_counts, 8 * sizeof(unsigned long long), 0);
-    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
+    for (int i = 0; i < gsap_sap::kEv && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         std::string msg = std::string("gsap_sap_create: ") + hipGetErrorString(e);
@@ -177,7 +179,7 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
                    &h->vr_partials, &h->slabp};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < gsap_sap::kEv; ++i)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
@@ -260,7 +262,7 @@ static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg
 }
 
 static void mark(gsap_sap* h, hipStream_t st) {
-    if (h->timing && h->nev < 8) (void)hipEventRecord(h->ev[h->nev++], st);
+    if (h->timing && h->nev < gsap_sap::kEv) (void)hipEventRecord(h->ev[h->nev++], st);
 }
 
 // Enqueue one complete update (all stages) on st.  seg == nullptr: one scene.
@@ -366,6 +368,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     w.tile = tile;
     w.ntiles = ntiles_e;
     w.slabs = slabs ? 1 : 0;
+    w.xcd_chunk = (uint32_t)h->sweep_chunk;
     w.group = (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP) ? group : nullptr;
     w.ctrl = ctrl;
     w.kind = kind;
@@ -381,6 +384,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     if (!count_only) {
         HIPCHK(h, launch_pair_hist(t0p, &ctrl->slots, h->slot_cap, a.hist_pair, npass_pair, bbits,
                                    h->pair_bits, st));
+        mark(h, st);
         SortArgs p{};
         p.n_dev = &ctrl->slots;
         p.cap = h->slot_cap;
@@ -395,7 +399,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.grid = (uint32_t)h->pair_grid;
         p.last_n = &ctrl->num_pairs;
         p.last_cap = cap;
+        const bool pev = h->timing && h->nev + (int)npass_pair + 2 <= gsap_sap::kEv;
+        p.pass_ev = pev ? &h->ev[h->nev] : nullptr;
         HIPCHK(h, sort_pairs(p, t0p, t1p, outp, st));
+        if (pev) h->nev += (int)npass_pair;
         if (sym)
             HIPCHK(h, launch_entries(outp, &ctrl->num_pairs, cap, n, (uint32_t*)h->rowstart.p,
                                      (uint32_t*)h->has.p, (uint32_t*)h->block_sums.p, entries,
@@ -837,7 +844,9 @@ static bool valid_tune(const gsap_tune* t) {
     const bool pi = t->pair_items == 8 || t->pair_items == 16;
     return si && pi && t->sort_bits >= 6 && t->sort_bits <= 8 && t->pair_bits >= 6 &&
            t->pair_bits <= 8 && t->pair_grid >= 256 && t->pair_grid <= 16384 &&
-           (t->slabs == 0 || t->slabs == 1);
+           (t->slabs == 0 || t->slabs == 1) &&
+           (t->sweep_chunk == 0 || t->sweep_chunk == 4 || t->sweep_chunk == 16 ||
+            t->sweep_chunk == 64);
 }
 
 static gsap_tune tune_of(const gsap_sap* h) {
@@ -848,6 +857,7 @@ static gsap_tune tune_of(const gsap_sap* h) {
     t.pair_bits = h->pair_bits;
     t.pair_grid = h->pair_grid;
     t.slabs = h->slabs;
+    t.sweep_chunk = h->sweep_chunk;
     t.ms = h->tune_ms;
     t.from_cache = h->tune_cached;
     return t;
@@ -860,6 +870,7 @@ static void apply_tune(gsap_sap* h, const gsap_tune& t) {
     h->pair_bits = t.pair_bits;
     h->pair_grid = t.pair_grid;
     h->slabs = t.slabs;
+    h->sweep_chunk = t.sweep_chunk;
     if (h->gexec) {  // the captured batched update used the old configuration
         (void)hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;
@@ -878,7 +889,7 @@ extern "C" int32_t gsap_sap_set_tune(gsap_sap* h, const gsap_tune* t) {
     if (!valid_tune(t))
         return fail(h, GSAP_ERR_INVALID_ARG,
                     "gsap_sap_set_tune: items 4/8/12/16 and 8/16, bits 6..8, grid 256..16384, "
-                    "slabs 0/1");
+                    "slabs 0/1, sweep_chunk 0/4/16/64");
     apply_tune(h, *t);
     h->tune_ms = 0.0f;
     h->tune_cached = 0;
@@ -912,8 +923,9 @@ static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t)
     while (fgets(line, sizeof line, f)) {
         char k[300];
         gsap_tune c{};
-        if (sscanf(line, "%299s %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
-                   &c.pair_items, &c.pair_bits, &c.pair_grid, &c.slabs, &c.ms) == 8 &&
+        if (sscanf(line, "%299s %d %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
+                   &c.pair_items, &c.pair_bits, &c.pair_grid, &c.slabs, &c.sweep_chunk,
+                   &c.ms) == 9 &&
             key == k && valid_tune(&c)) {
             *t = c;  // the last matching line wins
             hit = true;
@@ -926,8 +938,8 @@ static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t)
 static void cache_append(const char* path, const std::string& key, const gsap_tune& t) {
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
-            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.ms);
+    fprintf(f, "%s %d %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
+            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.sweep_chunk, t.ms);
     fclose(f);
 }
 
@@ -1014,10 +1026,15 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
             best = t;
         }
     };
-    if (!batched) {  // stage 1: plain or slab sweep
-        gsap_tune t = best;
-        t.slabs = 1 - t.slabs;
-        consider(t);
+    {  // stage 1: plain or slab sweep (single scene) x sweep XCD chunking
+        const gsap_tune base = best;
+        for (int sl = 0; sl < (batched ? 1 : 2); ++sl)
+            for (int ch : {0, 4, 16, 64}) {
+                gsap_tune t = base;
+                if (!batched) t.slabs = sl;
+                t.sweep_chunk = ch;
+                if (t.slabs != base.slabs || ch != base.sweep_chunk) consider(t);
+            }
     }
     {  // stage 2: object sort tile and digit width
         const gsap_tune base = best;

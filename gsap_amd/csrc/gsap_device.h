@@ -89,6 +89,22 @@ __device__ __forceinline__ void st_rel(unsigned long long* p, unsigned long long
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup w runs on XCD
+// w mod 8) and each XCD has its own L2.  xcd_block hands each XCD chunks of
+// k consecutive logical blocks instead (chunks dealt round-robin, so uneven
+// work still spreads over all XCDs), so neighbouring blocks -- which read
+// overlapping candidate windows -- share one L2.  k == 0: identity.  A
+// bijection on [0, g): the tail that does not fill a round of chunks keeps
+// the identity mapping.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t w, uint32_t g, uint32_t k) {
+    if (k == 0) return w;
+    const uint32_t span = kXcds * k, full = (g / span) * span;
+    if (w >= full) return w;
+    const uint32_t x = w % kXcds, i = w / kXcds;  // the i-th workgroup on XCD x
+    return ((i / k) * kXcds + x) * k + i % k;
+}
+
 // Scene of a tile and the tile's object range.
 struct TileInfo {
     uint32_t seg, seg_start, seg_end, lo, hi;

@@ -118,6 +118,7 @@ struct SweepArgs {
     int sym;                 // emit both orientations
     int count_only;
     int slabs;               // entries carry kFirstBit in orig and their slab's first position in aux
+    uint32_t xcd_chunk;      // sub-tiles per XCD chunk (0: hardware round-robin)
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
@@ -177,6 +178,7 @@ struct SortArgs {
     uint32_t grid;                      // persistent grid size
     const unsigned long long* last_n;   // final pass writes only positions < min(*last_n,
     unsigned long long last_cap;        //   last_cap) (pairs: the caller's capacity)
+    hipEvent_t* pass_ev;                // if set: recorded after each pass (timing)
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.

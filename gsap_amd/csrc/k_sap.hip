@@ -564,7 +564,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     const Rec* __restrict__ R = reinterpret_cast<const Rec*>(a.rec);
 
     const uint32_t subs = a.tile / kSweepTile;
-    const uint32_t tile = blockIdx.x / subs, sub = blockIdx.x % subs;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk);
+    const uint32_t tile = blk / subs, sub = blk % subs;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const uint32_t lo = ti.lo + sub * kSweepTile;
     if (lo >= ti.hi) return;  // the whole workgroup

@@ -311,6 +311,7 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
                            a, (int)p);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
         cur ^= 1;
     }
     *which = cur;

@@ -140,8 +140,10 @@ int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
                                  uint64_t* d_scene_pair_offsets, uint64_t* h_num_pairs,
                                  gsap_stats* stats);
 
-/* Per-stage timing of the last gsap_sap_collide (ms, hipEvents), for tuning.
- * Stage names: moments, project, sort, gather, sweep, pair_sort, finish. */
+/* Per-stage timing of the last gsap_sap_collide with timing enabled (ms,
+ * hipEvents on the launch stream).  Stages, in order: moments+project,
+ * object_sort, gather, sweep, then (unless count-only) pair_hist, one entry
+ * per pair-sort pass, pair_tail (row starts / scene offsets); last: finish. */
 int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages, int32_t* n_stages);
 int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable);
 
@@ -157,6 +159,7 @@ typedef struct gsap_tune {
     int32_t pair_bits;   /* pair sort: digit width, 6/7/8                                 */
     int32_t pair_grid;   /* pair sort: persistent workgroups per pass, 256..16384         */
     int32_t slabs;       /* 1: slab sweep (GSAP_FLAG_SLABS) for single-scene updates      */
+    int32_t sweep_chunk; /* sweep: sub-tiles per XCD chunk, 0 (round-robin)/4/16/64       */
     float ms;            /* out: measured update time of this configuration (autotune)    */
     int32_t from_cache;  /* out: 1 if the configuration came from the cache file          */
 } gsap_tune;
