@@ -596,13 +596,35 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         const uint32_t ge = lane < nobj ? a.grp_end[g0 + lane] : 0u;  // ascending in lane order
         const uint32_t wend = uniform(wave_max_u32(ge));
         CRec* __restrict__ O = (CRec*)(a.grp) + g0;
+        // Slab mode: a candidate past an object's sweep end may belong to the
+        // next slab (and overlap it).  That only happens in a wave whose
+        // objects are not all in one slab (all windows then end inside that
+        // slab); such a "mixed" wave also tests candidate slab <= object slab.
+        bool mixed = false;
+        if constexpr (SLAB) {
+            uint32_t sb = 0u, sbn = 0u;
+            if (lane < nobj) {
+                sb = R[reinterpret_cast<const Rec*>(a.grp)[g0 + lane].aux].aux;
+                sbn = ~sb;
+            } else {
+                sb = 0u;
+                sbn = 0u;
+            }
+            // max of sb and of ~sb over the valid lanes: all equal iff max(sb) == ~max(~sb)
+            mixed = uniform(wave_max_u32(sb)) != ~uniform(wave_max_u32(sbn));
+        }
 
         uint32_t kfirst = 0;
+        // Candidate chunks are double-buffered: the next chunk's records are
+        // requested before this chunk's objects are tested.
+        Rec cn{};
+        if (lo + 1u < wend) cn = R[min(lo + 1u + lane, wend - 1u)];
         for (uint32_t t0 = lo + 1; t0 < wend; t0 += 64u) {
             // retire objects whose sweep end is at or before this chunk
             kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
             const uint32_t t = t0 + lane;
-            Rec c = R[t < wend ? t : wend - 1u];
+            Rec c = cn;
+            if (t0 + 64u < wend) cn = R[min(t + 64u, wend - 1u)];
             if (t >= wend) {  // a candidate that overlaps nothing
                 if constexpr (KIND == 0) {
                     c.lo[0] = c.lo[1] = c.lo[2] = __builtin_inff();
@@ -614,9 +636,14 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             const uint32_t corig = SLAB ? (c.orig & ~kFirstBit) : c.orig;
             uint32_t gc = 0;
             if (GROUP) gc = a.group[corig];
+            // lanes whose candidate entry is its box's first slab
+            const bool cf = SLAB && (c.orig & kFirstBit) != 0u;
 
-            auto process = [&](const Rec& o, auto masked_tag) {
+            // The exact test is the only per-lane arithmetic on the common path;
+            // the other conditions are lane masks (scalar ANDs) or uniform branches.
+            auto process = [&](const Rec& o, auto masked_tag, auto mixed_tag) {
                 constexpr bool MASKED = decltype(masked_tag)::value;
+                constexpr bool MIXED = decltype(mixed_tag)::value;
                 bool hit;
                 if constexpr (KIND == 0) {
                     // Inclusive AABB test (aabb.h:47-57) on the vector ALU only:
@@ -641,9 +668,12 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     hit = t1 < e1;
                 }
                 const uint32_t oorig = SLAB ? (o.orig & ~kFirstBit) : o.orig;
-                if (MASKED) hit = hit & (t > o.aux);
-                if (SLAB) hit = hit & (c.aux <= o.aux) & (((o.orig | c.orig) & kFirstBit) != 0u);
-                if (GROUP) hit = hit & (a.group[oorig] != gc);
+                if (MASKED) hit = hit && t > o.aux;  // candidates after the object
+                if constexpr (SLAB) {
+                    if (MIXED) hit = hit && c.aux <= o.aux;  // candidate in the object's slab
+                    if (!(o.orig & kFirstBit)) hit = hit && cf;  // one of them is a first slab
+                }
+                if (GROUP) hit = hit && a.group[oorig] != gc;
                 const unsigned long long m = ballot(hit);
                 if (m == 0ull) return;
                 const uint32_t cnt = (uint32_t)__popcll(m) * mult;
@@ -671,19 +701,26 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     __builtin_amdgcn_wave_barrier();
                 }
             };
-            auto objects = [&](auto masked_tag) {
+            auto objects = [&](auto masked_tag, auto mixed_tag) {
                 uint32_t k = kfirst;
                 for (; k + 4 <= nobj; k += 4) {  // four scalar loads in flight
                     const Rec o0 = O[k], o1 = O[k + 1], o2 = O[k + 2], o3 = O[k + 3];
-                    process(o0, masked_tag);
-                    process(o1, masked_tag);
-                    process(o2, masked_tag);
-                    process(o3, masked_tag);
+                    process(o0, masked_tag, mixed_tag);
+                    process(o1, masked_tag, mixed_tag);
+                    process(o2, masked_tag, mixed_tag);
+                    process(o3, masked_tag, mixed_tag);
                 }
-                for (; k < nobj; ++k) process(O[k], masked_tag);
+                for (; k < nobj; ++k) process(O[k], masked_tag, mixed_tag);
             };
-            if (t0 < hi) objects(std::true_type{});  // an object may sit at or after t
-            else objects(std::false_type{});
+            // an object may sit at or after the chunk start only in the first
+            // chunks of the sub-tile
+            if (SLAB && mixed) {
+                if (t0 < hi) objects(std::true_type{}, std::true_type{});
+                else objects(std::false_type{}, std::true_type{});
+            } else {
+                if (t0 < hi) objects(std::true_type{}, std::false_type{});
+                else objects(std::false_type{}, std::false_type{});
+            }
         }
         // candidate tests executed: every active object takes whole 64-lane chunks
         if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 63u) / 64u) * 64u;
