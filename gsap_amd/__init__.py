@@ -70,7 +70,8 @@ class Tune(ctypes.Structure):
     _fields_ = [("sort_items", ctypes.c_int32), ("sort_bits", ctypes.c_int32),
                 ("pair_items", ctypes.c_int32), ("pair_bits", ctypes.c_int32),
                 ("pair_grid", ctypes.c_int32), ("slabs", ctypes.c_int32),
-                ("sweep_chunk", ctypes.c_int32), ("ms", ctypes.c_float),
+                ("sweep_chunk", ctypes.c_int32), ("pair_regions", ctypes.c_int32),
+                ("ms", ctypes.c_float),
                 ("from_cache", ctypes.c_int32)]
 
     def as_dict(self):

@@ -71,6 +71,7 @@ struct gsap_sap {
     int pair_grid = 2048; // pair sort persistent workgroups
     int slabs = 0;        // slab sweep for single-scene updates
     int sweep_chunk = 0;  // sweep sub-tiles per XCD chunk
+    int pair_regions = 1; // sweep output regions (1 or kMaxRegions)
     float tune_ms = 0.0f;
     int tune_cached = 0;
     uint32_t cap_em = 1;  // entries per object the sort buffers are sized for
@@ -84,7 +85,9 @@ struct gsap_sap {
     unsigned long long* h_counts = nullptr;  // pinned
     uint32_t cap_n = 0, cap_tiles = 0, cap_seg = 0;
     uint64_t cap_pair_tiles = 0;
-    uint64_t slot_cap = 0;  // capacity of each internal pair buffer
+    uint64_t slot_cap = 0;       // pairs per sweep region (the caller's capacity)
+    uint64_t region_stride = 0;  // slots between region starts
+    uint64_t sort_cap = 0;       // capacity of the second pair buffer (padded total)
     bool timing = false;
     static constexpr int kEv = 20;
     hipEvent_t ev[kEv] = {};
@@ -240,19 +243,26 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     return GSAP_OK;
 }
 
-// Internal pair buffers: the sweep writes exactly the pairs it finds (dense),
-// so they hold the caller's capacity.
+// Internal pair buffers.  The sweep writes the pairs it finds densely into
+// pair_regions regions of the first buffer, each able to hold the caller's
+// capacity; the sort reads them as one array (regions padded to a whole tile)
+// and ping-pongs with the second buffer, which holds the padded total.
+static constexpr uint64_t kPairTileMax = 256 * 16;  // largest pair-sort tile
 static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg) {
     (void)n;
     (void)nseg;
-    const uint64_t slots = cap > 0 ? cap : 1;
-    if (slots >= (1ull << 32) - (1ull << 20))
+    const uint64_t rcap = cap > 0 ? cap : 1;
+    if (rcap >= (1ull << 32) - (1ull << 20))
         return fail(h, GSAP_ERR_UNSUPPORTED, "pair capacity too large (pair indices are 32-bit)");
-    if (slots > h->slot_cap) {
-        for (int i = 0; i < 2; ++i) HIPCHK(h, grow(h->ptmp[i], (size_t)slots * 8));
-        h->slot_cap = slots;
-    }
-    const uint64_t tiles = (h->slot_cap + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
+    const uint64_t nreg = (uint64_t)h->pair_regions;
+    const uint64_t stride = (rcap + kPairTileMax - 1) / kPairTileMax * kPairTileMax + kPairTileMax;
+    const uint64_t sorted = rcap + (nreg + 1) * kPairTileMax;  // padded total, upper bound
+    HIPCHK(h, grow(h->ptmp[0], (size_t)(nreg * stride) * 8));
+    HIPCHK(h, grow(h->ptmp[1], (size_t)sorted * 8));
+    h->slot_cap = rcap;
+    h->region_stride = stride;
+    h->sort_cap = sorted;
+    const uint64_t tiles = (sorted + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
     if (tiles > h->cap_pair_tiles) {
         HIPCHK(h, grow(h->lb_pair, (size_t)tiles * 256 * 8));
         HIPCHK(h, hipMemset(h->lb_pair.p, 0, (size_t)tiles * 256 * 8));
@@ -379,15 +389,23 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     uint2* t1p = (uint2*)h->ptmp[1].p;
     w.out = count_only ? nullptr : t0p;
     w.slot_cap = count_only ? 0 : h->slot_cap;
+    w.region_stride = h->region_stride;
+    w.nregions = count_only ? 1u : (uint32_t)h->pair_regions;
     HIPCHK(h, launch_sweep(w, st));
     mark(h, st);
     if (!count_only) {
-        HIPCHK(h, launch_pair_hist(t0p, &ctrl->slots, h->slot_cap, a.hist_pair, npass_pair, bbits,
-                                   h->pair_bits, st));
+        const uint32_t nreg = (uint32_t)h->pair_regions;
+        HIPCHK(h, launch_pair_regions(ctrl, t0p, h->slot_cap, h->region_stride, nreg,
+                                      sort_tile(h->pair_items), st));
+        HIPCHK(h, launch_pair_hist(t0p, &ctrl->sort_n, h->sort_cap, a.hist_pair, npass_pair, bbits,
+                                   h->pair_bits, ctrl->region_start, h->region_stride, nreg, st));
         mark(h, st);
         SortArgs p{};
-        p.n_dev = &ctrl->slots;
-        p.cap = h->slot_cap;
+        p.n_dev = &ctrl->sort_n;
+        p.cap = h->sort_cap;
+        p.region_start = ctrl->region_start;
+        p.region_stride = h->region_stride;
+        p.nregions = nreg;
         p.hist = a.hist_pair;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
@@ -846,7 +864,8 @@ static bool valid_tune(const gsap_tune* t) {
            t->pair_bits <= 8 && t->pair_grid >= 256 && t->pair_grid <= 16384 &&
            (t->slabs == 0 || t->slabs == 1) &&
            (t->sweep_chunk == 0 || t->sweep_chunk == 4 || t->sweep_chunk == 16 ||
-            t->sweep_chunk == 64);
+            t->sweep_chunk == 64) &&
+           (t->pair_regions == 1 || t->pair_regions == kMaxRegions);
 }
 
 static gsap_tune tune_of(const gsap_sap* h) {
@@ -858,6 +877,7 @@ static gsap_tune tune_of(const gsap_sap* h) {
     t.pair_grid = h->pair_grid;
     t.slabs = h->slabs;
     t.sweep_chunk = h->sweep_chunk;
+    t.pair_regions = h->pair_regions;
     t.ms = h->tune_ms;
     t.from_cache = h->tune_cached;
     return t;
@@ -871,6 +891,7 @@ static void apply_tune(gsap_sap* h, const gsap_tune& t) {
     h->pair_grid = t.pair_grid;
     h->slabs = t.slabs;
     h->sweep_chunk = t.sweep_chunk;
+    h->pair_regions = t.pair_regions;
     if (h->gexec) {  // the captured batched update used the old configuration
         (void)hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;
@@ -889,7 +910,7 @@ extern "C" int32_t gsap_sap_set_tune(gsap_sap* h, const gsap_tune* t) {
     if (!valid_tune(t))
         return fail(h, GSAP_ERR_INVALID_ARG,
                     "gsap_sap_set_tune: items 4/8/12/16 and 8/16, bits 6..8, grid 256..16384, "
-                    "slabs 0/1, sweep_chunk 0/4/16/64");
+                    "slabs 0/1, sweep_chunk 0/4/16/64, pair_regions 1/8");
     apply_tune(h, *t);
     h->tune_ms = 0.0f;
     h->tune_cached = 0;
@@ -923,9 +944,9 @@ static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t)
     while (fgets(line, sizeof line, f)) {
         char k[300];
         gsap_tune c{};
-        if (sscanf(line, "%299s %d %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
+        if (sscanf(line, "%299s %d %d %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
                    &c.pair_items, &c.pair_bits, &c.pair_grid, &c.slabs, &c.sweep_chunk,
-                   &c.ms) == 9 &&
+                   &c.pair_regions, &c.ms) == 10 &&
             key == k && valid_tune(&c)) {
             *t = c;  // the last matching line wins
             hit = true;
@@ -938,8 +959,8 @@ static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t)
 static void cache_append(const char* path, const std::string& key, const gsap_tune& t) {
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
-            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.sweep_chunk, t.ms);
+    fprintf(f, "%s %d %d %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
+            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.sweep_chunk, t.pair_regions, t.ms);
     fclose(f);
 }
 
@@ -1026,15 +1047,18 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
             best = t;
         }
     };
-    {  // stage 1: plain or slab sweep (single scene) x sweep XCD chunking
+    {  // stage 1: plain or slab sweep (single scene) x sweep XCD chunking x pair regions
         const gsap_tune base = best;
         for (int sl = 0; sl < (batched ? 1 : 2); ++sl)
-            for (int ch : {0, 4, 16, 64}) {
-                gsap_tune t = base;
-                if (!batched) t.slabs = sl;
-                t.sweep_chunk = ch;
-                if (t.slabs != base.slabs || ch != base.sweep_chunk) consider(t);
-            }
+            for (int ch : {0, 4, 16, 64})
+                for (int rg : {1, kMaxRegions}) {
+                    gsap_tune t = base;
+                    if (!batched) t.slabs = sl;
+                    t.sweep_chunk = ch;
+                    t.pair_regions = rg;
+                    if (t.slabs != base.slabs || ch != base.sweep_chunk || rg != base.pair_regions)
+                        consider(t);
+                }
     }
     {  // stage 2: object sort tile and digit width
         const gsap_tune base = best;

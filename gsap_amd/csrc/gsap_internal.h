@@ -43,12 +43,15 @@ struct alignas(16) RecSphere {
 
 // Device control block of one SaP handle.  Zeroed once at allocation; the
 // first kernel of every update resets what the update accumulates.
+constexpr int kMaxRegions = 8;  // pair regions (one allocation counter each)
 struct Ctrl {
     unsigned long long num_pairs;    // pairs (or directed pairs) emitted by the sweep
     unsigned long long tests;        // sweep-test count
     unsigned long long num_entries;  // SYM_CSR row starts
     unsigned long long exec_tests;   // candidate tests executed (wave granularity)
-    unsigned long long slots;        // pair-buffer slots reserved by the sweep
+    unsigned long long region_fill[kMaxRegions];       // pairs written to each region
+    unsigned long long region_start[kMaxRegions + 1];  // logical start of each region
+    unsigned long long sort_n;       // pairs (+ tile padding) the pair sort reads
     uint32_t flags;                  // 1 = capacity overflow, 2 = non-finite input
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
@@ -111,8 +114,10 @@ struct SweepArgs {
     uint32_t tile;
     uint32_t ntiles;
     const uint32_t* group;   // per original index, or null
-    uint2* out;              // pair buffer (dense, unsorted)
-    unsigned long long slot_cap;
+    uint2* out;              // pair regions (each dense, unsorted)
+    unsigned long long slot_cap;   // pairs per region
+    unsigned long long region_stride;  // slots between region starts
+    uint32_t nregions;       // 1 or kMaxRegions (workgroup w uses region w mod nregions)
     Ctrl* ctrl;
     int kind;
     int sym;                 // emit both orientations
@@ -124,7 +129,14 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
 hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
                             unsigned long long cap, uint32_t* hist, int npass, int bbits, int bits,
-                            hipStream_t st);
+                            const unsigned long long* region_start,
+                            unsigned long long region_stride, uint32_t nregions, hipStream_t st);
+// Lay the sweep's pair regions out for the sort: clamp each fill to the
+// region capacity, pad it to a whole tile with sentinel pairs (sorted last),
+// and write the logical region starts and the padded total (ctrl->sort_n).
+hipError_t launch_pair_regions(Ctrl* ctrl, uint2* regions, unsigned long long region_cap,
+                               unsigned long long region_stride, uint32_t nregions, uint32_t tile,
+                               hipStream_t st);
 hipError_t launch_entries(const uint2* sorted_pairs, const unsigned long long* n_dev,
                           unsigned long long cap, uint32_t nobj, uint32_t* rowstart,
                           uint32_t* has, uint32_t* block_sums, int32_t* entries, Ctrl* ctrl,
@@ -179,6 +191,11 @@ struct SortArgs {
     const unsigned long long* last_n;   // final pass writes only positions < min(*last_n,
     unsigned long long last_cap;        //   last_cap) (pairs: the caller's capacity)
     hipEvent_t* pass_ev;                // if set: recorded after each pass (timing)
+    // pair sort, first pass: the input is nregions tile-aligned regions of
+    // region_stride slots, logically concatenated at region_start[]
+    const unsigned long long* region_start;
+    unsigned long long region_stride;
+    uint32_t nregions;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.

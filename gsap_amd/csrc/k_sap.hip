@@ -583,9 +583,14 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     uint32_t fill = 0;    // staged pairs of this wave
     unsigned long long ex = 0;
 
+    // Pair region of this workgroup: with several regions, the workgroups of
+    // each XCD (dealt round-robin) share one allocation counter instead of the
+    // whole grid contending for a single one.
+    const uint32_t region = a.nregions > 1 ? blockIdx.x % a.nregions : 0u;
+    uint2* const rout = a.out + (size_t)region * a.region_stride;
     auto flush = [&](uint32_t cnt) {  // cnt > 0, wave-uniform
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&a.ctrl->slots, (unsigned long long)cnt);
+        if (lane == 0) base = atomicAdd(&a.ctrl->region_fill[region], (unsigned long long)cnt);
         base = __builtin_amdgcn_readfirstlane((uint32_t)base) |
                ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32);
         return base;
@@ -682,7 +687,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     if (fill + cnt > kStage) {  // stage full: write it out
                         const unsigned long long base = flush(fill);
                         for (uint32_t i = lane; i < fill; i += 64u)
-                            if (base + i < cap) a.out[base + i] = stage[i];
+                            if (base + i < cap) rout[base + i] = stage[i];
                         fill = 0;
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -759,7 +764,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             for (int v = 0; v < kWaves; ++v) {
                 const uint32_t f = s_fill[v];
                 for (uint32_t i = lane; i < f; i += 64u)
-                    if (base + i < cap) a.out[base + i] = s_stage[v][i];
+                    if (base + i < cap) rout[base + i] = s_stage[v][i];
                 base += f;
             }
         }
@@ -835,9 +840,37 @@ __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
         ctrl->num_entries = 0;
         ctrl->tests = 0;
         ctrl->exec_tests = 0;
-        ctrl->slots = 0;
+        for (int r = 0; r < kMaxRegions; ++r) ctrl->region_fill[r] = 0;
         ctrl->flags = 0;
     }
+}
+
+// One workgroup: region layout for the pair sort (see launch_pair_regions).
+__global__ __launch_bounds__(kWG) void k_pair_regions(Ctrl* ctrl, uint2* regions,
+                                                      unsigned long long region_cap,
+                                                      unsigned long long region_stride,
+                                                      uint32_t nregions, uint32_t tile) {
+    __shared__ unsigned long long s_fill[kMaxRegions], s_pad[kMaxRegions];
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        unsigned long long run = 0;
+        for (uint32_t r = 0; r < nregions; ++r) {
+            const unsigned long long f = ctrl->region_fill[r] < region_cap ? ctrl->region_fill[r]
+                                                                           : region_cap;
+            // a single region needs no padding: it is the whole, contiguous input
+            const unsigned long long p = nregions > 1 ? (f + tile - 1) / tile * tile : f;
+            s_fill[r] = f;
+            s_pad[r] = p;
+            ctrl->region_start[r] = run;
+            run += p;
+        }
+        for (uint32_t r = nregions; r <= (uint32_t)kMaxRegions; ++r) ctrl->region_start[r] = run;
+        ctrl->sort_n = run;
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < nregions; ++r)  // sentinel pairs sort after every real pair
+        for (unsigned long long k = s_fill[r] + tid; k < s_pad[r]; k += kWG)
+            regions[(size_t)r * region_stride + k] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
 }
 
 // Per-scene ranges of the sorted (scene-major) canonical pair list.
@@ -926,6 +959,14 @@ hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
                          hipStream_t st) {
     (void)sym;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, ctrl, cap, d_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_regions(Ctrl* ctrl, uint2* regions, unsigned long long region_cap,
+                               unsigned long long region_stride, uint32_t nregions, uint32_t tile,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_pair_regions, dim3(1), dim3(kWG), 0, st, ctrl, regions, region_cap,
+                       region_stride, nregions, tile);
     return hipGetLastError();
 }
 

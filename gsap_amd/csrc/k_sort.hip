@@ -36,6 +36,19 @@ struct KPair {
     }
 };
 
+// Offset from a logical position of the region concatenation to its slot in
+// the region buffer (regions are tile-aligned, so a tile or a 64-key chunk
+// lies in one region): the last region starting at or before `lo`.
+__device__ __forceinline__ unsigned long long region_offset(const unsigned long long* start,
+                                                            unsigned long long stride,
+                                                            uint32_t nregions,
+                                                            unsigned long long lo) {
+    uint32_t r = 0;
+    for (uint32_t q = 1; q < nregions; ++q)
+        if (start[q] <= lo) r = q;
+    return (unsigned long long)r * stride - start[r];
+}
+
 __device__ __forceinline__ unsigned long long lb_word(uint32_t tag, bool inc, uint32_t v) {
     return ((unsigned long long)tag << 33) | (inc ? (1ull << 32) : 0ull) | (unsigned long long)v;
 }
@@ -92,11 +105,14 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
         K key[ITEMS];
         uint32_t val[ITEMS], rank[ITEMS], dig[ITEMS];
         const uint32_t base = ti.lo + wave * 64u * ITEMS + lane;
+        const K* src = kin;
+        if (pass == 0 && a.nregions > 1)  // first pass of a pair sort reads the sweep's regions
+            src = kin + region_offset(a.region_start, a.region_stride, a.nregions, ti.lo);
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t idx = base + 64u * k;
             if (idx < ti.hi) {
-                key[k] = kin[idx];
+                key[k] = src[idx];
                 if (KT::kHasVals) val[k] = vin[idx];
             }
         }
@@ -236,7 +252,10 @@ template <class KT, int BITS>
 __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__ keys, uint32_t n_host,
                                               const unsigned long long* __restrict__ n_dev,
                                               unsigned long long cap, uint32_t* __restrict__ hist,
-                                              int npass, int bbits, int flip_float) {
+                                              int npass, int bbits, int flip_float,
+                                              const unsigned long long* region_start,
+                                              unsigned long long region_stride,
+                                              uint32_t nregions) {
     constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u;
     __shared__ uint32_t s_h[kMaxPasses * RADIX];
     for (uint32_t i = threadIdx.x; i < (uint32_t)npass * RADIX; i += kWG) s_h[i] = 0u;
@@ -251,8 +270,10 @@ __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__
     for (uint32_t b0 = blockIdx.x * kWG + (threadIdx.x & ~63u); b0 < n; b0 += gridDim.x * kWG) {
         const uint32_t i = b0 + lane;
         const bool valid = i < n;
+        const typename KT::T* src = keys;
+        if (nregions > 1) src = keys + region_offset(region_start, region_stride, nregions, b0);
         typename KT::T k{};
-        if (valid) k = keys[i];
+        if (valid) k = src[i];
         if constexpr (KT::kHasVals) {
             if (flip_float) k = flip_f32(__uint_as_float(k));
         }
@@ -365,9 +386,9 @@ hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int
     if (grid == 0) grid = 1;
     if (grid > 2048) grid = 2048;
     switch (bits) {
-        case 6: hipLaunchKernelGGL((k_hist<KU32, 6>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float); break;
-        case 7: hipLaunchKernelGGL((k_hist<KU32, 7>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float); break;
-        case 8: hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float); break;
+        case 6: hipLaunchKernelGGL((k_hist<KU32, 6>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, nullptr, 0ull, 1u); break;
+        case 7: hipLaunchKernelGGL((k_hist<KU32, 7>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, nullptr, 0ull, 1u); break;
+        case 8: hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, nullptr, 0ull, 1u); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -375,13 +396,14 @@ hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int
 
 hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
                             unsigned long long cap, uint32_t* hist, int npass, int bbits, int bits,
+                            const unsigned long long* rs, unsigned long long rstride, uint32_t nreg,
                             hipStream_t st) {
     unsigned long long g = (cap + kWG * 16 - 1) / (kWG * 16);
     uint32_t grid = (uint32_t)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
     switch (bits) {
-        case 6: hipLaunchKernelGGL((k_hist<KPair, 6>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0); break;
-        case 7: hipLaunchKernelGGL((k_hist<KPair, 7>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0); break;
-        case 8: hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0); break;
+        case 6: hipLaunchKernelGGL((k_hist<KPair, 6>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, rs, rstride, nreg); break;
+        case 7: hipLaunchKernelGGL((k_hist<KPair, 7>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, rs, rstride, nreg); break;
+        case 8: hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, rs, rstride, nreg); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -160,6 +160,8 @@ typedef struct gsap_tune {
     int32_t pair_grid;   /* pair sort: persistent workgroups per pass, 256..16384         */
     int32_t slabs;       /* 1: slab sweep (GSAP_FLAG_SLABS) for single-scene updates      */
     int32_t sweep_chunk; /* sweep: sub-tiles per XCD chunk, 0 (round-robin)/4/16/64       */
+    int32_t pair_regions;/* sweep output regions with one allocation counter each, 1 or 8;
+                          * 8 holds 8x the pair capacity in device memory              */
     float ms;            /* out: measured update time of this configuration (autotune)    */
     int32_t from_cache;  /* out: 1 if the configuration came from the cache file          */
 } gsap_tune;

@@ -17,11 +17,12 @@ pytestmark = pytest.mark.gpu
 def test_digit_widths_same_pairs(dev, sort_bits, pair_bits, slabs):
     b = W.small(20000, seed=11)
     ref = O.brute(b)
-    for sort_items, pair_items, grid, chunk in ((4, 8, 512, 0), (12, 16, 4096, 16),
-                                                (16, 8, 256, 64), (8, 16, 1024, 4)):
+    for sort_items, pair_items, grid, chunk, reg in ((4, 8, 512, 0, 1), (12, 16, 4096, 16, 8),
+                                                     (16, 8, 256, 64, 8), (8, 16, 1024, 4, 1)):
         s = G.Sap()
         s.set_tune(sort_items=sort_items, sort_bits=sort_bits, pair_items=pair_items,
-                   pair_bits=pair_bits, pair_grid=grid, slabs=slabs, sweep_chunk=chunk)
+                   pair_bits=pair_bits, pair_grid=grid, slabs=slabs, sweep_chunk=chunk,
+                   pair_regions=reg)
         pairs, _, _ = s.collide(to_dev(b, dev))
         assert np.array_equal(pairs_np(pairs), ref), (sort_items, pair_items, grid)
 
@@ -33,7 +34,8 @@ def test_digit_widths_batched(dev):
     outs = []
     for bits in (6, 7, 8):
         s = G.Sap()
-        s.set_tune(sort_bits=bits, pair_bits=bits, sort_items=12, pair_items=8)
+        s.set_tune(sort_bits=bits, pair_bits=bits, sort_items=12, pair_items=8,
+                   pair_regions=8 if bits == 7 else 1)
         P, _ = s.collide_batched(bd, off, None, None)
         pairs = torch.empty((P, 2), dtype=torch.int32, device=dev)
         po = torch.empty(len(off), dtype=torch.int64, device=dev)
@@ -46,7 +48,7 @@ def test_digit_widths_batched(dev):
 def test_set_tune_rejects_invalid(dev):
     s = G.Sap()
     for bad in (dict(sort_items=5), dict(sort_bits=5), dict(pair_bits=9), dict(pair_items=12),
-                dict(pair_grid=100), dict(slabs=2), dict(sweep_chunk=5)):
+                dict(pair_grid=100), dict(slabs=2), dict(sweep_chunk=5), dict(pair_regions=4)):
         with pytest.raises(G.GsapError) as e:
             s.set_tune(**bad)
         assert e.value.code == G.ERR_INVALID_ARG
@@ -70,7 +72,7 @@ def test_autotune_single_and_cache(dev, tmp_path):
     t2 = s2.autotune(bd, cache=cache)
     assert t2["from_cache"] == 1
     for k in ("sort_items", "sort_bits", "pair_items", "pair_bits", "pair_grid", "slabs",
-              "sweep_chunk"):
+              "sweep_chunk", "pair_regions"):
         assert t2[k] == t[k]
     with open(cache) as f:
         assert len(f.read().splitlines()) == 1
