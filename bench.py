@@ -324,7 +324,12 @@ def main():
                               sample_stride=stride, cap=max(16, 4 * P // stride + 1024))
         setup, sweep, canon = O.last_times()
         est = setup + stride * (sweep + canon)
+        # the reference's own sweep work on this input: its 4-cell pass plus
+        # cross pass (window sizes only), for comparison with sweep_tests
+        _, ref_work = O.sap_ref_cells(boxes, count_only=True)
         cpu = {"value": round(n / est, 1), "unit": "updates/s", "cores": 1, "kind": "port",
+               "reference_sweep_tests": ref_work["cell_tests"] + ref_work["cross_tests"],
+               "reference_sweep_work": ref_work,
                "sample": f"axis+projection+radix sort of all {n} boxes ({setup:.2f}s), sweep and "
                          f"canonical sort of every {stride}th sorted object ({sweep + canon:.2f}s), "
                          f"extrapolated x{stride}: {est:.1f}s per update; "

@@ -399,6 +399,129 @@ int64_t or_sap(const float* boxes, int kind, size_t n, int axis_mode, const floa
     return (int64_t)P;
 }
 
+/* One sweep over a subset of boxes with given lower / upper keys (float,
+ * compared through the order-preserving flip): per sorted box, candidates up
+ * to the last lower key <= its upper key.  Adds the window sizes to *tests and,
+ * unless count_only, the overlapping pairs (min, max original index). */
+static void sweep_subset(const float* boxes, int kind, const uint32_t* ids, size_t m,
+                         const float* lok, const float* hik, int count_only, uint64_t* tests,
+                         uint32_t* pairs, uint64_t cap, uint64_t* P) {
+    uint32_t* keys = (uint32_t*)malloc((m ? m : 1) * 4);
+    uint32_t* perm = (uint32_t*)malloc((m ? m : 1) * 4);
+    for (size_t k = 0; k < m; ++k) { keys[k] = or_flip(lok[k]); perm[k] = (uint32_t)k; }
+    or_radix_sort_kv(keys, perm, m, 8);
+    for (size_t s = 0; s < m; ++s) {
+        const uint32_t a = perm[s];
+        const size_t end = upper_bound_u32(keys, m, or_flip(hik[a]));
+        if (end > s + 1) *tests += end - s - 1;
+        if (count_only) continue;
+        for (size_t t = s + 1; t < end; ++t) {
+            const uint32_t i = ids[a], j = ids[perm[t]];
+            if (overlap(boxes, kind, i, j)) {
+                if (*P < cap) {
+                    pairs[2 * *P] = i < j ? i : j;
+                    pairs[2 * *P + 1] = i < j ? j : i;
+                }
+                ++*P;
+            }
+        }
+    }
+    free(keys);
+    free(perm);
+}
+
+/* The reference's cell subdivision and cross pass, restated for work counts
+ * (Sap.cpp:211-295; projectSubMultiple SapKernel.cu:150-213;
+ * optimalAxisMultiple :2634-2678 -> cullCrossSphereMultiple :556-635,
+ * cullRelevantSphereMultiple :967-1059, setRelevantSphere :475-491;
+ * sweepPrunSphereSubElong / SubCross :2364-2517).
+ *   axis: the coordinate axis of largest variance (axisVecMultiple); the two
+ *   others a1 < a2 carry the split planes at 0 (mPoint1 = mPoint2 = 0).
+ *   cell q of a box by the signs of its centre on a1, a2 (1: both >= 0, 2: a1
+ *   < 0 <= a2, 3: a1 >= 0 > a2, 0: both < 0); cell pass keys lo + q*interval,
+ *   hi + q*interval with interval larger than the world extent (the reference
+ *   carries the previous frame's extent), so windows stay in their cell.
+ *   maxR: the largest half-extent (radius) of the boxes straddling a plane;
+ *   relevant boxes: within maxR of a plane (hi >= -maxR and lo <= maxR on a1
+ *   or a2); cross pass: the relevant boxes swept again without cell offsets.
+ *   The union of both passes, deduplicated, is the pair set.  For spheres
+ *   lo/hi = c -/+ r, as in the reference; for AABBs the box itself.
+ * Returns the number of unique pairs (canonical in `pairs` when <= cap);
+ * stats: tests of the cell pass, of the cross pass, relevant boxes, pairs
+ * before deduplication.  count_only: window sizes only (no pairs). */
+int64_t or_sap_ref_cells(const float* boxes, int kind, size_t n, int count_only, uint32_t* pairs,
+                         uint64_t cap, uint64_t stats[4]) {
+    stats[0] = stats[1] = stats[2] = stats[3] = 0;
+    if (n == 0) return 0;
+    double mom[9];
+    or_moments(boxes, kind, n, mom);
+    float u[3];
+    int lam[3];
+    or_axis_coord_maxvar(mom, n, u, lam);
+    const int a0 = lam[0];
+    const int a1 = a0 == 0 ? 1 : 0, a2 = a0 == 2 ? 1 : 2;
+    float wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < n; ++i) {
+        float lo[3], hi[3];
+        box_lohi(boxes, kind, i, lo, hi);
+        for (int k = 0; k < 3; ++k) {
+            if (lo[k] < wlo[k]) wlo[k] = lo[k];
+            if (hi[k] > whi[k]) whi[k] = hi[k];
+        }
+    }
+    float interval = 0.0f;
+    for (int k = 0; k < 3; ++k)
+        if (whi[k] - wlo[k] > interval) interval = whi[k] - wlo[k];
+    interval = interval * 2.0f + 1.0f;
+    uint32_t* ids = (uint32_t*)malloc(n * 4);
+    float* lok = (float*)malloc(n * 4);
+    float* hik = (float*)malloc(n * 4);
+    float maxr = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        float lo[3], hi[3];
+        box_lohi(boxes, kind, i, lo, hi);
+        const float c1 = 0.5f * (lo[a1] + hi[a1]), c2 = 0.5f * (lo[a2] + hi[a2]);
+        const int q = (c1 >= 0.0f && c2 >= 0.0f) ? 1 : (c1 < 0.0f && c2 >= 0.0f) ? 2
+                    : (c1 >= 0.0f && c2 < 0.0f) ? 3 : 0;
+        ids[i] = (uint32_t)i;
+        lok[i] = lo[a0] + (float)q * interval;
+        hik[i] = hi[a0] + (float)q * interval;
+        const int st1 = hi[a1] >= 0.0f && lo[a1] <= 0.0f, st2 = hi[a2] >= 0.0f && lo[a2] <= 0.0f;
+        if (st1 || st2) {
+            const float h1 = 0.5f * (hi[a1] - lo[a1]), h2 = 0.5f * (hi[a2] - lo[a2]);
+            const float h = h1 > h2 ? h1 : h2;
+            if (h > maxr) maxr = h;
+        }
+    }
+    uint64_t P = 0;
+    sweep_subset(boxes, kind, ids, n, lok, hik, count_only, &stats[0], pairs, cap, &P);
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        float lo[3], hi[3];
+        box_lohi(boxes, kind, i, lo, hi);
+        if ((hi[a1] >= -maxr && lo[a1] <= maxr) || (hi[a2] >= -maxr && lo[a2] <= maxr)) {
+            ids[m] = (uint32_t)i;
+            lok[m] = lo[a0];
+            hik[m] = hi[a0];
+            ++m;
+        }
+    }
+    stats[2] = m;
+    sweep_subset(boxes, kind, ids, m, lok, hik, count_only, &stats[1], pairs, cap, &P);
+    stats[3] = P;
+    free(ids); free(lok); free(hik);
+    if (count_only || P > cap) return (int64_t)P;
+    or_canon_sort(pairs, P);
+    uint64_t w = 0;  /* dedupe: pairs found by both passes */
+    for (uint64_t k = 0; k < P; ++k)
+        if (w == 0 || pairs[2 * k] != pairs[2 * (w - 1)] || pairs[2 * k + 1] != pairs[2 * (w - 1) + 1]) {
+            pairs[2 * w] = pairs[2 * k];
+            pairs[2 * w + 1] = pairs[2 * k + 1];
+            ++w;
+        }
+    return (int64_t)w;
+}
+
 uint64_t or_sweep_tests(const float* boxes, int kind, size_t n, const float axis[3]) {
     if (n == 0) return 0;
     uint32_t* keys = (uint32_t*)malloc(n * 4);

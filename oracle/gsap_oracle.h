@@ -81,6 +81,12 @@ void or_last_times(double out[3]);
 /* Sweep-test count for a given axis (label source for the learned axis). */
 uint64_t or_sweep_tests(const float* boxes, int kind, size_t n, const float axis[3]);
 
+/* The reference's 4-cell sweep plus cross pass (Sap.cpp:211-295), for its work
+ * counts: stats = {cell-pass tests, cross-pass tests, relevant boxes, pairs
+ * before deduplication}.  Returns the unique pair count. */
+int64_t or_sap_ref_cells(const float* boxes, int kind, size_t n, int count_only, uint32_t* pairs,
+                         uint64_t cap, uint64_t stats[4]);
+
 /* Brute force O(n^2), canonical output. */
 int64_t or_brute(const float* boxes, int kind, size_t n, const uint32_t* group,
                  uint32_t* pairs, uint64_t cap);

@@ -50,6 +50,8 @@ def lib():
         L.or_last_times.argtypes = [P]
         L.or_sweep_tests.restype = u64
         L.or_sweep_tests.argtypes = [P, ctypes.c_int, sz, P]
+        L.or_sap_ref_cells.restype = ctypes.c_int64
+        L.or_sap_ref_cells.argtypes = [P, ctypes.c_int, sz, ctypes.c_int, P, u64, P]
         L.or_brute.restype = i64
         L.or_brute.argtypes = [P, ctypes.c_int, sz, P, P, u64]
         L.or_canon_sort.argtypes = [P, u64]
@@ -156,6 +158,29 @@ def last_times():
     t = np.zeros(3, np.float64)
     lib().or_last_times(_p(t))
     return tuple(float(x) for x in t)
+
+
+def sap_ref_cells(boxes, kind=KIND_AABB, count_only=False, cap=None):
+    """The reference's 4-cell sweep + cross pass (Sap.cpp:211-295): (canonical
+    pairs or None when count_only, {cell_tests, cross_tests, relevant,
+    raw_pairs})."""
+    b = _boxes(boxes, kind)
+    n = len(b)
+    stats = np.zeros(4, np.uint64)
+    if count_only:
+        lib().or_sap_ref_cells(_p(b), kind, n, 1, None, 0, _p(stats))
+        pairs = None
+    else:
+        cap = max(16, 64 * n) if cap is None else cap
+        while True:
+            buf = np.empty((cap, 2), np.uint32)
+            P = lib().or_sap_ref_cells(_p(b), kind, n, 0, _p(buf), cap, _p(stats))
+            if int(stats[3]) <= cap:
+                pairs = buf[:P].copy()
+                break
+            cap = int(stats[3])
+    keys = ("cell_tests", "cross_tests", "relevant", "raw_pairs")
+    return pairs, {k: int(v) for k, v in zip(keys, stats)}
 
 
 def sweep_tests(boxes, axis, kind=KIND_AABB):

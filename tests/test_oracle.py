@@ -296,3 +296,41 @@ def test_quaternion_stays_unit_and_torque():
     s.update_velocity(1.0, p)
     r = s.ppos[0, :3] - s.pos[0, :3]
     assert np.allclose(s.angvel[0, :3] - w0[0, :3], np.cross(r, [0, 1, 0]), atol=1e-5)
+
+
+def test_ref_cells_equals_brute_force():
+    """The reference's cell + cross pass finds the same pairs as brute force,
+    on centred scenes (boxes on both sides of the split planes) for AABBs and
+    spheres, and on a scene entirely inside one cell."""
+    rng = np.random.default_rng(17)
+    c = rng.normal(0.0, 6.0, (6000, 3)).astype(np.float32)
+    h = rng.uniform(0.1, 0.9, (6000, 3)).astype(np.float32)
+    boxes = np.concatenate([c - h, c + h], 1)
+    pairs, st = O.sap_ref_cells(boxes)
+    assert np.array_equal(pairs, O.brute(boxes))
+    assert st["relevant"] > 0 and st["cross_tests"] > 0 and st["raw_pairs"] > len(pairs)
+    sph = np.concatenate([c, rng.uniform(0.2, 0.8, (6000, 1)).astype(np.float32)], 1)
+    pairs, st = O.sap_ref_cells(sph, kind=O.KIND_SPHERE)
+    assert np.array_equal(pairs, O.brute(sph, kind=O.KIND_SPHERE))
+    one = np.abs(boxes) + 1.0  # all centres positive: one cell, no straddlers
+    one = np.concatenate([np.minimum(one[:, :3], one[:, 3:]), np.maximum(one[:, :3], one[:, 3:])], 1)
+    pairs, st = O.sap_ref_cells(one)
+    assert np.array_equal(pairs, O.brute(one))
+    # count-only mode reports the same work
+    _, st2 = O.sap_ref_cells(boxes, count_only=True)
+    _, st1 = O.sap_ref_cells(boxes)
+    assert st2["cell_tests"] == st1["cell_tests"] and st2["cross_tests"] == st1["cross_tests"]
+
+
+def test_ref_cells_straddlers_across_planes():
+    """Hand case: two boxes overlapping across a split plane, each in a
+    different cell, are found by the cross pass only."""
+    b = np.array([[0, -0.3, 1, 1, -0.1, 2],       # centre y < 0
+                  [0.5, -0.2, 1.5, 1.5, 0.6, 2.5],  # centre y > 0, straddles y = 0
+                  [50, 5, 5, 51, 6, 6]], np.float32)
+    # make x the major axis (largest variance)
+    far = np.array([[100, 1, 1, 101, 2, 2], [-100, 1, 1, -99, 2, 2]], np.float32)
+    boxes = np.concatenate([b, far], 0)
+    pairs, st = O.sap_ref_cells(boxes)
+    assert pairs.tolist() == [[0, 1]]
+    assert st["relevant"] >= 2
