@@ -252,7 +252,9 @@ static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg
     (void)n;
     (void)nseg;
     const uint64_t rcap = cap > 0 ? cap : 1;
-    if (rcap >= (1ull << 32) - (1ull << 20))
+    // pair positions are 32-bit; the margin keeps every sort index (padding,
+    // unrolled histogram rounds) below 2^32
+    if (rcap >= (1ull << 32) - (1ull << 22))
         return fail(h, GSAP_ERR_UNSUPPORTED, "pair capacity too large (pair indices are 32-bit)");
     const uint64_t nreg = (uint64_t)h->pair_regions;
     const uint64_t stride = (rcap + kPairTileMax - 1) / kPairTileMax * kPairTileMax + kPairTileMax;

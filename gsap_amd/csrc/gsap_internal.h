@@ -203,7 +203,8 @@ hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1
                      int* which, hipStream_t st);
 // Pairs: passes ping-pong between p0 and p1; the final pass writes `out`.
 hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st);
-constexpr uint32_t kSweepStage = 512;  // pairs staged in LDS per sweep wave
+constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
+constexpr uint32_t kSweepStageSlab = 600;  // the same, slab sweep (fewer flushes)
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
                            int flip_float, hipStream_t st);

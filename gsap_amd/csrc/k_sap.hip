@@ -555,7 +555,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 #else
     typedef const Rec CRec;
 #endif
-    constexpr uint32_t kSt = MODE == 2 ? 1u : kStage;
+    // the slab sweep emits denser hits, so it gets the larger stage (8
+    // workgroups of 4 waves still fit a CU's LDS either way)
+    constexpr uint32_t kCap = SLAB ? kSweepStageSlab : kStage;
+    constexpr uint32_t kSt = MODE == 2 ? 1u : kCap;
     __shared__ uint2 s_stage[kWaves][kSt];
     __shared__ uint32_t s_fill[kWaves];
     __shared__ uint32_t s_np[kWaves];
@@ -684,7 +687,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 const uint32_t cnt = (uint32_t)__popcll(m) * mult;
                 npairs += cnt;
                 if constexpr (MODE != 2) {
-                    if (fill + cnt > kStage) {  // stage full: write it out
+                    if (fill + cnt > kCap) {  // stage full: write it out
                         const unsigned long long base = flush(fill);
                         for (uint32_t i = lane; i < fill; i += 64u)
                             if (base + i < cap) rout[base + i] = stage[i];

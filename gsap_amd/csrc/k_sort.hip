@@ -267,25 +267,42 @@ __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__
     }
     const uint32_t lane = lane_id();
     const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
-    for (uint32_t b0 = blockIdx.x * kWG + (threadIdx.x & ~63u); b0 < n; b0 += gridDim.x * kWG) {
-        const uint32_t i = b0 + lane;
-        const bool valid = i < n;
-        const typename KT::T* src = keys;
-        if (nregions > 1) src = keys + region_offset(region_start, region_stride, nregions, b0);
-        typename KT::T k{};
-        if (valid) k = src[i];
-        if constexpr (KT::kHasVals) {
-            if (flip_float) k = flip_f32(__uint_as_float(k));
+    // Each wave takes UNROLL 64-key chunks per round (loads issued together,
+    // so enough bytes are in flight to stream the keys at full bandwidth).
+    constexpr uint32_t UNROLL = 4;
+    const uint32_t stride = gridDim.x * kWG;
+    for (uint32_t b0 = blockIdx.x * kWG + (threadIdx.x & ~63u); b0 < n; b0 += UNROLL * stride) {
+        typename KT::T kk[UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; ++u) {
+            const uint32_t c0 = b0 + u * stride;  // no wrap: n + UNROLL * stride < 2^32 (n < 2^32 - 2^22)
+            kk[u] = typename KT::T{};
+            if (c0 + lane < n) {
+                const typename KT::T* src = keys;
+                if (nregions > 1)
+                    src = keys + region_offset(region_start, region_stride, nregions, c0);
+                kk[u] = src[c0 + lane];
+            }
         }
-        const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
-        for (int p = 0; p < npass; ++p) {
-            const uint32_t d = KT::digit(k, p * BITS, bbits, MASK);
-            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
-            const bool head = valid && (lane == 0u || prev != d);
-            const unsigned long long nxt = ballot(head) & above;
-            if (head) {
-                const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
-                atomicAdd(&s_h[p * RADIX + d], end - lane);
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; ++u) {
+            const uint32_t c0 = b0 + u * stride;
+            if (c0 >= n) break;  // wave-uniform
+            const bool valid = c0 + lane < n;
+            typename KT::T k = kk[u];
+            if constexpr (KT::kHasVals) {
+                if (flip_float) k = flip_f32(__uint_as_float(k));
+            }
+            const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+            for (int p = 0; p < npass; ++p) {
+                const uint32_t d = KT::digit(k, p * BITS, bbits, MASK);
+                const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+                const bool head = valid && (lane == 0u || prev != d);
+                const unsigned long long nxt = ballot(head) & above;
+                if (head) {
+                    const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+                    atomicAdd(&s_h[p * RADIX + d], end - lane);
+                }
             }
         }
     }
