@@ -79,7 +79,7 @@ struct gsap_sap {
     Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
-    Buf axis2, vr_partials, slabp;  // slab mode
+    Buf axis2, vr_partials, slabp, slab_cnt;  // slab mode
     bool has_model = false;
     float conf_min = 0.95f;
     unsigned long long* h_counts = nullptr;  // pinned
@@ -179,7 +179,7 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
                    &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
                    &h->tile_seg, &h->grp, &h->grp_end, &h->feats, &h->tile_sums, &h->seg_mom,
                    &h->cand, &h->mlp_wb, &h->mlp_master, &h->ax_tests_axes, &h->axis2,
-                   &h->vr_partials, &h->slabp};
+                   &h->vr_partials, &h->slabp, &h->slab_cnt};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
     for (int i = 0; i < gsap_sap::kEv; ++i)
@@ -239,6 +239,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     if (slab_mode(h) && !h->slabp.p) {
         HIPCHK(h, grow(h->axis2, 4 * sizeof(float)));
         HIPCHK(h, grow(h->slabp, kSlabParams * sizeof(float)));
+        HIPCHK(h, grow(h->slab_cnt, 256 * sizeof(uint32_t)));
     }
     return GSAP_OK;
 }
@@ -324,6 +325,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.axis2 = (float*)h->axis2.p;
     a.vr_partials = (float*)h->vr_partials.p;
     a.slab = (float*)h->slabp.p;
+    a.slab_cnt = (uint32_t*)h->slab_cnt.p;
     const uint32_t n_e = slabs ? 2 * n : n;  // sorted entries
     const uint32_t ntiles_e = slabs ? (n_e + tile - 1) / tile : ntiles;
     const bool learned = h->desc.axis_mode == GSAP_AXIS_LEARNED;

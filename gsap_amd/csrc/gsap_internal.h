@@ -92,6 +92,7 @@ struct ObjPipeArgs {
     float* axis2;           // [4] slab axis v (largest spread orthogonal to the sweep axis)
     float* vr_partials;     // [ntiles][kVr] per-tile ranges for the slab parameters
     float* slab;            // [kSlabParams] slab and key-quantisation parameters
+    uint32_t* slab_cnt;     // [256] entries per slab (zeroed by k_moments, counted by k_project)
 };
 constexpr int kVr = 5;           // max v extent, min lo_v, max hi_v, min lo_u, max hi_u
 constexpr int kSlabParams = 8;   // v0, 1/W, S, u0, 1/du, W, -, -

@@ -117,6 +117,7 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
         for (uint32_t i = b0 + tid; i < b1; i += kWG) a.hist_obj[i] = 0u;
         if (tile == 0) {
             for (uint32_t i = tid; i < a.npass_pair * 256u; i += kWG) a.hist_pair[i] = 0u;
+            if (a.slabs) a.slab_cnt[tid] = 0u;  // kWG == 256 slabs
             if (tid < 2 * kMaxPasses) a.ctrl->tile_ctr[tid] = 0u;
             if (tid == 0) a.ctrl->epoch += 1u;
         }
@@ -399,6 +400,9 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
         // last).  Key = slab << 24 | 24-bit quantised lower bound on u.
         const float v[3] = {a.axis2[0], a.axis2[1], a.axis2[2]};
         const float* sp = a.slab;
+        __shared__ uint32_t s_sc[256];  // entries per slab in this tile
+        s_sc[tid] = 0u;
+        __syncthreads();
         for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
             float lo[3], hi[3], c4[4], pl, ph, ql, qh;
             load_lohi(a.in, a.kind, i, lo, hi, c4);
@@ -417,7 +421,11 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
             reinterpret_cast<uint2*>(a.vals)[i] = make_uint2(i | kFirstBit, i);
             hist_key(s_h, k0, npass, bits);
             hist_key(s_h, k1, npass, bits);
+            atomicAdd(&s_sc[s0], 1u);
+            if (s1 > s0) atomicAdd(&s_sc[s1], 1u);
         }
+        __syncthreads();
+        if (s_sc[tid]) atomicAdd(&a.slab_cnt[tid], s_sc[tid]);
     }
     __syncthreads();
     uint32_t* h = a.hist_obj + (size_t)ti.seg * npass * radix;
@@ -437,12 +445,22 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
                                                 const uint32_t* __restrict__ V) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
     __shared__ uint32_t s_end[kWG];
+    __shared__ uint32_t s_sb[257];  // slab mode: first sorted position of each slab (+ end)
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const float* ax = a.axis + 4 * (size_t)ti.seg;
     const float u[3] = {ax[0], ax[1], ax[2]};
     const bool exact = axis_exact(KIND, u);
     unsigned long long tests = 0;
+    if (a.slabs) {  // entries are sorted slab-major: slab starts are the prefix of the counts
+        __shared__ uint32_t s_tmp[kWaves];
+        uint32_t tot;
+        const uint32_t c = a.slab_cnt[tid];
+        const uint32_t ex = block_excl_scan(c, s_tmp, &tot);
+        s_sb[tid] = ex;
+        if (tid == 0) s_sb[256] = tot;
+        __syncthreads();
+    }
     for (uint32_t sub = ti.lo; sub < ti.hi; sub += kWG) {
         const uint32_t s = sub + tid;
         const bool valid = s < ti.hi;
@@ -459,7 +477,7 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
             // first t in [s+1, seg_end) with K[t] > hk
             const uint32_t hk = a.slabs ? ((key & 0xFF000000u) | qkey(ph, a.slab)) : flip_f32(ph);
             uint32_t b = s + 1, e = ti.seg_end;
-            if (a.slabs && key == kSlabNull) e = b;  // unused entry: no candidates
+            if (a.slabs) e = key == kSlabNull ? b : s_sb[(key >> 24) + 1];  // within the slab
             while (b < e) {
                 const uint32_t mid = b + ((e - b) >> 1);
                 if (K[mid] <= hk) b = mid + 1; else e = mid;
@@ -488,13 +506,7 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
                 } else {
                     // first position of this entry's slab: candidates of another
                     // slab are told apart by it in the sweep
-                    const uint32_t lk = key & 0xFF000000u;
-                    uint32_t lb = ti.seg_start, le = s;
-                    while (lb < le) {
-                        const uint32_t mid = lb + ((le - lb) >> 1);
-                        if (K[mid] < lk) lb = mid + 1; else le = mid;
-                    }
-                    r.aux = lb;
+                    r.aux = s_sb[key >> 24];
                 }
             }
             reinterpret_cast<Rec*>(a.rec)[s] = r;
