@@ -71,7 +71,7 @@ class Tune(ctypes.Structure):
                 ("pair_items", ctypes.c_int32), ("pair_bits", ctypes.c_int32),
                 ("pair_grid", ctypes.c_int32), ("slabs", ctypes.c_int32),
                 ("sweep_chunk", ctypes.c_int32), ("pair_regions", ctypes.c_int32),
-                ("ms", ctypes.c_float),
+                ("pair_local", ctypes.c_int32), ("ms", ctypes.c_float),
                 ("from_cache", ctypes.c_int32)]
 
     def as_dict(self):
@@ -294,7 +294,10 @@ class Sap:
     def set_tune(self, **kw):
         t = Tune()
         check(lib().gsap_sap_get_tune(self.h, ctypes.byref(t)))
+        names = {k for k, _ in Tune._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise ValueError(f"unknown tune field {k!r}")
             setattr(t, k, v)
         check(lib().gsap_sap_set_tune(self.h, ctypes.byref(t)))
 

@@ -72,6 +72,7 @@ struct gsap_sap {
     int slabs = 0;        // slab sweep for single-scene updates
     int sweep_chunk = 0;  // sweep sub-tiles per XCD chunk
     int pair_regions = 1; // sweep output regions (1 or kMaxRegions)
+    int pair_local = 0;   // pair sort: onesweep over the high key bits + per-bucket LDS sort
     float tune_ms = 0.0f;
     int tune_cached = 0;
     uint32_t cap_em = 1;  // entries per object the sort buffers are sized for
@@ -80,6 +81,7 @@ struct gsap_sap {
     Buf seg_start, seg_n, seg_tile0, tile_seg;
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
     Buf axis2, vr_partials, slabp, slab_cnt;  // slab mode
+    Buf wstart;                               // window starts of the pair sort's local stage
     bool has_model = false;
     float conf_min = 0.95f;
     unsigned long long* h_counts = nullptr;  // pinned
@@ -179,7 +181,7 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
                    &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
                    &h->tile_seg, &h->grp, &h->grp_end, &h->feats, &h->tile_sums, &h->seg_mom,
                    &h->cand, &h->mlp_wb, &h->mlp_master, &h->ax_tests_axes, &h->axis2,
-                   &h->vr_partials, &h->slabp, &h->slab_cnt};
+                   &h->vr_partials, &h->slabp, &h->slab_cnt, &h->wstart};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
     for (int i = 0; i < gsap_sap::kEv; ++i)
@@ -262,6 +264,7 @@ static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg
     const uint64_t sorted = rcap + (nreg + 1) * kPairTileMax;  // padded total, upper bound
     HIPCHK(h, grow(h->ptmp[0], (size_t)(nreg * stride) * 8));
     HIPCHK(h, grow(h->ptmp[1], (size_t)sorted * 8));
+    HIPCHK(h, grow(h->wstart, (size_t)pair_local_wstart_words(rcap) * 4));
     h->slot_cap = rcap;
     h->region_stride = stride;
     h->sort_cap = sorted;
@@ -287,7 +290,15 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     const bool sym = h->desc.output_kind == GSAP_OUT_SYM_CSR;
     const uint32_t tile = sort_tile(h->items);
     const uint32_t bbits = bits_for(n);
-    const uint32_t npass_pair = (2 * bbits + h->pair_bits - 1) / h->pair_bits;
+    // Pair sort.  Whole-key onesweep: ceil(2 bbits / pair_bits) passes.  With
+    // the local stage: 8-bit onesweep passes over key bits [L, 2 bbits) with
+    // 8 <= L <= 24, then every run of equal key >> L sorted by the low L bits.
+    const uint32_t kbits = 2 * bbits;
+    const bool local = h->pair_local && kbits >= 16;
+    const uint32_t npass_g = !local ? 0 : (kbits > 24 ? (kbits - 24 + 7) / 8 : 1);
+    const uint32_t shift0 = local ? kbits - 8 * npass_g : 0;
+    const uint32_t pair_bits = local ? 8 : (uint32_t)h->pair_bits;
+    const uint32_t npass_pair = local ? npass_g : (kbits + pair_bits - 1) / pair_bits;
     const uint32_t npass_obj = (32 + h->obj_bits - 1) / h->obj_bits;
     Ctrl* ctrl = (Ctrl*)h->ctrl.p;
     h->nev = 0;
@@ -399,32 +410,45 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     mark(h, st);
     if (!count_only) {
         const uint32_t nreg = (uint32_t)h->pair_regions;
-        HIPCHK(h, launch_pair_regions(ctrl, t0p, h->slot_cap, h->region_stride, nreg,
-                                      sort_tile(h->pair_items), st));
+        HIPCHK(h, launch_pair_regions(ctrl, h->slot_cap, nreg, sort_tile(h->pair_items), st));
         HIPCHK(h, launch_pair_hist(t0p, &ctrl->sort_n, h->sort_cap, a.hist_pair, npass_pair, bbits,
-                                   h->pair_bits, ctrl->region_start, h->region_stride, nreg, st));
+                                   pair_bits, shift0, ctrl->region_start, ctrl->region_valid,
+                                   h->region_stride, nreg, st));
         mark(h, st);
         SortArgs p{};
         p.n_dev = &ctrl->sort_n;
+        p.n_dev_rest = &ctrl->sort_real;
         p.cap = h->sort_cap;
         p.region_start = ctrl->region_start;
+        p.region_valid = ctrl->region_valid;
         p.region_stride = h->region_stride;
         p.nregions = nreg;
+        p.shift0 = (int)shift0;
         p.hist = a.hist_pair;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;
         p.ctr_base = kMaxPasses;
-        p.bits = h->pair_bits;
+        p.bits = (int)pair_bits;
         p.items = h->pair_items;
         p.bbits = (int)bbits;
         p.grid = (uint32_t)h->pair_grid;
-        p.last_n = &ctrl->num_pairs;
+        p.last_n = local ? nullptr : &ctrl->num_pairs;
         p.last_cap = cap;
-        const bool pev = h->timing && h->nev + (int)npass_pair + 2 <= gsap_sap::kEv;
+        const bool pev = h->timing && h->nev + (int)npass_pair + 3 <= gsap_sap::kEv;
         p.pass_ev = pev ? &h->ev[h->nev] : nullptr;
-        HIPCHK(h, sort_pairs(p, t0p, t1p, outp, st));
+        HIPCHK(h, sort_pairs(p, t0p, t1p, local ? nullptr : outp, st));
         if (pev) h->nev += (int)npass_pair;
+        if (local) {
+            // the onesweep passes leave the bucket-sorted pairs in t1p after an
+            // odd number of passes, in t0p after an even one
+            uint2* x = (npass_pair & 1u) ? t1p : t0p;
+            uint2* y = (npass_pair & 1u) ? t0p : t1p;
+            HIPCHK(h, launch_pair_local(x, y, outp, &ctrl->sort_real, &ctrl->num_pairs, cap,
+                                        h->slot_cap, (uint32_t*)h->wstart.p, (int)bbits,
+                                        (int)shift0, st));
+            mark(h, st);
+        }
         if (sym)
             HIPCHK(h, launch_entries(outp, &ctrl->num_pairs, cap, n, (uint32_t*)h->rowstart.p,
                                      (uint32_t*)h->has.p, (uint32_t*)h->block_sums.p, entries,
@@ -869,7 +893,8 @@ static bool valid_tune(const gsap_tune* t) {
            (t->slabs == 0 || t->slabs == 1) &&
            (t->sweep_chunk == 0 || t->sweep_chunk == 4 || t->sweep_chunk == 16 ||
             t->sweep_chunk == 64) &&
-           (t->pair_regions == 1 || t->pair_regions == kMaxRegions);
+           (t->pair_regions == 1 || t->pair_regions == kMaxRegions) &&
+           (t->pair_local == 0 || t->pair_local == 1);
 }
 
 static gsap_tune tune_of(const gsap_sap* h) {
@@ -882,6 +907,7 @@ static gsap_tune tune_of(const gsap_sap* h) {
     t.slabs = h->slabs;
     t.sweep_chunk = h->sweep_chunk;
     t.pair_regions = h->pair_regions;
+    t.pair_local = h->pair_local;
     t.ms = h->tune_ms;
     t.from_cache = h->tune_cached;
     return t;
@@ -896,6 +922,7 @@ static void apply_tune(gsap_sap* h, const gsap_tune& t) {
     h->slabs = t.slabs;
     h->sweep_chunk = t.sweep_chunk;
     h->pair_regions = t.pair_regions;
+    h->pair_local = t.pair_local;
     if (h->gexec) {  // the captured batched update used the old configuration
         (void)hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;
@@ -914,7 +941,7 @@ extern "C" int32_t gsap_sap_set_tune(gsap_sap* h, const gsap_tune* t) {
     if (!valid_tune(t))
         return fail(h, GSAP_ERR_INVALID_ARG,
                     "gsap_sap_set_tune: items 4/8/12/16 and 8/16, bits 6..8, grid 256..16384, "
-                    "slabs 0/1, sweep_chunk 0/4/16/64, pair_regions 1/8");
+                    "slabs 0/1, sweep_chunk 0/4/16/64, pair_regions 1/8, pair_local 0/1");
     apply_tune(h, *t);
     h->tune_ms = 0.0f;
     h->tune_cached = 0;
@@ -948,9 +975,9 @@ static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t)
     while (fgets(line, sizeof line, f)) {
         char k[300];
         gsap_tune c{};
-        if (sscanf(line, "%299s %d %d %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
+        if (sscanf(line, "%299s %d %d %d %d %d %d %d %d %d %f", k, &c.sort_items, &c.sort_bits,
                    &c.pair_items, &c.pair_bits, &c.pair_grid, &c.slabs, &c.sweep_chunk,
-                   &c.pair_regions, &c.ms) == 10 &&
+                   &c.pair_regions, &c.pair_local, &c.ms) == 11 &&
             key == k && valid_tune(&c)) {
             *t = c;  // the last matching line wins
             hit = true;
@@ -963,8 +990,9 @@ static bool cache_lookup(const char* path, const std::string& key, gsap_tune* t)
 static void cache_append(const char* path, const std::string& key, const gsap_tune& t) {
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
-            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.sweep_chunk, t.pair_regions, t.ms);
+    fprintf(f, "%s %d %d %d %d %d %d %d %d %d %.4f\n", key.c_str(), t.sort_items, t.sort_bits,
+            t.pair_items, t.pair_bits, t.pair_grid, t.slabs, t.sweep_chunk, t.pair_regions,
+            t.pair_local, t.ms);
     fclose(f);
 }
 
@@ -1074,18 +1102,22 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
                 if (items != base.sort_items || bits != base.sort_bits) consider(t);
             }
     }
-    {  // stage 3: pair sort tile, digit width and grid
+    {  // stage 3: pair sort tile, digit width and grid; whole key or high bits + local stage
         const gsap_tune base = best;
-        for (int items : {8, 16})
-            for (int bits : {6, 7, 8})
-                for (int grid : {512, 1024, 2048, 4096}) {
-                    gsap_tune t = base;
-                    t.pair_items = items;
-                    t.pair_bits = bits;
-                    t.pair_grid = grid;
-                    if (items != base.pair_items || bits != base.pair_bits || grid != base.pair_grid)
-                        consider(t);
-                }
+        for (int loc : {0, 1})
+            for (int items : {8, 16})
+                for (int bits : {6, 7, 8})
+                    for (int grid : {512, 1024, 2048, 4096}) {
+                        if (loc && bits != base.pair_bits) continue;  // 8-bit digits with the local stage
+                        gsap_tune t = base;
+                        t.pair_local = loc;
+                        t.pair_items = items;
+                        t.pair_bits = bits;
+                        t.pair_grid = grid;
+                        if (loc != base.pair_local || items != base.pair_items ||
+                            bits != base.pair_bits || grid != base.pair_grid)
+                            consider(t);
+                    }
     }
     h->timing = timing;
     (void)hipEventDestroy(e0);

@@ -51,7 +51,9 @@ struct Ctrl {
     unsigned long long exec_tests;   // candidate tests executed (wave granularity)
     unsigned long long region_fill[kMaxRegions];       // pairs written to each region
     unsigned long long region_start[kMaxRegions + 1];  // logical start of each region
+    unsigned long long region_valid[kMaxRegions];      // logical end of each region's pairs
     unsigned long long sort_n;       // pairs (+ tile padding) the pair sort reads
+    unsigned long long sort_real;    // pairs the pair sort holds (padding excluded)
     uint32_t flags;                  // 1 = capacity overflow, 2 = non-finite input
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
@@ -128,16 +130,19 @@ struct SweepArgs {
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
+// Digit histograms of the pair keys for npass digits starting at bit shift0;
+// padding slots past region_valid[r] are not counted.
 hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
                             unsigned long long cap, uint32_t* hist, int npass, int bbits, int bits,
-                            const unsigned long long* region_start,
+                            int shift0, const unsigned long long* region_start,
+                            const unsigned long long* region_valid,
                             unsigned long long region_stride, uint32_t nregions, hipStream_t st);
-// Lay the sweep's pair regions out for the sort: clamp each fill to the
-// region capacity, pad it to a whole tile with sentinel pairs (sorted last),
-// and write the logical region starts and the padded total (ctrl->sort_n).
-hipError_t launch_pair_regions(Ctrl* ctrl, uint2* regions, unsigned long long region_cap,
-                               unsigned long long region_stride, uint32_t nregions, uint32_t tile,
-                               hipStream_t st);
+// Lay the sweep's pair regions out for the sort: clamp the fills so that
+// together they hold at most region_cap pairs, round each region up to a whole
+// tile (the padding is never read), and write the logical region starts and
+// ends, the padded total (ctrl->sort_n) and the pair total (ctrl->sort_real).
+hipError_t launch_pair_regions(Ctrl* ctrl, unsigned long long region_cap, uint32_t nregions,
+                               uint32_t tile, hipStream_t st);
 hipError_t launch_entries(const uint2* sorted_pairs, const unsigned long long* n_dev,
                           unsigned long long cap, uint32_t nobj, uint32_t* rowstart,
                           uint32_t* has, uint32_t* block_sums, int32_t* entries, Ctrl* ctrl,
@@ -193,17 +198,36 @@ struct SortArgs {
     unsigned long long last_cap;        //   last_cap) (pairs: the caller's capacity)
     hipEvent_t* pass_ev;                // if set: recorded after each pass (timing)
     // pair sort, first pass: the input is nregions tile-aligned regions of
-    // region_stride slots, logically concatenated at region_start[]
+    // region_stride slots, logically concatenated at region_start[]; the
+    // slots from region_valid[r] to the next region's start are padding
     const unsigned long long* region_start;
+    const unsigned long long* region_valid;
     unsigned long long region_stride;
     uint32_t nregions;
+    const unsigned long long* n_dev_rest;  // element count of the passes after the first (or null)
+    int shift0;                            // bit offset of the first digit
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
 hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      int* which, hipStream_t st);
-// Pairs: passes ping-pong between p0 and p1; the final pass writes `out`.
+// Pairs: passes ping-pong between p0 and p1; the final pass writes `out`
+// (or, with out == nullptr, the ping-pong buffer: p1 after an odd number of
+// passes, p0 after an even number).
 hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipStream_t st);
+
+// ---- local stage of the pair sort (k_pair_local.hip) ----
+// After the onesweep passes over key bits [L, 2 bbits), sort every run of
+// equal key >> L ("bucket") by the low L bits (8 <= L <= 24).  x holds the
+// bucket-sorted pairs (*n_dev of them), y is free scratch of the same size;
+// out receives positions < min(*n_dev, *last_n, last_cap).
+constexpr uint32_t kLocW = 2048;   // window: pairs whose bucket starts are one workgroup's
+constexpr uint32_t kLocC = 8192;   // pairs sorted in LDS by one workgroup
+uint64_t pair_local_wstart_words(uint64_t cap);  // u32 words of the window-start table
+hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
+                             const unsigned long long* last_n, unsigned long long last_cap,
+                             unsigned long long cap, uint32_t* wstart, int bbits, int L,
+                             hipStream_t st);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 600;  // the same, slab sweep (fewer flushes)
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.

@@ -1,0 +1,390 @@
+// k_pair_local.hip -- local stage of the canonical pair sort.
+//
+// The canonical pair list is ordered by the logical key (a << bbits) | b, the
+// order of the reference's pair list (Sap.cpp:269-330, SapKernel.cu:2249-2360).
+// With this stage the onesweep passes sort only the key bits [L, 2 bbits); the
+// pairs then lie in runs ("buckets") of equal key >> L, in bucket order.  Keys
+// are unique, so what is left is to order every bucket by its low L bits,
+// independently of every other bucket: work that needs no look-back and one
+// trip through HBM instead of one per digit.
+//  * k_pair_wstart: for each window j of kLocW positions, the first bucket
+//    start at or after j * kLocW (a galloping 64-lane search over the sorted
+//    pairs); wstart[J] = the pair count.
+//  * k_pair_local: one workgroup per window sorts the buckets that start in
+//    it, positions [wstart[j], wstart[j+1]).  Up to kLocC pairs whose high
+//    bits lie within 2^(32-L) of the first one's: in LDS, on unique 32-bit
+//    keys (high - first high) << L | low bits.  One counting pass on the 12
+//    highest bits in which the window's keys differ puts every key in its
+//    bin; a key's place in the bin is the number of the bin's keys smaller
+//    than it.  Other windows (more than kLocC pairs, buckets too far apart
+//    for a 32-bit key, or a bin of more than kBinMax keys): stable LSD
+//    passes of wave64 ballot ranking through global memory, one chunk at a
+//    time, ping-ponging over the same positions with the free sort buffer.
+#include "gsap_device.h"
+
+namespace gsap {
+
+namespace {
+
+constexpr int kLT = 512;                // threads of a local-sort workgroup
+constexpr int kLWaves = kLT / 64;
+constexpr int kLocItems = kLocC / kLT;  // keys per thread of an LDS sort
+constexpr uint32_t kBinBits = 12;       // counting pass of an LDS sort: 4096 bins
+constexpr uint32_t kBins = 1u << kBinBits;
+constexpr uint32_t kBinMax = 32;        // largest bin ranked by comparisons
+constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
+constexpr uint32_t kGlobChunk = kLT * kGlobItems;
+
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_and_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v &= (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+
+// Exclusive scan over the kLT threads of a workgroup (s_tmp: kLWaves words).
+__device__ __forceinline__ uint32_t lblock_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+    const uint32_t l = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (l == 63) s_tmp[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kLWaves; ++i) {
+        const uint32_t t = s_tmp[i];
+        off += i < (int)w ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+__device__ __forceinline__ unsigned long long pkey(uint2 p, int bbits) {
+    return ((unsigned long long)p.x << bbits) | (unsigned long long)p.y;
+}
+__device__ __forceinline__ uint2 pkey_pair(unsigned long long k, int bbits) {
+    return make_uint2((uint32_t)(k >> bbits), (uint32_t)(k & ((1ull << bbits) - 1ull)));
+}
+
+// Stable rank of a row of 64 keys among the earlier keys of this wave with
+// the same digit: peers found by `bw` ballots, the highest lane of each peer
+// group advances the wave's counter of the digit.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, int bw, uint32_t* cnt,
+                                              unsigned long long gtm) {
+    unsigned long long m = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if (b < bw) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+    }
+    const uint32_t before = cnt[d];
+    if (valid && (m & gtm) == 0ull) cnt[d] = before + (uint32_t)__popcll(m);
+    return before + mbcnt(m);
+}
+
+}  // namespace
+
+// One wave per window boundary j in [0, J]: the smallest q >= j * kLocW with
+// q == n or key[q] >> L != key[q-1] >> L.  The pairs are sorted by key >> L,
+// so "key[q] >> L differs from key[t-1] >> L" is monotone in q >= t.
+__global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P,
+                                                     const unsigned long long* __restrict__ n_dev,
+                                                     int bbits, int L,
+                                                     uint32_t* __restrict__ wstart) {
+    const uint32_t n = (uint32_t)*n_dev;
+    const uint32_t J = (n + kLocW - 1) / kLocW;
+    const uint32_t lane = lane_id();
+    for (uint32_t j = blockIdx.x * kWaves + (threadIdx.x >> 6); j <= J;
+         j += gridDim.x * kWaves) {
+        const unsigned long long t = (unsigned long long)j * kLocW;
+        unsigned long long res = t;
+        if (t >= n) {
+            res = n;
+        } else if (t > 0) {
+            const unsigned long long h0 = pkey(P[t - 1], bbits) >> L;
+            // gallop: probes lo + lane * step, the step growing 64x per round
+            unsigned long long lo = t, hi = n, step = 1;
+            for (;;) {
+                const unsigned long long q = lo + (unsigned long long)lane * step;
+                const bool pr = q >= n || (pkey(P[q], bbits) >> L) != h0;
+                const unsigned long long m = ballot(pr);
+                if (m) {
+                    const uint32_t i = (uint32_t)__builtin_ctzll(m);
+                    if (i == 0) {
+                        hi = lo;
+                    } else {
+                        hi = lo + (unsigned long long)i * step;
+                        if (hi > n) hi = n;
+                        lo = lo + (unsigned long long)(i - 1) * step + 1;
+                    }
+                    break;
+                }
+                lo += 63ull * step + 1;
+                step *= 64;
+            }
+            // refine: the answer lies in [lo, hi] and hi satisfies the predicate
+            while (lo < hi) {
+                const unsigned long long stride = (hi - lo + 63) / 64;
+                const unsigned long long q = lo + (unsigned long long)lane * stride;
+                const bool pr = q >= hi || (pkey(P[q], bbits) >> L) != h0;
+                const unsigned long long m = ballot(pr);
+                if (m == 0) {
+                    lo += 63ull * stride + 1;
+                    continue;
+                }
+                const uint32_t i = (uint32_t)__builtin_ctzll(m);
+                if (i == 0) {
+                    hi = lo;
+                    break;
+                }
+                const unsigned long long nhi = lo + (unsigned long long)i * stride;
+                lo = lo + (unsigned long long)(i - 1) * stride + 1;
+                if (nhi < hi) hi = nhi;
+            }
+            res = hi;
+        }
+        if (lane == 0) wstart[j] = (uint32_t)res;
+    }
+}
+
+__global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2* __restrict__ out,
+                                                       const unsigned long long* __restrict__ n_dev,
+                                                       const unsigned long long* __restrict__ last_n,
+                                                       unsigned long long last_cap,
+                                                       const uint32_t* __restrict__ wstart,
+                                                       int bbits, int L) {
+    __shared__ uint32_t s_e[kLocC];
+    // bins of the counting pass, or the per-wave digit counters of the LSD passes
+    __shared__ uint32_t s_work[kBins > kLWaves * 256 ? kBins : kLWaves * 256];
+    uint32_t(*const s_wh)[256] = reinterpret_cast<uint32_t(*)[256]>(s_work);
+    static_assert(kBins >= kLWaves * 256, "the digit counters share the bins' memory");
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_tmp[kLWaves];
+    __shared__ uint32_t s_red[3 * kLWaves];
+
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
+    const uint32_t n = (uint32_t)*n_dev;
+    unsigned long long out_lim = n;
+    {
+        const unsigned long long ln = *last_n;
+        if (ln < out_lim) out_lim = ln;
+        if (last_cap < out_lim) out_lim = last_cap;
+    }
+    const uint32_t J = (n + kLocW - 1) / kLocW;
+    const uint32_t lowmask = (1u << L) - 1u;  // 8 <= L <= 24
+
+    for (uint32_t j = blockIdx.x; j < J; j += gridDim.x) {
+        const uint32_t s = wstart[j], e = wstart[j + 1];
+        if (s >= e) continue;  // no bucket starts in this window
+        const uint32_t S = e - s;
+        // 32-bit keys relative to the window's first bucket: (high - hs) << L | low
+        const unsigned long long hs = pkey(X[s], bbits) >> L;
+        const unsigned long long he = pkey(X[e - 1], bbits) >> L;
+        bool in_lds = S <= kLocC && (he - hs) < (1ull << (32 - L));
+        if (in_lds) {
+            const uint32_t nk = (S + kLT - 1) / kLT;
+            uint32_t kk[kLocItems], vor = 0u, vand = ~0u;
+#pragma unroll
+            for (int k = 0; k < kLocItems; ++k) {
+                const uint32_t q = tid + (uint32_t)kLT * k;
+                kk[k] = 0u;
+                if ((uint32_t)k < nk && q < S) {
+                    const unsigned long long v = pkey(X[s + q], bbits);
+                    kk[k] = ((uint32_t)((v >> L) - hs) << L) | ((uint32_t)v & lowmask);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kLocItems; ++k) {
+                const uint32_t q = tid + (uint32_t)kLT * k;
+                if ((uint32_t)k < nk && q < S) {
+                    vor |= kk[k];
+                    vand &= kk[k];
+                }
+            }
+            vor = wave_or_u32(vor);
+            vand = wave_and_u32(vand);
+            if (lane == 0) {
+                s_red[wave] = vor;
+                s_red[kLWaves + wave] = vand;
+            }
+            for (uint32_t i = tid; i < kBins; i += kLT) s_work[i] = 0u;
+            __syncthreads();
+            auto emit = [&](uint32_t pos, uint32_t v) {
+                if (s + pos < out_lim)
+                    out[s + pos] = pkey_pair(((hs + (v >> L)) << L) | (v & lowmask), bbits);
+            };
+            // Counting pass on the 12 highest bits in which the window's keys
+            // differ (the bits above are common to all of them), then each
+            // key's place in its bin = the number of the bin's smaller keys.
+            vor = 0u;
+            vand = ~0u;
+#pragma unroll
+            for (int w = 0; w < kLWaves; ++w) {
+                vor |= s_red[w];
+                vand &= s_red[kLWaves + w];
+            }
+            const uint32_t diff = vor ^ vand;
+            const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
+            const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
+#pragma unroll
+            for (int k = 0; k < kLocItems; ++k) {
+                const uint32_t q = tid + (uint32_t)kLT * k;
+                if ((uint32_t)k < nk && q < S) atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u);
+            }
+            __syncthreads();
+            // bin starts (exclusive scan, kBins / kLT bins per thread) and the largest bin
+            constexpr uint32_t BPT = kBins / kLT;
+            uint32_t cb[BPT], sum = 0, mx = 0, all;
+#pragma unroll
+            for (uint32_t i = 0; i < BPT; ++i) {
+                cb[i] = s_work[tid * BPT + i];
+                sum += cb[i];
+                mx = cb[i] > mx ? cb[i] : mx;
+            }
+            mx = wave_max_u32(mx);
+            if (lane == 0) s_red[2 * kLWaves + wave] = mx;
+            uint32_t run = lblock_excl_scan(sum, s_tmp, &all);
+#pragma unroll
+            for (uint32_t i = 0; i < BPT; ++i) {
+                s_work[tid * BPT + i] = run;
+                run += cb[i];
+            }
+            mx = 0;
+#pragma unroll
+            for (int w = 0; w < kLWaves; ++w)
+                mx = s_red[2 * kLWaves + w] > mx ? s_red[2 * kLWaves + w] : mx;
+            __syncthreads();
+            if (mx <= kBinMax) {
+                // keys to their bins (any order inside a bin); a bin's counter
+                // ends at the next bin's start
+#pragma unroll
+                for (int k = 0; k < kLocItems; ++k) {
+                    const uint32_t q = tid + (uint32_t)kLT * k;
+                    if ((uint32_t)k < nk && q < S)
+                        s_e[atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
+                }
+                __syncthreads();
+                for (uint32_t pz = tid; pz < S; pz += kLT) {
+                    const uint32_t x = s_e[pz];
+                    const uint32_t bn = (x >> sd) & (kBins - 1u);
+                    const uint32_t be = s_work[bn];
+                    const uint32_t bs = bn ? s_work[bn - 1u] : 0u;
+                    uint32_t rank = 0;
+                    for (uint32_t i = bs; i < be; ++i) {
+                        const uint32_t y = s_e[i];
+                        rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
+                    }
+                    emit(bs + rank, x);
+                }
+            } else {
+                in_lds = false;  // a crowded bin: the global-memory path
+            }
+            __syncthreads();  // s_e and s_work are reused
+        }
+        if (!in_lds) {
+            // Through global memory: keys relative to the range's first bucket,
+            // T bits, LSD passes X -> Y -> X ... with the last one into out.
+            const unsigned long long hs = pkey(X[s], bbits) >> L;
+            const unsigned long long he = pkey(X[e - 1], bbits) >> L;
+            const unsigned long long kb = hs << L;
+            const unsigned long long kmax = ((he - hs) << L) | lowmask;
+            const uint32_t T = 64u - (uint32_t)__clzll((long long)kmax);
+            const uint32_t npass = (T + 7u) / 8u;
+            for (uint32_t p = 0; p < npass; ++p) {
+                const bool last = p + 1 == npass;
+                const uint2* src = (p & 1u) ? Y : X;
+                uint2* dst = last ? out : ((p & 1u) ? X : Y);
+                const uint32_t sh = 8u * p;
+                const int bw = (int)min(8u, T - sh);
+                const uint32_t mask = (1u << bw) - 1u;
+                if (tid < 256u) s_base[tid] = 0u;
+                __syncthreads();
+                for (uint32_t q = tid; q < S; q += kLT)
+                    atomicAdd(&s_base[(uint32_t)((pkey(src[s + q], bbits) - kb) >> sh) & mask], 1u);
+                __syncthreads();
+                uint32_t all;
+                const uint32_t c0v = tid < 256u ? s_base[tid] : 0u;
+                const uint32_t ex = lblock_excl_scan(c0v, s_tmp, &all);
+                if (tid < 256u) s_base[tid] = s + ex;
+                __syncthreads();
+                for (uint32_t c0 = 0; c0 < S; c0 += kGlobChunk) {
+                    const uint32_t cs = min(S - c0, kGlobChunk);
+                    const uint32_t nk = (cs + kLT - 1) / kLT;
+                    const uint32_t w0 = c0 + wave * 64u * nk;
+                    for (uint32_t i = lane; i < 256u; i += 64u) s_wh[wave][i] = 0u;
+                    unsigned long long kk[kGlobItems];
+                    uint32_t rk[kGlobItems];
+#pragma unroll
+                    for (int k = 0; k < kGlobItems; ++k) {
+                        const uint32_t q = w0 + 64u * k + lane;
+                        kk[k] = 0ull;
+                        if ((uint32_t)k < nk && q < c0 + cs) kk[k] = pkey(src[s + q], bbits) - kb;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kGlobItems; ++k) {
+                        if ((uint32_t)k < nk) {
+                            const uint32_t q = w0 + 64u * k + lane;
+                            rk[k] = wave_rank((uint32_t)(kk[k] >> sh) & mask, q < c0 + cs, bw,
+                                              s_wh[wave], gtm);
+                        }
+                    }
+                    __syncthreads();
+                    if (tid < 256u) {
+                        uint32_t run = s_base[tid];
+#pragma unroll
+                        for (int w = 0; w < kLWaves; ++w) {
+                            const uint32_t cw = s_wh[w][tid];
+                            s_wh[w][tid] = run;
+                            run += cw;
+                        }
+                        s_base[tid] = run;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int k = 0; k < kGlobItems; ++k) {
+                        if ((uint32_t)k < nk) {
+                            const uint32_t q = w0 + 64u * k + lane;
+                            if (q < c0 + cs) {
+                                const uint32_t g = s_wh[wave][(uint32_t)(kk[k] >> sh) & mask] + rk[k];
+                                if (!last || g < out_lim) dst[g] = pkey_pair(kk[k] + kb, bbits);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+    }
+}
+
+uint64_t pair_local_wstart_words(uint64_t cap) { return cap / kLocW + 2; }
+
+hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
+                             const unsigned long long* last_n, unsigned long long last_cap,
+                             unsigned long long cap, uint32_t* wstart, int bbits, int L,
+                             hipStream_t st) {
+    if (L < 8 || L > 24) return hipErrorInvalidValue;
+    const unsigned long long J = (cap + kLocW - 1) / kLocW;  // windows, at most
+    unsigned long long gw = (J + 1 + kWaves - 1) / kWaves;
+    if (gw > 2048) gw = 2048;
+    hipLaunchKernelGGL(k_pair_wstart, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits, L,
+                       wstart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    unsigned long long gl = J == 0 ? 1 : J;
+    if (gl > 4096) gl = 4096;
+    hipLaunchKernelGGL(k_pair_local, dim3((uint32_t)gl), dim3(kLT), 0, st, x, y, out, n_dev, last_n,
+                       last_cap, wstart, bbits, L);
+    return hipGetLastError();
+}
+
+}  // namespace gsap

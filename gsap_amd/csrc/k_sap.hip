@@ -861,31 +861,27 @@ __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
 }
 
 // One workgroup: region layout for the pair sort (see launch_pair_regions).
-__global__ __launch_bounds__(kWG) void k_pair_regions(Ctrl* ctrl, uint2* regions,
-                                                      unsigned long long region_cap,
-                                                      unsigned long long region_stride,
-                                                      uint32_t nregions, uint32_t tile) {
-    __shared__ unsigned long long s_fill[kMaxRegions], s_pad[kMaxRegions];
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) {
-        unsigned long long run = 0;
-        for (uint32_t r = 0; r < nregions; ++r) {
-            const unsigned long long f = ctrl->region_fill[r] < region_cap ? ctrl->region_fill[r]
-                                                                           : region_cap;
-            // a single region needs no padding: it is the whole, contiguous input
-            const unsigned long long p = nregions > 1 ? (f + tile - 1) / tile * tile : f;
-            s_fill[r] = f;
-            s_pad[r] = p;
-            ctrl->region_start[r] = run;
-            run += p;
-        }
-        for (uint32_t r = nregions; r <= (uint32_t)kMaxRegions; ++r) ctrl->region_start[r] = run;
-        ctrl->sort_n = run;
+__global__ void k_pair_regions(Ctrl* ctrl, unsigned long long region_cap, uint32_t nregions,
+                               uint32_t tile) {
+    if (threadIdx.x != 0) return;
+    // Together the regions hold at most region_cap pairs (the sort buffers'
+    // size); a region is rounded up to whole tiles, so a tile of the first
+    // sort pass lies in one region, and its slots past region_valid are
+    // padding that no pass reads.
+    unsigned long long run = 0, real = 0;
+    for (uint32_t r = 0; r < nregions; ++r) {
+        unsigned long long f = ctrl->region_fill[r];
+        if (f > region_cap - real) f = region_cap - real;
+        const unsigned long long p = nregions > 1 ? (f + tile - 1) / tile * tile : f;
+        ctrl->region_start[r] = run;
+        ctrl->region_valid[r] = run + f;
+        run += p;
+        real += f;
     }
-    __syncthreads();
-    for (uint32_t r = 0; r < nregions; ++r)  // sentinel pairs sort after every real pair
-        for (unsigned long long k = s_fill[r] + tid; k < s_pad[r]; k += kWG)
-            regions[(size_t)r * region_stride + k] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    for (uint32_t r = nregions; r <= (uint32_t)kMaxRegions; ++r) ctrl->region_start[r] = run;
+    for (uint32_t r = nregions; r < (uint32_t)kMaxRegions; ++r) ctrl->region_valid[r] = run;
+    ctrl->sort_n = run;
+    ctrl->sort_real = real;
 }
 
 // Per-scene ranges of the sorted (scene-major) canonical pair list.
@@ -977,11 +973,9 @@ hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
     return hipGetLastError();
 }
 
-hipError_t launch_pair_regions(Ctrl* ctrl, uint2* regions, unsigned long long region_cap,
-                               unsigned long long region_stride, uint32_t nregions, uint32_t tile,
-                               hipStream_t st) {
-    hipLaunchKernelGGL(k_pair_regions, dim3(1), dim3(kWG), 0, st, ctrl, regions, region_cap,
-                       region_stride, nregions, tile);
+hipError_t launch_pair_regions(Ctrl* ctrl, unsigned long long region_cap, uint32_t nregions,
+                               uint32_t tile, hipStream_t st) {
+    hipLaunchKernelGGL(k_pair_regions, dim3(1), dim3(64), 0, st, ctrl, region_cap, nregions, tile);
     return hipGetLastError();
 }
 

@@ -36,17 +36,15 @@ struct KPair {
     }
 };
 
-// Offset from a logical position of the region concatenation to its slot in
-// the region buffer (regions are tile-aligned, so a tile or a 64-key chunk
-// lies in one region): the last region starting at or before `lo`.
-__device__ __forceinline__ unsigned long long region_offset(const unsigned long long* start,
-                                                            unsigned long long stride,
-                                                            uint32_t nregions,
-                                                            unsigned long long lo) {
+// Region of a logical position of the region concatenation (regions are
+// tile-aligned, so a tile or a 64-key chunk lies in one region): the last
+// region starting at or before `lo`.
+__device__ __forceinline__ uint32_t region_of(const unsigned long long* start, uint32_t nregions,
+                                              unsigned long long lo) {
     uint32_t r = 0;
     for (uint32_t q = 1; q < nregions; ++q)
         if (start[q] <= lo) r = q;
-    return (unsigned long long)r * stride - start[r];
+    return r;
 }
 
 __device__ __forceinline__ unsigned long long lb_word(uint32_t tag, bool inc, uint32_t v) {
@@ -71,10 +69,10 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const int shift = pass * BITS;
+    const int shift = a.shift0 + pass * BITS;
     uint32_t n = a.n;
     if (a.n_dev) {
-        unsigned long long nd = *a.n_dev;
+        unsigned long long nd = (pass > 0 && a.n_dev_rest) ? *a.n_dev_rest : *a.n_dev;
         n = (uint32_t)(nd > a.cap ? a.cap : nd);
     }
     const uint32_t ntiles = a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE;
@@ -99,15 +97,22 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= ntiles) break;
-        const TileInfo ti = tile_info(a.seg, n, tile, TILE);
+        TileInfo ti = tile_info(a.seg, n, tile, TILE);
         const bool first = a.seg.tile_seg ? (tile == a.seg.seg_tile0[ti.seg]) : (tile == 0u);
 
         K key[ITEMS];
         uint32_t val[ITEMS], rank[ITEMS], dig[ITEMS];
         const uint32_t base = ti.lo + wave * 64u * ITEMS + lane;
         const K* src = kin;
-        if (pass == 0 && a.nregions > 1)  // first pass of a pair sort reads the sweep's regions
-            src = kin + region_offset(a.region_start, a.region_stride, a.nregions, ti.lo);
+        if (pass == 0 && a.nregions > 1) {
+            // first pass of a pair sort: the sweep's regions, without their
+            // padding (the tile's pairs are a prefix of it, so the valid items
+            // stay a prefix of every wave's row)
+            const uint32_t r = region_of(a.region_start, a.nregions, ti.lo);
+            src = kin + ((unsigned long long)r * a.region_stride - a.region_start[r]);
+            const unsigned long long ve = a.region_valid[r];
+            if (ve < ti.hi) ti.hi = (uint32_t)ve;
+        }
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t idx = base + 64u * k;
@@ -252,8 +257,9 @@ template <class KT, int BITS>
 __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__ keys, uint32_t n_host,
                                               const unsigned long long* __restrict__ n_dev,
                                               unsigned long long cap, uint32_t* __restrict__ hist,
-                                              int npass, int bbits, int flip_float,
+                                              int npass, int bbits, int flip_float, int shift0,
                                               const unsigned long long* region_start,
+                                              const unsigned long long* region_valid,
                                               unsigned long long region_stride,
                                               uint32_t nregions) {
     constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u;
@@ -273,29 +279,34 @@ __global__ __launch_bounds__(kWG) void k_hist(const typename KT::T* __restrict__
     const uint32_t stride = gridDim.x * kWG;
     for (uint32_t b0 = blockIdx.x * kWG + (threadIdx.x & ~63u); b0 < n; b0 += UNROLL * stride) {
         typename KT::T kk[UNROLL];
+        uint32_t lim[UNROLL];  // end of the chunk's valid keys (a region's padding excluded)
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; ++u) {
             const uint32_t c0 = b0 + u * stride;  // no wrap: n + UNROLL * stride < 2^32 (n < 2^32 - 2^22)
             kk[u] = typename KT::T{};
-            if (c0 + lane < n) {
+            lim[u] = n;
+            if (c0 < n) {
                 const typename KT::T* src = keys;
-                if (nregions > 1)
-                    src = keys + region_offset(region_start, region_stride, nregions, c0);
-                kk[u] = src[c0 + lane];
+                if (nregions > 1) {
+                    const uint32_t r = region_of(region_start, nregions, c0);
+                    src = keys + ((unsigned long long)r * region_stride - region_start[r]);
+                    if (region_valid[r] < lim[u]) lim[u] = (uint32_t)region_valid[r];
+                }
+                if (c0 + lane < lim[u]) kk[u] = src[c0 + lane];
             }
         }
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; ++u) {
             const uint32_t c0 = b0 + u * stride;
             if (c0 >= n) break;  // wave-uniform
-            const bool valid = c0 + lane < n;
+            const bool valid = c0 + lane < lim[u];
             typename KT::T k = kk[u];
             if constexpr (KT::kHasVals) {
                 if (flip_float) k = flip_f32(__uint_as_float(k));
             }
             const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
             for (int p = 0; p < npass; ++p) {
-                const uint32_t d = KT::digit(k, p * BITS, bbits, MASK);
+                const uint32_t d = KT::digit(k, shift0 + p * BITS, bbits, MASK);
                 const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
                 const bool head = valid && (lane == 0u || prev != d);
                 const unsigned long long nxt = ballot(head) & above;
@@ -403,9 +414,9 @@ hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int
     if (grid == 0) grid = 1;
     if (grid > 2048) grid = 2048;
     switch (bits) {
-        case 6: hipLaunchKernelGGL((k_hist<KU32, 6>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, nullptr, 0ull, 1u); break;
-        case 7: hipLaunchKernelGGL((k_hist<KU32, 7>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, nullptr, 0ull, 1u); break;
-        case 8: hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, nullptr, 0ull, 1u); break;
+        case 6: hipLaunchKernelGGL((k_hist<KU32, 6>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, 0, nullptr, nullptr, 0ull, 1u); break;
+        case 7: hipLaunchKernelGGL((k_hist<KU32, 7>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, 0, nullptr, nullptr, 0ull, 1u); break;
+        case 8: hipLaunchKernelGGL((k_hist<KU32, 8>), dim3(grid), dim3(kWG), 0, st, keys, n, nullptr, 0ull, hist, npass, 0, flip_float, 0, nullptr, nullptr, 0ull, 1u); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -413,14 +424,15 @@ hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int
 
 hipError_t launch_pair_hist(const uint2* pairs, const unsigned long long* n_dev,
                             unsigned long long cap, uint32_t* hist, int npass, int bbits, int bits,
-                            const unsigned long long* rs, unsigned long long rstride, uint32_t nreg,
-                            hipStream_t st) {
+                            int shift0, const unsigned long long* rs,
+                            const unsigned long long* rv, unsigned long long rstride,
+                            uint32_t nreg, hipStream_t st) {
     unsigned long long g = (cap + kWG * 16 - 1) / (kWG * 16);
     uint32_t grid = (uint32_t)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
     switch (bits) {
-        case 6: hipLaunchKernelGGL((k_hist<KPair, 6>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, rs, rstride, nreg); break;
-        case 7: hipLaunchKernelGGL((k_hist<KPair, 7>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, rs, rstride, nreg); break;
-        case 8: hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, rs, rstride, nreg); break;
+        case 6: hipLaunchKernelGGL((k_hist<KPair, 6>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, shift0, rs, rv, rstride, nreg); break;
+        case 7: hipLaunchKernelGGL((k_hist<KPair, 7>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, shift0, rs, rv, rstride, nreg); break;
+        case 8: hipLaunchKernelGGL((k_hist<KPair, 8>), dim3(grid), dim3(kWG), 0, st, pairs, 0u, n_dev, cap, hist, npass, bbits, 0, shift0, rs, rv, rstride, nreg); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

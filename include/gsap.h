@@ -143,7 +143,8 @@ int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
 /* Per-stage timing of the last gsap_sap_collide with timing enabled (ms,
  * hipEvents on the launch stream).  Stages, in order: moments+project,
  * object_sort, gather, sweep, then (unless count-only) pair_hist, one entry
- * per pair-sort pass, pair_tail (row starts / scene offsets); last: finish. */
+ * per pair-sort onesweep pass, pair_local (only with gsap_tune.pair_local),
+ * pair_tail (row starts / scene offsets); last: finish. */
 int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages, int32_t* n_stages);
 int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable);
 
@@ -162,6 +163,9 @@ typedef struct gsap_tune {
     int32_t sweep_chunk; /* sweep: sub-tiles per XCD chunk, 0 (round-robin)/4/16/64       */
     int32_t pair_regions;/* sweep output regions with one allocation counter each, 1 or 8;
                           * 8 holds 8x the pair capacity in device memory              */
+    int32_t pair_local;  /* pair sort: 0 = onesweep passes over the whole key; 1 = onesweep
+                          * passes over the high key bits only (8-bit digits, pair_bits
+                          * unused), then each run of equal high bits sorted in LDS        */
     float ms;            /* out: measured update time of this configuration (autotune)    */
     int32_t from_cache;  /* out: 1 if the configuration came from the cache file          */
 } gsap_tune;
@@ -169,7 +173,8 @@ int32_t gsap_sap_get_tune(gsap_sap* h, gsap_tune* out);
 int32_t gsap_sap_set_tune(gsap_sap* h, const gsap_tune* t);
 /* Time the update of this input under candidate configurations (a staged
  * search: sweep kind, then object sort items x digit width, then pair sort
- * items x digit width x grid), keep the fastest in the handle and return it.
+ * items x digit width x grid, with and without the local stage), keep the
+ * fastest in the handle and return it.
  * h_scene_offsets == NULL tunes the single-scene update of d_in[0..n)
  * (d_group as in gsap_sap_collide); otherwise the batched update of n_scenes
  * scenes (n and d_group are ignored).
@@ -243,7 +248,7 @@ int32_t gsap_radix_sort_f32(gsap_radix_sort* s, void* stream, float* d_keys, uin
 typedef struct gsap_prb gsap_prb;
 
 enum {
-    GSAP_PRB_ABS_SPRING = 1u,     /* k, c absolute (BASELINE scene) not mass/dt derived */
+    GSAP_PRB_ABS_SPRING = 1u,     /* k, c absolute (walled cube scene), not mass/dt derived */
     GSAP_PRB_WALLS = 2u,          /* x and z walls                                       */
     GSAP_PRB_FLOOR_CLAMP = 4u,    /* reference y clamp in integrate                      */
     GSAP_PRB_CAP_VEL = 8u,        /* velocity cap (setMaxVelocity)                       */

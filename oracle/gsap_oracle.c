@@ -714,7 +714,7 @@ static inline void spring_coeff(float im, float jm, float dt2inv, float dtinv, c
 /* One boundary plane: penetration along unit normal nrm (floor: y = 0 with
  * normal +y), force -k*(dist-r)*n*scale + c*(-v): clearForceAndBoundary-
  * ConditionK, ParticleRigidBodyKernel.h:106-140.  Walls add the same law for
- * the x and z planes (BASELINE scene), applied after the floor. */
+ * the x and z planes (the walled box of falling cubes), applied after the floor. */
 static inline int plane_force(f4* out, float dist, f4 nrm, f4 pos, f4 vel, float mass,
                               float dt2inv, float dtinv, const or_prb_params* p) {
     if (!(dist < pos.w)) return 0;

@@ -1,0 +1,133 @@
+"""Pair sort with the local stage: onesweep passes over the high key bits only,
+then every run of equal high bits ("bucket") sorted by the low bits in LDS, or
+through global memory when a window's buckets do not fit.  The canonical pair
+list must be bit-identical to the brute-force oracle and to the whole-key
+sort, across key widths (local bits L = 8 ... 24, one to three onesweep
+passes), bucket shapes (one bucket larger than a workgroup's LDS sort, windows
+with more buckets than a 32-bit LDS key can rank), output kinds, batched
+scenes, pair regions with padding, and a capacity overflow."""
+import numpy as np
+import pytest
+
+import gsap_amd as G
+from gsap_amd import workloads as W
+from oracle import oracle as O
+from tests.gpu_util import pairs_np, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(b, dev, **tune):
+    s = G.Sap()
+    s.set_tune(**tune)
+    pairs, _, _ = s.collide(to_dev(b, dev))
+    return pairs_np(pairs)
+
+
+# n -> key bits K = 2 * bits(n): 200 -> 16 (L = 8), 3000 -> 24 (L = 16),
+# 20000 -> 30 (L = 22, one onesweep pass)
+@pytest.mark.parametrize("n", [200, 3000, 20000])
+@pytest.mark.parametrize("regions", [1, 8])
+def test_local_equals_brute(dev, n, regions):
+    b = W.scene(n, np.random.default_rng(n), "uniform")
+    ref = O.brute(b)
+    for items, grid in ((8, 512), (16, 4096)):
+        got = _pairs(b, dev, pair_local=1, pair_items=items, pair_grid=grid,
+                     pair_regions=regions)
+        assert np.array_equal(got, ref), (items, grid)
+
+
+def test_local_oversized_bucket_equals_brute(dev):
+    # boxes 0..2 span the whole scene: their buckets (ids 0..2 are the smaller
+    # index of ~20000 pairs each) exceed the LDS sort and take the global path
+    b = W.scene(20000, np.random.default_rng(5), "cluster").copy()
+    lo, hi = b[:, :3].min(0), b[:, 3:].max(0)
+    for i in (0, 1, 2, 19999):
+        b[i, :3] = lo - 1.0
+        b[i, 3:] = hi + 1.0
+    ref = O.brute(b)
+    for slabs in (0, 1):
+        for regions in (1, 8):
+            got = _pairs(b, dev, pair_local=1, slabs=slabs, pair_regions=regions)
+            assert np.array_equal(got, ref), (slabs, regions)
+
+
+@pytest.mark.parametrize("n,sigma", [(300_000, (20.0, 5.0, 1.0)),    # K = 38, two passes, L = 22
+                                     (700_000, (20.0, 5.0, 1.0)),    # K = 40, L = 24
+                                     (1_100_000, (60.0, 30.0, 20.0))])  # K = 42, three passes
+def test_local_equals_whole_key_sort(dev, n, sigma):
+    import torch
+    b = W.aniso(n, seed=3, sigma=sigma)
+    bd = to_dev(b, dev)
+    ref, _, _ = G.Sap().collide(bd)
+    for slabs in (0, 1):
+        for regions in (1, 8):
+            s = G.Sap()
+            s.set_tune(pair_local=1, slabs=slabs, pair_regions=regions)
+            got, _, _ = s.collide(bd)
+            assert torch.equal(got, ref), (slabs, regions)
+
+
+def test_local_sparse_windows_many_buckets(dev):
+    # 700,000 tiny boxes (K = 40, L = 24): about one pair per box, so a window
+    # of 2048 pairs spans more buckets than a 32-bit LDS key can rank (256)
+    import torch
+    r = np.random.default_rng(9)
+    c = r.uniform(0.0, 400.0, (700_000, 3))
+    h = r.uniform(0.2, 0.6, (700_000, 3))
+    bd = to_dev(W.boxes_from(c, h), dev)
+    ref, _, _ = G.Sap().collide(bd)
+    assert 0 < ref.shape[0] < 4 * 700_000
+    s = G.Sap()
+    s.set_tune(pair_local=1)
+    got, _, _ = s.collide(bd)
+    assert torch.equal(got, ref)
+
+
+def test_local_sym_csr_grouped(dev):
+    import torch
+    sph = np.concatenate([np.random.default_rng(1).uniform(0, 30, (20000, 3)),
+                          np.full((20000, 1), 0.5)], 1).astype(np.float32)
+    g = to_dev((np.arange(20000) // 27).astype(np.int32), dev)
+    out = []
+    for loc in (0, 1):
+        h = G.Sap(input_kind=G.INPUT_SPHERE, output_kind=G.OUT_SYM_CSR,
+                  flags=G.FLAG_SKIP_SAME_GROUP)
+        h.set_tune(pair_local=loc)
+        out.append(h.collide(to_dev(sph, dev), group=g))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
+def test_local_batched(dev):
+    import torch
+    boxes, off = W.scenes(48, seed=8, lo=1000, hi=20000)
+    bd = to_dev(boxes, dev)
+    res = []
+    for loc in (0, 1):
+        s = G.Sap()
+        s.set_tune(pair_local=loc, pair_regions=8)
+        P, _ = s.collide_batched(bd, off, None, None)
+        pairs = torch.empty((P, 2), dtype=torch.int32, device=dev)
+        po = torch.empty(len(off), dtype=torch.int64, device=dev)
+        for _ in range(2):  # capture, then replay
+            s.collide_batched(bd, off, pairs, po)
+            res.append((pairs.clone(), po.clone()))
+    for p, o in res[1:]:
+        assert torch.equal(p, res[0][0]) and torch.equal(o, res[0][1])
+
+
+@pytest.mark.parametrize("loc,regions", [(0, 8), (1, 1), (1, 8)])
+def test_local_capacity_error_then_retry(dev, loc, regions):
+    import torch
+    b = W.small()
+    ref = O.brute(b)
+    s = G.Sap()
+    s.set_tune(pair_local=loc, pair_regions=regions)
+    bd = to_dev(b, dev)
+    small_buf = torch.empty((100, 2), dtype=torch.int32, device=dev)  # about 875 pairs
+    with pytest.raises(G.GsapError) as e:
+        s.collide_into(bd, small_buf)
+    assert e.value.code == G.ERR_CAPACITY
+    big = torch.empty((len(ref), 2), dtype=torch.int32, device=dev)
+    P, _, _ = s.collide_into(bd, big)
+    assert P == len(ref) and np.array_equal(pairs_np(big), ref)

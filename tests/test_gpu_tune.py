@@ -48,7 +48,8 @@ def test_digit_widths_batched(dev):
 def test_set_tune_rejects_invalid(dev):
     s = G.Sap()
     for bad in (dict(sort_items=5), dict(sort_bits=5), dict(pair_bits=9), dict(pair_items=12),
-                dict(pair_grid=100), dict(slabs=2), dict(sweep_chunk=5), dict(pair_regions=4)):
+                dict(pair_grid=100), dict(slabs=2), dict(sweep_chunk=5), dict(pair_regions=4),
+                dict(pair_local=2)):
         with pytest.raises(G.GsapError) as e:
             s.set_tune(**bad)
         assert e.value.code == G.ERR_INVALID_ARG
@@ -72,7 +73,7 @@ def test_autotune_single_and_cache(dev, tmp_path):
     t2 = s2.autotune(bd, cache=cache)
     assert t2["from_cache"] == 1
     for k in ("sort_items", "sort_bits", "pair_items", "pair_bits", "pair_grid", "slabs",
-              "sweep_chunk", "pair_regions"):
+              "sweep_chunk", "pair_regions", "pair_local"):
         assert t2[k] == t[k]
     with open(cache) as f:
         assert len(f.read().splitlines()) == 1
