@@ -406,14 +406,22 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     w.slot_cap = count_only ? 0 : h->slot_cap;
     w.region_stride = h->region_stride;
     w.nregions = count_only ? 1u : (uint32_t)h->pair_regions;
+    // with the local stage (at most kSweepHistPasses onesweep passes) the
+    // sweep counts the pair sort's digits as it writes the pairs
+    const bool sweep_hist = !count_only && local && npass_pair <= kSweepHistPasses;
+    w.hist = sweep_hist ? a.hist_pair : nullptr;
+    w.hist_npass = (int)npass_pair;
+    w.hist_shift0 = (int)shift0;
+    w.bbits = (int)bbits;
     HIPCHK(h, launch_sweep(w, st));
     mark(h, st);
     if (!count_only) {
         const uint32_t nreg = (uint32_t)h->pair_regions;
         HIPCHK(h, launch_pair_regions(ctrl, h->slot_cap, nreg, sort_tile(h->pair_items), st));
-        HIPCHK(h, launch_pair_hist(t0p, &ctrl->sort_n, h->sort_cap, a.hist_pair, npass_pair, bbits,
-                                   pair_bits, shift0, ctrl->region_start, ctrl->region_valid,
-                                   h->region_stride, nreg, st));
+        if (!sweep_hist)
+            HIPCHK(h, launch_pair_hist(t0p, &ctrl->sort_n, h->sort_cap, a.hist_pair, npass_pair,
+                                       bbits, pair_bits, shift0, ctrl->region_start,
+                                       ctrl->region_valid, h->region_stride, nreg, st));
         mark(h, st);
         SortArgs p{};
         p.n_dev = &ctrl->sort_n;

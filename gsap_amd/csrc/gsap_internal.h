@@ -127,6 +127,12 @@ struct SweepArgs {
     int count_only;
     int slabs;               // entries carry kFirstBit in orig and their slab's first position in aux
     uint32_t xcd_chunk;      // sub-tiles per XCD chunk (0: hardware round-robin)
+    // if set: 8-bit digit histograms of the written pairs' keys (a << bbits) | b
+    // at bits hist_shift0 + 8 q, q < hist_npass <= kSweepHistPasses
+    uint32_t* hist;
+    int hist_npass;
+    int hist_shift0;
+    int bbits;
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
@@ -229,7 +235,8 @@ hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long
                              unsigned long long cap, uint32_t* wstart, int bbits, int L,
                              hipStream_t st);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
-constexpr uint32_t kSweepStageSlab = 600;  // the same, slab sweep (fewer flushes)
+constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
+constexpr uint32_t kSweepHistPasses = 2;   // pair-sort digit histograms the sweep can count
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,
                            int flip_float, hipStream_t st);

@@ -572,6 +572,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     constexpr uint32_t kCap = SLAB ? kSweepStageSlab : kStage;
     constexpr uint32_t kSt = MODE == 2 ? 1u : kCap;
     __shared__ uint2 s_stage[kWaves][kSt];
+    // digit histograms of the pairs written, for the pair sort's first passes
+    __shared__ uint32_t s_ph[MODE == 2 ? 1 : kSweepHistPasses * 256];
     __shared__ uint32_t s_fill[kWaves];
     __shared__ uint32_t s_np[kWaves];
     __shared__ unsigned long long s_ex[kWaves];
@@ -585,6 +587,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     const uint32_t lo = ti.lo + sub * kSweepTile;
     if (lo >= ti.hi) return;  // the whole workgroup
     if (threadIdx.x == 0) s_done = 0u;
+    if constexpr (MODE != 2) {
+        if (a.hist)
+            for (uint32_t i = threadIdx.x; i < kSweepHistPasses * 256u; i += kWG) s_ph[i] = 0u;
+    }
     __syncthreads();  // the only block-level barrier
     const uint32_t hi = min(lo + kSweepTile, ti.hi);
     const uint32_t lane = lane_id(), w = uniform(threadIdx.x >> 6);
@@ -609,6 +615,22 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         base = __builtin_amdgcn_readfirstlane((uint32_t)base) |
                ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32);
         return base;
+    };
+    // write staged pair i of stage st at base + i (within the capacity) and
+    // count its sort digits
+    auto put = [&](const uint2* st, uint32_t i, unsigned long long base) {
+        if (base + i < cap) {
+            const uint2 pr = st[i];
+            rout[base + i] = pr;
+            if constexpr (MODE != 2) {
+                if (a.hist) {
+                    const unsigned long long k =
+                        ((unsigned long long)pr.x << a.bbits) | (unsigned long long)pr.y;
+                    for (int q = 0; q < a.hist_npass; ++q)
+                        atomicAdd(&s_ph[q * 256 + (uint32_t)((k >> (a.hist_shift0 + 8 * q)) & 255u)], 1u);
+                }
+            }
+        }
     };
 
     if (g0 < hi) {
@@ -701,8 +723,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 if constexpr (MODE != 2) {
                     if (fill + cnt > kCap) {  // stage full: write it out
                         const unsigned long long base = flush(fill);
-                        for (uint32_t i = lane; i < fill; i += 64u)
-                            if (base + i < cap) rout[base + i] = stage[i];
+                        for (uint32_t i = lane; i < fill; i += 64u) put(stage, i, base);
                         fill = 0;
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -778,10 +799,13 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 #pragma unroll
             for (int v = 0; v < kWaves; ++v) {
                 const uint32_t f = s_fill[v];
-                for (uint32_t i = lane; i < f; i += 64u)
-                    if (base + i < cap) rout[base + i] = s_stage[v][i];
+                for (uint32_t i = lane; i < f; i += 64u) put(s_stage[v], i, base);
                 base += f;
             }
+        }
+        if (a.hist) {  // every wave's digits are counted (they arrived after counting)
+            for (uint32_t i = lane; i < (uint32_t)a.hist_npass * 256u; i += 64u)
+                if (s_ph[i]) atomicAdd(&a.hist[i], s_ph[i]);
         }
     }
 }
