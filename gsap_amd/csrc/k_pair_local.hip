@@ -182,187 +182,229 @@ __global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2
     const uint32_t J = (n + kLocW - 1) / kLocW;
     const uint32_t lowmask = (1u << L) - 1u;  // 8 <= L <= 24
 
-    for (uint32_t j = blockIdx.x; j < J; j += gridDim.x) {
-        const uint32_t s = wstart[j], e = wstart[j + 1];
-        if (s >= e) continue;  // no bucket starts in this window
-        const uint32_t S = e - s;
-        // 32-bit keys relative to the window's first bucket: (high - hs) << L | low
-        const unsigned long long hs = pkey(X[s], bbits) >> L;
-        const unsigned long long he = pkey(X[e - 1], bbits) >> L;
-        bool in_lds = S <= kLocC && (he - hs) < (1ull << (32 - L));
-        if (in_lds) {
-            const uint32_t nk = (S + kLT - 1) / kLT;
-            uint32_t kk[kLocItems], vor = 0u, vand = ~0u;
+    // Software pipeline over this workgroup's windows j, j + G, ...: window
+    // starts are read two windows ahead, and the next window's pairs are
+    // loaded into registers while this window is sorted.
+    const uint32_t G = gridDim.x;
+    uint32_t j = blockIdx.x;
+    uint32_t s = 0, e = 0, s1 = 0, e1 = 0;
+    if (j < J) {
+        s = wstart[j];
+        e = wstart[j + 1];
+    }
+    if (j + G < J) {
+        s1 = wstart[j + G];
+        e1 = wstart[j + G + 1];
+    }
+    uint2 raw[kLocItems], rfirst = make_uint2(0u, 0u), rlast = make_uint2(0u, 0u);
+    auto fetch = [&](uint32_t fs, uint32_t fe) {  // an LDS-sized window's pairs
+        const uint32_t fS = fe - fs;
+        if (fS == 0 || fS > kLocC) return;
+        const uint32_t fnk = (fS + kLT - 1) / kLT;
+        rfirst = X[fs];
+        rlast = X[fe - 1];
 #pragma unroll
-            for (int k = 0; k < kLocItems; ++k) {
-                const uint32_t q = tid + (uint32_t)kLT * k;
-                kk[k] = 0u;
-                if ((uint32_t)k < nk && q < S) {
-                    const unsigned long long v = pkey(X[s + q], bbits);
-                    kk[k] = ((uint32_t)((v >> L) - hs) << L) | ((uint32_t)v & lowmask);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kLocItems; ++k) {
-                const uint32_t q = tid + (uint32_t)kLT * k;
-                if ((uint32_t)k < nk && q < S) {
-                    vor |= kk[k];
-                    vand &= kk[k];
-                }
-            }
-            vor = wave_or_u32(vor);
-            vand = wave_and_u32(vand);
-            if (lane == 0) {
-                s_red[wave] = vor;
-                s_red[kLWaves + wave] = vand;
-            }
-            for (uint32_t i = tid; i < kBins; i += kLT) s_work[i] = 0u;
-            __syncthreads();
-            auto emit = [&](uint32_t pos, uint32_t v) {
-                if (s + pos < out_lim)
-                    out[s + pos] = pkey_pair(((hs + (v >> L)) << L) | (v & lowmask), bbits);
-            };
-            // Counting pass on the 12 highest bits in which the window's keys
-            // differ (the bits above are common to all of them), then each
-            // key's place in its bin = the number of the bin's smaller keys.
-            vor = 0u;
-            vand = ~0u;
-#pragma unroll
-            for (int w = 0; w < kLWaves; ++w) {
-                vor |= s_red[w];
-                vand &= s_red[kLWaves + w];
-            }
-            const uint32_t diff = vor ^ vand;
-            const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
-            const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
-#pragma unroll
-            for (int k = 0; k < kLocItems; ++k) {
-                const uint32_t q = tid + (uint32_t)kLT * k;
-                if ((uint32_t)k < nk && q < S) atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u);
-            }
-            __syncthreads();
-            // bin starts (exclusive scan, kBins / kLT bins per thread) and the largest bin
-            constexpr uint32_t BPT = kBins / kLT;
-            uint32_t cb[BPT], sum = 0, mx = 0, all;
-#pragma unroll
-            for (uint32_t i = 0; i < BPT; ++i) {
-                cb[i] = s_work[tid * BPT + i];
-                sum += cb[i];
-                mx = cb[i] > mx ? cb[i] : mx;
-            }
-            mx = wave_max_u32(mx);
-            if (lane == 0) s_red[2 * kLWaves + wave] = mx;
-            uint32_t run = lblock_excl_scan(sum, s_tmp, &all);
-#pragma unroll
-            for (uint32_t i = 0; i < BPT; ++i) {
-                s_work[tid * BPT + i] = run;
-                run += cb[i];
-            }
-            mx = 0;
-#pragma unroll
-            for (int w = 0; w < kLWaves; ++w)
-                mx = s_red[2 * kLWaves + w] > mx ? s_red[2 * kLWaves + w] : mx;
-            __syncthreads();
-            if (mx <= kBinMax) {
-                // keys to their bins (any order inside a bin); a bin's counter
-                // ends at the next bin's start
-#pragma unroll
+        for (int k = 0; k < kLocItems; ++k) {
+            const uint32_t q = tid + (uint32_t)kLT * k;
+            raw[k] = make_uint2(0u, 0u);
+            if ((uint32_t)k < fnk && q < fS) raw[k] = X[fs + q];
+        }
+    };
+    fetch(s, e);
+    for (; j < J; j += G) {
+        uint32_t s2 = 0, e2 = 0;
+        if (j + 2u * G < J) {
+            s2 = wstart[j + 2u * G];
+            e2 = wstart[j + 2u * G + 1u];
+        }
+        bool fetched = false;
+        if (s < e) {  // some bucket starts in this window
+            const uint32_t S = e - s;
+            // 32-bit keys relative to the window's first bucket: (high - hs) << L | low
+            const unsigned long long hs = pkey(rfirst, bbits) >> L;
+            const unsigned long long he = pkey(rlast, bbits) >> L;
+            bool in_lds = S <= kLocC && (he - hs) < (1ull << (32 - L));
+            if (in_lds) {
+                const uint32_t nk = (S + kLT - 1) / kLT;
+                uint32_t kk[kLocItems], vor = 0u, vand = ~0u;
+    #pragma unroll
                 for (int k = 0; k < kLocItems; ++k) {
                     const uint32_t q = tid + (uint32_t)kLT * k;
-                    if ((uint32_t)k < nk && q < S)
-                        s_e[atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
+                    kk[k] = 0u;
+                    if ((uint32_t)k < nk && q < S) {
+                        const unsigned long long v = pkey(raw[k], bbits);
+                        kk[k] = ((uint32_t)((v >> L) - hs) << L) | ((uint32_t)v & lowmask);
+                    }
+                }
+                fetch(s1, e1);
+                fetched = true;
+    #pragma unroll
+                for (int k = 0; k < kLocItems; ++k) {
+                    const uint32_t q = tid + (uint32_t)kLT * k;
+                    if ((uint32_t)k < nk && q < S) {
+                        vor |= kk[k];
+                        vand &= kk[k];
+                    }
+                }
+                vor = wave_or_u32(vor);
+                vand = wave_and_u32(vand);
+                if (lane == 0) {
+                    s_red[wave] = vor;
+                    s_red[kLWaves + wave] = vand;
+                }
+                for (uint32_t i = tid; i < kBins; i += kLT) s_work[i] = 0u;
+                __syncthreads();
+                auto emit = [&](uint32_t pos, uint32_t v) {
+                    if (s + pos < out_lim)
+                        out[s + pos] = pkey_pair(((hs + (v >> L)) << L) | (v & lowmask), bbits);
+                };
+                // Counting pass on the 12 highest bits in which the window's keys
+                // differ (the bits above are common to all of them), then each
+                // key's place in its bin = the number of the bin's smaller keys.
+                vor = 0u;
+                vand = ~0u;
+    #pragma unroll
+                for (int w = 0; w < kLWaves; ++w) {
+                    vor |= s_red[w];
+                    vand &= s_red[kLWaves + w];
+                }
+                const uint32_t diff = vor ^ vand;
+                const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
+                const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
+    #pragma unroll
+                for (int k = 0; k < kLocItems; ++k) {
+                    const uint32_t q = tid + (uint32_t)kLT * k;
+                    if ((uint32_t)k < nk && q < S) atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u);
                 }
                 __syncthreads();
-                for (uint32_t pz = tid; pz < S; pz += kLT) {
-                    const uint32_t x = s_e[pz];
-                    const uint32_t bn = (x >> sd) & (kBins - 1u);
-                    const uint32_t be = s_work[bn];
-                    const uint32_t bs = bn ? s_work[bn - 1u] : 0u;
-                    uint32_t rank = 0;
-                    for (uint32_t i = bs; i < be; ++i) {
-                        const uint32_t y = s_e[i];
-                        rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
-                    }
-                    emit(bs + rank, x);
+                // bin starts (exclusive scan, kBins / kLT bins per thread) and the largest bin
+                constexpr uint32_t BPT = kBins / kLT;
+                uint32_t cb[BPT], sum = 0, mx = 0, all;
+    #pragma unroll
+                for (uint32_t i = 0; i < BPT; ++i) {
+                    cb[i] = s_work[tid * BPT + i];
+                    sum += cb[i];
+                    mx = cb[i] > mx ? cb[i] : mx;
                 }
-            } else {
-                in_lds = false;  // a crowded bin: the global-memory path
+                mx = wave_max_u32(mx);
+                if (lane == 0) s_red[2 * kLWaves + wave] = mx;
+                uint32_t run = lblock_excl_scan(sum, s_tmp, &all);
+    #pragma unroll
+                for (uint32_t i = 0; i < BPT; ++i) {
+                    s_work[tid * BPT + i] = run;
+                    run += cb[i];
+                }
+                mx = 0;
+    #pragma unroll
+                for (int w = 0; w < kLWaves; ++w)
+                    mx = s_red[2 * kLWaves + w] > mx ? s_red[2 * kLWaves + w] : mx;
+                __syncthreads();
+                if (mx <= kBinMax) {
+                    // keys to their bins (any order inside a bin); a bin's counter
+                    // ends at the next bin's start
+    #pragma unroll
+                    for (int k = 0; k < kLocItems; ++k) {
+                        const uint32_t q = tid + (uint32_t)kLT * k;
+                        if ((uint32_t)k < nk && q < S)
+                            s_e[atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
+                    }
+                    __syncthreads();
+                    for (uint32_t pz = tid; pz < S; pz += kLT) {
+                        const uint32_t x = s_e[pz];
+                        const uint32_t bn = (x >> sd) & (kBins - 1u);
+                        const uint32_t be = s_work[bn];
+                        const uint32_t bs = bn ? s_work[bn - 1u] : 0u;
+                        uint32_t rank = 0;
+                        for (uint32_t i = bs; i < be; ++i) {
+                            const uint32_t y = s_e[i];
+                            rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
+                        }
+                        emit(bs + rank, x);
+                    }
+                } else {
+                    in_lds = false;  // a crowded bin: the global-memory path
+                }
+                __syncthreads();  // s_e and s_work are reused
             }
-            __syncthreads();  // s_e and s_work are reused
-        }
-        if (!in_lds) {
-            // Through global memory: keys relative to the range's first bucket,
-            // T bits, LSD passes X -> Y -> X ... with the last one into out.
-            const unsigned long long hs = pkey(X[s], bbits) >> L;
-            const unsigned long long he = pkey(X[e - 1], bbits) >> L;
-            const unsigned long long kb = hs << L;
-            const unsigned long long kmax = ((he - hs) << L) | lowmask;
-            const uint32_t T = 64u - (uint32_t)__clzll((long long)kmax);
-            const uint32_t npass = (T + 7u) / 8u;
-            for (uint32_t p = 0; p < npass; ++p) {
-                const bool last = p + 1 == npass;
-                const uint2* src = (p & 1u) ? Y : X;
-                uint2* dst = last ? out : ((p & 1u) ? X : Y);
-                const uint32_t sh = 8u * p;
-                const int bw = (int)min(8u, T - sh);
-                const uint32_t mask = (1u << bw) - 1u;
-                if (tid < 256u) s_base[tid] = 0u;
-                __syncthreads();
-                for (uint32_t q = tid; q < S; q += kLT)
-                    atomicAdd(&s_base[(uint32_t)((pkey(src[s + q], bbits) - kb) >> sh) & mask], 1u);
-                __syncthreads();
-                uint32_t all;
-                const uint32_t c0v = tid < 256u ? s_base[tid] : 0u;
-                const uint32_t ex = lblock_excl_scan(c0v, s_tmp, &all);
-                if (tid < 256u) s_base[tid] = s + ex;
-                __syncthreads();
-                for (uint32_t c0 = 0; c0 < S; c0 += kGlobChunk) {
-                    const uint32_t cs = min(S - c0, kGlobChunk);
-                    const uint32_t nk = (cs + kLT - 1) / kLT;
-                    const uint32_t w0 = c0 + wave * 64u * nk;
-                    for (uint32_t i = lane; i < 256u; i += 64u) s_wh[wave][i] = 0u;
-                    unsigned long long kk[kGlobItems];
-                    uint32_t rk[kGlobItems];
-#pragma unroll
-                    for (int k = 0; k < kGlobItems; ++k) {
-                        const uint32_t q = w0 + 64u * k + lane;
-                        kk[k] = 0ull;
-                        if ((uint32_t)k < nk && q < c0 + cs) kk[k] = pkey(src[s + q], bbits) - kb;
-                    }
-#pragma unroll
-                    for (int k = 0; k < kGlobItems; ++k) {
-                        if ((uint32_t)k < nk) {
-                            const uint32_t q = w0 + 64u * k + lane;
-                            rk[k] = wave_rank((uint32_t)(kk[k] >> sh) & mask, q < c0 + cs, bw,
-                                              s_wh[wave], gtm);
-                        }
-                    }
+            if (!in_lds) {
+                // Through global memory: keys relative to the range's first bucket,
+                // T bits, LSD passes X -> Y -> X ... with the last one into out.
+                const unsigned long long hs = pkey(X[s], bbits) >> L;
+                const unsigned long long he = pkey(X[e - 1], bbits) >> L;
+                const unsigned long long kb = hs << L;
+                const unsigned long long kmax = ((he - hs) << L) | lowmask;
+                const uint32_t T = 64u - (uint32_t)__clzll((long long)kmax);
+                const uint32_t npass = (T + 7u) / 8u;
+                for (uint32_t p = 0; p < npass; ++p) {
+                    const bool last = p + 1 == npass;
+                    const uint2* src = (p & 1u) ? Y : X;
+                    uint2* dst = last ? out : ((p & 1u) ? X : Y);
+                    const uint32_t sh = 8u * p;
+                    const int bw = (int)min(8u, T - sh);
+                    const uint32_t mask = (1u << bw) - 1u;
+                    if (tid < 256u) s_base[tid] = 0u;
                     __syncthreads();
-                    if (tid < 256u) {
-                        uint32_t run = s_base[tid];
-#pragma unroll
-                        for (int w = 0; w < kLWaves; ++w) {
-                            const uint32_t cw = s_wh[w][tid];
-                            s_wh[w][tid] = run;
-                            run += cw;
-                        }
-                        s_base[tid] = run;
-                    }
+                    for (uint32_t q = tid; q < S; q += kLT)
+                        atomicAdd(&s_base[(uint32_t)((pkey(src[s + q], bbits) - kb) >> sh) & mask], 1u);
                     __syncthreads();
-#pragma unroll
-                    for (int k = 0; k < kGlobItems; ++k) {
-                        if ((uint32_t)k < nk) {
+                    uint32_t all;
+                    const uint32_t c0v = tid < 256u ? s_base[tid] : 0u;
+                    const uint32_t ex = lblock_excl_scan(c0v, s_tmp, &all);
+                    if (tid < 256u) s_base[tid] = s + ex;
+                    __syncthreads();
+                    for (uint32_t c0 = 0; c0 < S; c0 += kGlobChunk) {
+                        const uint32_t cs = min(S - c0, kGlobChunk);
+                        const uint32_t nk = (cs + kLT - 1) / kLT;
+                        const uint32_t w0 = c0 + wave * 64u * nk;
+                        for (uint32_t i = lane; i < 256u; i += 64u) s_wh[wave][i] = 0u;
+                        unsigned long long kk[kGlobItems];
+                        uint32_t rk[kGlobItems];
+    #pragma unroll
+                        for (int k = 0; k < kGlobItems; ++k) {
                             const uint32_t q = w0 + 64u * k + lane;
-                            if (q < c0 + cs) {
-                                const uint32_t g = s_wh[wave][(uint32_t)(kk[k] >> sh) & mask] + rk[k];
-                                if (!last || g < out_lim) dst[g] = pkey_pair(kk[k] + kb, bbits);
+                            kk[k] = 0ull;
+                            if ((uint32_t)k < nk && q < c0 + cs) kk[k] = pkey(src[s + q], bbits) - kb;
+                        }
+    #pragma unroll
+                        for (int k = 0; k < kGlobItems; ++k) {
+                            if ((uint32_t)k < nk) {
+                                const uint32_t q = w0 + 64u * k + lane;
+                                rk[k] = wave_rank((uint32_t)(kk[k] >> sh) & mask, q < c0 + cs, bw,
+                                                  s_wh[wave], gtm);
                             }
                         }
+                        __syncthreads();
+                        if (tid < 256u) {
+                            uint32_t run = s_base[tid];
+    #pragma unroll
+                            for (int w = 0; w < kLWaves; ++w) {
+                                const uint32_t cw = s_wh[w][tid];
+                                s_wh[w][tid] = run;
+                                run += cw;
+                            }
+                            s_base[tid] = run;
+                        }
+                        __syncthreads();
+    #pragma unroll
+                        for (int k = 0; k < kGlobItems; ++k) {
+                            if ((uint32_t)k < nk) {
+                                const uint32_t q = w0 + 64u * k + lane;
+                                if (q < c0 + cs) {
+                                    const uint32_t g = s_wh[wave][(uint32_t)(kk[k] >> sh) & mask] + rk[k];
+                                    if (!last || g < out_lim) dst[g] = pkey_pair(kk[k] + kb, bbits);
+                                }
+                            }
+                        }
+                        __syncthreads();
                     }
-                    __syncthreads();
                 }
             }
         }
+        if (!fetched) fetch(s1, e1);
+        s = s1;
+        e = e1;
+        s1 = s2;
+        e1 = e2;
     }
 }
 
