@@ -178,7 +178,7 @@ def bench_particles(G, W, torch, dev, nb, steps, with_cpu, cache=None):
     prb.set_particle_shape_inv_inertia(sc["shape_inv_inertia"])
     prb.set_physics(broadphase_kind=G.INPUT_AABB, **kw)
     sap = G.Sap(max_n=npart, input_kind=G.INPUT_AABB, output_kind=G.OUT_SYM_CSR,
-                flags=G.FLAG_SKIP_SAME_GROUP | G.FLAG_SLABS)
+                axis_mode=G.AXIS_AUTO, flags=G.FLAG_SKIP_SAME_GROUP | G.FLAG_SLABS)
     dt = 1e-4
     prb.step(sap, dt)  # particle AABBs of the first step, to tune the broadphase on
     pp = prb.ptrs()
@@ -221,7 +221,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--axis", default="pca", choices=["pca", "coord", "learned"])
+    ap.add_argument("--axis", default="auto", choices=["auto", "pca", "coord", "learned"],
+                    help="sweep axis rule (auto: PCA or coordinate axes by the sweep-test model)")
     ap.add_argument("--no-slabs", action="store_true",
                     help="plain sweep along one axis (no slab split on a second axis)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -262,7 +263,8 @@ def main():
     boxes = W.aniso(args.n, seed=1)
     n = len(boxes)
     bd = torch.from_numpy(boxes).to(dev)
-    mode = {"pca": G.AXIS_PCA, "coord": G.AXIS_COORD_MAXVAR, "learned": G.AXIS_LEARNED}[args.axis]
+    mode = {"auto": G.AXIS_AUTO, "pca": G.AXIS_PCA, "coord": G.AXIS_COORD_MAXVAR,
+            "learned": G.AXIS_LEARNED}[args.axis]
     slabs = not args.no_slabs
     sap = G.Sap(max_n=n, axis_mode=mode, flags=G.FLAG_SLABS if slabs else 0)
     tune = sap.autotune(bd, cache=args.tune_cache)  # measured tiles, digit widths, sweep kind
@@ -327,7 +329,8 @@ def main():
         from oracle import oracle as O
         stride = args.cpu_stride
         t = time.perf_counter()
-        _, tests_s, _ = O.sap(boxes, axis_mode={"pca": O.AXIS_PCA}.get(args.axis, O.AXIS_PCA),
+        # the reference's own axis rule (coordinate axis of largest variance)
+        _, tests_s, _ = O.sap(boxes, axis_mode=O.AXIS_COORD_MAXVAR,
                               sample_stride=stride, cap=max(16, 4 * P // stride + 1024))
         setup, sweep, canon = O.last_times()
         est = setup + stride * (sweep + canon)
@@ -337,7 +340,8 @@ def main():
         cpu = {"value": round(n / est, 1), "unit": "updates/s", "cores": 1, "kind": "port",
                "reference_sweep_tests": ref_work["cell_tests"] + ref_work["cross_tests"],
                "reference_sweep_work": ref_work,
-               "sample": f"axis+projection+radix sort of all {n} boxes ({setup:.2f}s), sweep and "
+               "sample": f"coordinate max-variance axis, projection and radix sort of all {n} "
+                         f"boxes ({setup:.2f}s), sweep and "
                          f"canonical sort of every {stride}th sorted object ({sweep + canon:.2f}s), "
                          f"extrapolated x{stride}: {est:.1f}s per update; "
                          f"{time.perf_counter() - t:.1f}s measured"}
@@ -380,6 +384,16 @@ def main():
                                             stage_dict(np.array(plain.stage_times()),
                                                        plain.tune["pair_local"])[0].items()}}
         plain.close()
+        if args.axis != "pca":  # the same slab sweep with the PCA axis pair
+            pca = G.Sap(max_n=n, axis_mode=G.AXIS_PCA, flags=G.FLAG_SLABS)
+            pca.set_tune(**{k: v for k, v in sap.tune.items() if k not in ("ms", "from_cache")})
+            ms_pca = _time_ms(torch, lambda: pca.collide_into(bd, pairs), max(1, args.steps // 2), 1)
+            _, _, stq = pca.collide_into(bd, pairs)
+            out["pca_axis"] = {"ms_per_step": round(ms_pca, 4),
+                               "updates_per_s": round(n / (ms_pca / 1e3), 1),
+                               "sweep_tests": int(stq.sweep_tests),
+                               "axis_used": [round(x, 6) for x in stq.axis]}
+            pca.close()
     if not args.no_extras:
         out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup,
                                        args.tune_cache)

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libgsap.so")
 
 OK, ERR_INVALID_ARG, ERR_CAPACITY, ERR_NONFINITE, ERR_HIP, ERR_UNSUPPORTED, ERR_OOM = range(7)
 INPUT_AABB, INPUT_SPHERE = 0, 1
-AXIS_PCA, AXIS_COORD_MAXVAR, AXIS_FIXED, AXIS_LEARNED = 0, 1, 2, 3
+AXIS_PCA, AXIS_COORD_MAXVAR, AXIS_FIXED, AXIS_LEARNED, AXIS_AUTO = 0, 1, 2, 3, 4
 OUT_CANON, OUT_SYM_CSR = 0, 1
 FLAG_SKIP_SAME_GROUP = 1
 FLAG_SLABS = 2

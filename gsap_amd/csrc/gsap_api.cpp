@@ -138,7 +138,7 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
     *out = nullptr;
     if (desc->input_kind != GSAP_INPUT_AABB && desc->input_kind != GSAP_INPUT_SPHERE)
         return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad input_kind");
-    if (desc->axis_mode < GSAP_AXIS_PCA || desc->axis_mode > GSAP_AXIS_LEARNED)
+    if (desc->axis_mode < GSAP_AXIS_PCA || desc->axis_mode > GSAP_AXIS_AUTO)
         return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad axis_mode");
     if (desc->output_kind != GSAP_OUT_CANON && desc->output_kind != GSAP_OUT_SYM_CSR)
         return fail(nullptr, GSAP_ERR_INVALID_ARG, "gsap_sap_create: bad output_kind");
@@ -219,7 +219,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         h->cap_em = std::max(h->cap_em, em);
     }
     if (ntiles > h->cap_tiles) {
-        HIPCHK(h, grow(h->partials, (size_t)ntiles * 10 * sizeof(double)));
+        HIPCHK(h, grow(h->partials, (size_t)ntiles * kMomStride * sizeof(double)));
         HIPCHK(h, grow(h->lb_obj, ((size_t)ntiles * em + 1) * 256 * 8));
         HIPCHK(h, hipMemset(h->lb_obj.p, 0, ((size_t)ntiles * em + 1) * 256 * 8));
         if (slab_mode(h)) HIPCHK(h, grow(h->vr_partials, (size_t)ntiles * kVr * 4));

@@ -10,6 +10,9 @@ constexpr int kWG = 256;          // threads per workgroup for the object stages
 constexpr int kWaves = kWG / 64;  // wave64
 constexpr int kSweepTile = 256;   // sorted objects per sweep workgroup
 constexpr int kMaxPasses = 8;
+// Per-tile partials of k_moments: [0..8] sum c, sum c c^T; [9..11] sum of the
+// extents per coordinate; [12..14] largest extent per coordinate.
+constexpr int kMomStride = 16;
 
 // Scene (segment) table for the batched path.  A "tile" is a run of at most
 // `tile` consecutive objects of one scene; every object stage maps one tile
@@ -73,7 +76,7 @@ struct ObjPipeArgs {
     int axis_mode;          // GSAP_AXIS_*
     float fixed_axis[3];
     float* axis;            // [nseg][4] device: axis used (w = source)
-    double* partials;       // [ntiles][10] moment partials
+    double* partials;       // [ntiles][kMomStride] moment and extent partials
     Ctrl* ctrl;
     uint32_t* hist_obj;     // [nseg][npass][256]
     uint32_t npass_obj;

@@ -105,7 +105,7 @@ __device__ static void choose_axis(const double m[9], uint32_t n, int mode, cons
 }
 
 __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
-    __shared__ double s_red[kWaves][9];
+    __shared__ double s_red[kWaves][15];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
 
@@ -123,12 +123,13 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
         }
     }
 
-    double s[9];
+    // s[0..8]: sum c, sum c c^T; s[9..11]: sum of the extents; e[0..2]: largest extents
+    double s[12], e[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 9; ++k) s[k] = 0.0;
+    for (int k = 0; k < 12; ++k) s[k] = 0.0;
     bool bad = false;
     for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
-        double c[3];
+        double c[3], x[3];
         if (a.kind == 0) {
             const float2* b = reinterpret_cast<const float2*>(a.in) + 3 * (size_t)i;
             float2 p0 = b[0], p1 = b[1], p2 = b[2];
@@ -137,27 +138,43 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
             c[0] = ((double)p0.x + (double)p1.y) * 0.5;
             c[1] = ((double)p0.y + (double)p2.x) * 0.5;
             c[2] = ((double)p1.x + (double)p2.y) * 0.5;
+            x[0] = (double)p1.y - (double)p0.x;
+            x[1] = (double)p2.x - (double)p0.y;
+            x[2] = (double)p2.y - (double)p1.x;
         } else {
             float4 q = reinterpret_cast<const float4*>(a.in)[i];
             bad |= !(isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w));
             c[0] = q.x; c[1] = q.y; c[2] = q.z;
+            x[0] = x[1] = x[2] = 2.0 * (double)q.w;
         }
         s[0] += c[0]; s[1] += c[1]; s[2] += c[2];
         s[3] += c[0] * c[0]; s[4] += c[0] * c[1]; s[5] += c[0] * c[2];
         s[6] += c[1] * c[1]; s[7] += c[1] * c[2]; s[8] += c[2] * c[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s[9 + k] += x[k];
+            e[k] = fmax(e[k], x[k]);
+        }
     }
     const uint32_t w = tid >> 6, l = lane_id();
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 12; ++k) {
         double v = wave_sum(s[k]);
         if (l == 0) s_red[w][k] = v;
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double v = e[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if (l == 0) s_red[w][12 + k] = v;
+    }
     if (ballot(bad) && l == 0) atomicOr(&a.ctrl->flags, kFlagNonFinite);
     __syncthreads();
-    if (tid < 9) {
+    if (tid < 15) {
         double v = 0.0;
-        for (int ww = 0; ww < kWaves; ++ww) v += s_red[ww][tid];
-        a.partials[(size_t)tile * 10 + tid] = v;
+        for (int ww = 0; ww < kWaves; ++ww) v = tid < 12 ? v + s_red[ww][tid] : fmax(v, s_red[ww][tid]);
+        a.partials[(size_t)tile * kMomStride + tid] = v;
     }
 }
 
@@ -211,40 +228,132 @@ __device__ static void slab_axis(const double m[9], uint32_t n, const float uf[3
 // of its own rather than a last-tile ticket: the kernel boundary orders the
 // partials, where a ticket needs a device-scope fence per tile (an L2
 // write-back on a multi-XCD part).
+// Sweep-test model of an axis (pair), from the covariance C, the mean extent
+// per coordinate xm and the largest extent per coordinate xx.  A sweep along
+// u tests each object against the objects whose lower ends fall inside its
+// projected interval: about n^2 E[ext_u] f_u with f_u = 1 / (2 sqrt(pi var_u))
+// for a Gaussian (0.98x that for a uniform spread of the same variance).  In
+// slabs of width W_v (the largest extent on v) on a second axis v it is about
+// n^2 E[ext_u] W_v / (4 pi sqrt(det C_uv)).  The extent of an AABB on a unit
+// axis is sum |u_k| ext_k, so tilted axes pay for their longer projections.
+// Costs below drop the constant factors; +inf when the spread vanishes.
+__device__ static double proj_mean_extent(const double u[3], const double xm[3]) {
+    return fabs(u[0]) * xm[0] + fabs(u[1]) * xm[1] + fabs(u[2]) * xm[2];
+}
+__device__ static double quad(const double C[3][3], const double u[3], const double v[3]) {
+    double r = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r += u[i] * C[i][j] * v[j];
+    return r;
+}
+__device__ static double axis_cost(const double C[3][3], const double xm[3], const double u[3]) {
+    const double var = quad(C, u, u);
+    return var > 0.0 ? proj_mean_extent(u, xm) / sqrt(var) : __builtin_inf();
+}
+__device__ static double pair_cost(const double C[3][3], const double xm[3], const double xx[3],
+                                   const double u[3], const double v[3]) {
+    const double det = quad(C, u, u) * quad(C, v, v) - quad(C, u, v) * quad(C, u, v);
+    const double wv = fabs(v[0]) * xx[0] + fabs(v[1]) * xx[1] + fabs(v[2]) * xx[2];
+    return det > 0.0 ? proj_mean_extent(u, xm) * wv / sqrt(det) : __builtin_inf();
+}
+
+// GSAP_AXIS_AUTO: the PCA axis (pair) unless a coordinate axis (pair) has a
+// strictly smaller model cost; candidates in a fixed order, so ties keep the
+// earlier one.  src = 0 (PCA) or 1 (coordinate).
+__device__ static void choose_auto(const double m[15], uint32_t n, bool slabs, float u[3], float v[3],
+                                   float* src) {
+    choose_axis(m, n, 0, nullptr, nullptr, u, src);
+    if (slabs) slab_axis(m, n, u, v);
+    if (n < 2) return;
+    const double inv = 1.0 / (double)n;
+    const double mu[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+    double C[3][3];
+    C[0][0] = m[3] * inv - mu[0] * mu[0];
+    C[0][1] = C[1][0] = m[4] * inv - mu[0] * mu[1];
+    C[0][2] = C[2][0] = m[5] * inv - mu[0] * mu[2];
+    C[1][1] = m[6] * inv - mu[1] * mu[1];
+    C[1][2] = C[2][1] = m[7] * inv - mu[1] * mu[2];
+    C[2][2] = m[8] * inv - mu[2] * mu[2];
+    const double xm[3] = {m[9] * inv, m[10] * inv, m[11] * inv};
+    const double xx[3] = {m[12], m[13], m[14]};
+    const double pu[3] = {u[0], u[1], u[2]};
+    int bu = -1, bv = -1;
+    if (!slabs) {
+        double best = axis_cost(C, xm, pu);
+        for (int a = 0; a < 3; ++a) {
+            const double e[3] = {a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0};
+            const double c = axis_cost(C, xm, e);
+            if (c < best) { best = c; bu = a; }
+        }
+    } else {
+        const double pv[3] = {v[0], v[1], v[2]};
+        double best = pair_cost(C, xm, xx, pu, pv);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                if (a == b) continue;
+                const double e[3] = {a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0};
+                const double f[3] = {b == 0 ? 1.0 : 0.0, b == 1 ? 1.0 : 0.0, b == 2 ? 1.0 : 0.0};
+                const double c = pair_cost(C, xm, xx, e, f);
+                if (c < best) { best = c; bu = a; bv = b; }
+            }
+    }
+    if (bu < 0) return;
+    u[0] = u[1] = u[2] = 0.0f;
+    u[bu] = 1.0f;
+    if (slabs) {
+        v[0] = v[1] = v[2] = 0.0f;
+        v[bv] = 1.0f;
+    }
+    *src = 1.0f;
+}
+
 __global__ __launch_bounds__(kWG) void k_axis(ObjPipeArgs a) {
-    __shared__ double s_red[kWaves][9];
+    __shared__ double s_red[kWaves][15];
     const uint32_t seg = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = lane_id();
     const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[seg] : 0u;
     const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[seg + 1] : a.ntiles;
     const uint32_t nseg_obj = a.seg.tile_seg ? a.seg.seg_n[seg] : a.n;
-    double m[9];
+    // m[0..11] sums, m[12..14] maxima (exact in any order)
+    double m[15];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) m[k] = 0.0;
+    for (int k = 0; k < 15; ++k) m[k] = 0.0;
     for (uint32_t t = t0 + tid; t < t1; t += kWG)
 #pragma unroll
-        for (int k = 0; k < 9; ++k) m[k] += a.partials[(size_t)t * 10 + k];
+        for (int k = 0; k < 15; ++k) {
+            const double p = a.partials[(size_t)t * kMomStride + k];
+            m[k] = k < 12 ? m[k] + p : fmax(m[k], p);
+        }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        double v = wave_sum(m[k]);
+    for (int k = 0; k < 15; ++k) {
+        double v = m[k];
+        if (k < 12) {
+            v = wave_sum(v);
+        } else {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        }
         if (l == 0) s_red[w][k] = v;
     }
     __syncthreads();
     if (tid == 0) {
-        double mm[9];
-        for (int k = 0; k < 9; ++k) {
+        double mm[15];
+        for (int k = 0; k < 15; ++k) {
             mm[k] = 0.0;
-            for (int ww = 0; ww < kWaves; ++ww) mm[k] += s_red[ww][k];
+            for (int ww = 0; ww < kWaves; ++ww) mm[k] = k < 12 ? mm[k] + s_red[ww][k] : fmax(mm[k], s_red[ww][k]);
         }
-        float u[3], src;
+        float u[3], v[3], src;
         const float* learned = a.learned_axis ? a.learned_axis + 4 * (size_t)seg : nullptr;
-        choose_axis(mm, nseg_obj, a.axis_mode, a.fixed_axis, learned, u, &src);
+        if (a.axis_mode == 4) {
+            choose_auto(mm, nseg_obj, a.slabs != 0, u, v, &src);
+        } else {
+            choose_axis(mm, nseg_obj, a.axis_mode, a.fixed_axis, learned, u, &src);
+            if (a.slabs) slab_axis(mm, nseg_obj, u, v);
+        }
         float* ax = a.axis + 4 * (size_t)seg;
         ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = src;
         if (a.seg_mom)
             for (int k = 0; k < 9; ++k) a.seg_mom[9 * (size_t)seg + k] = mm[k];
         if (a.slabs) {
-            float v[3];
-            slab_axis(mm, nseg_obj, u, v);
             a.axis2[0] = v[0]; a.axis2[1] = v[1]; a.axis2[2] = v[2]; a.axis2[3] = 0.0f;
         }
     }

@@ -56,7 +56,14 @@ enum { GSAP_INPUT_AABB = 0,   /* 6 floats per object: lo.x lo.y lo.z hi.x hi.y h
 enum { GSAP_AXIS_PCA = 0,          /* major eigenvector of the fp64 covariance     */
        GSAP_AXIS_COORD_MAXVAR = 1, /* reference rule: coordinate axis of max var.  */
        GSAP_AXIS_FIXED = 2,        /* desc.fixed_axis                              */
-       GSAP_AXIS_LEARNED = 3       /* learned selector, PCA fallback               */ };
+       GSAP_AXIS_LEARNED = 3,      /* learned selector, PCA fallback               */
+       /* Cheapest of the PCA axis and the three coordinate axes (with
+        * GSAP_FLAG_SLABS: of the PCA pair and the six coordinate pairs) by
+        * the sweep-test model E[extent on u] / sqrt(var u), or E[extent on u] x
+        * (largest extent on v) / sqrt(det cov(u, v)) for slabs, from the same
+        * fp64 moments plus per-coordinate mean and largest extents.  No
+        * reference counterpart; axis_source reports PCA or COORD_MAXVAR. */
+       GSAP_AXIS_AUTO = 4 };
 
 enum { GSAP_OUT_CANON = 0,  /* uint32x2 (i<j), lexicographic, each pair once    */
        GSAP_OUT_SYM_CSR = 1 /* reference format: int32x2 both orientations,

@@ -166,6 +166,131 @@ int or_axis_coord_maxvar(const double mom[9], size_t n, float axis[3], int lamda
     return a;
 }
 
+/* Per-coordinate extent sums and maxima (fp64): hi_k - lo_k of each AABB,  */
+/* 2r of each sphere's hull box.                                            */
+void or_extents(const float* boxes, int kind, size_t n, double sum[3], double mx[3]) {
+    for (int k = 0; k < 3; ++k) { sum[k] = 0.0; mx[k] = 0.0; }
+    for (size_t i = 0; i < n; ++i) {
+        double x[3];
+        if (kind == OR_KIND_AABB) {
+            const float* b = boxes + 6 * i;
+            for (int k = 0; k < 3; ++k) x[k] = (double)b[k + 3] - (double)b[k];
+        } else {
+            const float* b = boxes + 4 * i;
+            for (int k = 0; k < 3; ++k) x[k] = 2.0 * (double)b[3];
+        }
+        for (int k = 0; k < 3; ++k) {
+            sum[k] += x[k];
+            if (x[k] > mx[k]) mx[k] = x[k];
+        }
+    }
+}
+
+/* Slab axis of the slab sweep: the top eigenvector of P C P, P = I - u u^T  */
+/* (the direction of largest spread orthogonal to u), any unit vector        */
+/* orthogonal to u when there is no spread across u.  Generalises the        */
+/* reference's choice of the two minor coordinate axes for its 4-cell split  */
+/* (PB/sapDetection/SapKernel.cu:62-97, lamda[1], lamda[2]).                 */
+void or_slab_axis(const double mom[9], size_t n, const float uf[3], float v[3]) {
+    double C[3][3], P[3][3], PC[3][3], A[3][3], V[3][3], ev[3];
+    const double u[3] = {uf[0], uf[1], uf[2]};
+    cov_from_moments(mom, n, C);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = (i == j ? 1.0 : 0.0) - u[i] * u[j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) PC[i][j] = P[i][0] * C[0][j] + P[i][1] * C[1][j] + P[i][2] * C[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = PC[i][0] * P[0][j] + PC[i][1] * P[1][j] + PC[i][2] * P[2][j];
+    jacobi3(A, V, ev);
+    int best = 0;
+    for (int k = 1; k < 3; ++k)
+        if (ev[k] > ev[best]) best = k;
+    double w[3] = {V[0][best], V[1][best], V[2][best]};
+    const double d = w[0] * u[0] + w[1] * u[1] + w[2] * u[2];
+    for (int k = 0; k < 3; ++k) w[k] -= d * u[k];
+    double nn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (!(nn > 1e-6)) {
+        int k = 0;
+        for (int j = 1; j < 3; ++j)
+            if (fabs(u[j]) < fabs(u[k])) k = j;
+        double e[3] = {0.0, 0.0, 0.0};
+        e[k] = 1.0;
+        w[0] = u[1] * e[2] - u[2] * e[1];
+        w[1] = u[2] * e[0] - u[0] * e[2];
+        w[2] = u[0] * e[1] - u[1] * e[0];
+        nn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (!(nn > 0)) { w[0] = 0; w[1] = 1; w[2] = 0; nn = 1; }
+    }
+    for (int k = 0; k < 3; ++k) v[k] = (float)(w[k] / nn);
+}
+
+/* Sweep-test model (no reference counterpart; the reference always sweeps  */
+/* the coordinate axis of largest variance, SapKernel.cu:62-97).  A sweep   */
+/* along unit u tests about n^2 E[ext_u] / (2 sqrt(pi var_u)) pairs for a    */
+/* Gaussian spread; in slabs of width W_v on a second axis v about           */
+/* n^2 E[ext_u] W_v / (4 pi sqrt(det C_uv)).  ext_u = sum |u_k| ext_k for an */
+/* AABB, W_v <= sum |v_k| max ext_k.  Constants dropped; +inf without spread.*/
+static double quad3(const double C[3][3], const double u[3], const double v[3]) {
+    double r = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r += u[i] * C[i][j] * v[j];
+    return r;
+}
+double or_axis_cost(const double mom[9], const double xsum[3], const double xmax[3], size_t n,
+                    const float uf[3], const float vf[3]) {
+    double C[3][3];
+    cov_from_moments(mom, n, C);
+    const double inv = n ? 1.0 / (double)n : 0.0;
+    const double u[3] = {uf[0], uf[1], uf[2]};
+    const double em = fabs(u[0]) * (xsum[0] * inv) + fabs(u[1]) * (xsum[1] * inv) +
+                      fabs(u[2]) * (xsum[2] * inv);
+    if (!vf) {
+        const double var = quad3(C, u, u);
+        return var > 0.0 ? em / sqrt(var) : INFINITY;
+    }
+    const double v[3] = {vf[0], vf[1], vf[2]};
+    const double det = quad3(C, u, u) * quad3(C, v, v) - quad3(C, u, v) * quad3(C, u, v);
+    const double wv = fabs(v[0]) * xmax[0] + fabs(v[1]) * xmax[1] + fabs(v[2]) * xmax[2];
+    return det > 0.0 ? em * wv / sqrt(det) : INFINITY;
+}
+
+/* Automatic axis: the PCA axis (with slabs: the PCA axis and its slab axis) */
+/* unless a coordinate axis (pair) has a strictly smaller model cost,        */
+/* candidates in the order x, y, z (pairs (a,b), a != b, a-major).  Returns  */
+/* 0 when PCA is kept, 1 for a coordinate choice.                            */
+int or_axis_auto(const double mom[9], const double xsum[3], const double xmax[3], size_t n,
+                 int slabs, float u[3], float v[3]) {
+    or_pca_axis(mom, n, u, NULL);
+    if (slabs) or_slab_axis(mom, n, u, v);
+    if (n < 2) return 0;
+    double best = or_axis_cost(mom, xsum, xmax, n, u, slabs ? v : NULL);
+    int bu = -1, bv = -1;
+    for (int a = 0; a < 3; ++a) {
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        e[a] = 1.0f;
+        if (!slabs) {
+            const double c = or_axis_cost(mom, xsum, xmax, n, e, NULL);
+            if (c < best) { best = c; bu = a; }
+            continue;
+        }
+        for (int b = 0; b < 3; ++b) {
+            if (a == b) continue;
+            float f[3] = {0.0f, 0.0f, 0.0f};
+            f[b] = 1.0f;
+            const double c = or_axis_cost(mom, xsum, xmax, n, e, f);
+            if (c < best) { best = c; bu = a; bv = b; }
+        }
+    }
+    if (bu < 0) return 0;
+    u[0] = u[1] = u[2] = 0.0f;
+    u[bu] = 1.0f;
+    if (slabs) {
+        v[0] = v[1] = v[2] = 0.0f;
+        v[bv] = 1.0f;
+    }
+    return 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Projection to 32-bit order-preserving keys.  The reference projects       */
 /* sphere centres, key = dot(axis,c) - r (+ a cell offset),                  */
@@ -341,8 +466,15 @@ static void choose_axis(const float* boxes, int kind, size_t n, int axis_mode,
     }
     double mom[9];
     or_moments(boxes, kind, n, mom);
-    if (axis_mode == OR_AXIS_COORD_MAXVAR) or_axis_coord_maxvar(mom, n, u, NULL);
-    else or_pca_axis(mom, n, u, NULL);
+    if (axis_mode == OR_AXIS_COORD_MAXVAR) {
+        or_axis_coord_maxvar(mom, n, u, NULL);
+    } else if (axis_mode == OR_AXIS_AUTO) {
+        double xs[3], xm[3];
+        or_extents(boxes, kind, n, xs, xm);
+        or_axis_auto(mom, xs, xm, n, 0, u, NULL);
+    } else {
+        or_pca_axis(mom, n, u, NULL);
+    }
 }
 
 /* ------------------------------------------------------------------------ */

@@ -35,7 +35,7 @@ extern "C" {
 #endif
 
 enum { OR_KIND_AABB = 0, OR_KIND_SPHERE = 1 };
-enum { OR_AXIS_PCA = 0, OR_AXIS_COORD_MAXVAR = 1, OR_AXIS_FIXED = 2 };
+enum { OR_AXIS_PCA = 0, OR_AXIS_COORD_MAXVAR = 1, OR_AXIS_FIXED = 2, OR_AXIS_AUTO = 4 };
 
 uint32_t or_flip(float f);
 float or_unflip(uint32_t u);
@@ -48,6 +48,18 @@ void or_moments(const float* boxes, int kind, size_t n, double out[9]);
 void or_pca_axis(const double mom[9], size_t n, float axis[3], double evals[3]);
 /* Reference rule: coordinate axis of largest diagonal variance. */
 int or_axis_coord_maxvar(const double mom[9], size_t n, float axis[3], int lamda[3]);
+
+/* Per-coordinate sums and maxima of the box extents (hi - lo; 2r for spheres). */
+void or_extents(const float* boxes, int kind, size_t n, double sum[3], double mx[3]);
+/* Top eigenvector of P C P, P = I - u u^T: the slab axis of the slab sweep. */
+void or_slab_axis(const double mom[9], size_t n, const float u[3], float v[3]);
+/* Sweep-test model cost of axis u (v == NULL) or of the slab pair (u, v). */
+double or_axis_cost(const double mom[9], const double xsum[3], const double xmax[3], size_t n,
+                    const float u[3], const float v[3]);
+/* PCA axis (pair) unless a coordinate axis (pair) is strictly cheaper by the
+ * model; returns 0 (PCA kept) or 1 (coordinate).  v is written when slabs. */
+int or_axis_auto(const double mom[9], const double xsum[3], const double xmax[3], size_t n,
+                 int slabs, float u[3], float v[3]);
 
 /* Conservative projection of every box onto axis: keys = flip(lower end),
  * hikeys = flip(upper end). */

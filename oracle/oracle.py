@@ -15,7 +15,7 @@ _LIB = os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 KIND_AABB, KIND_SPHERE = 0, 1
-AXIS_PCA, AXIS_COORD_MAXVAR, AXIS_FIXED = 0, 1, 2
+AXIS_PCA, AXIS_COORD_MAXVAR, AXIS_FIXED, AXIS_AUTO = 0, 1, 2, 4
 
 PRB_ABS_SPRING, PRB_WALLS, PRB_FLOOR_CLAMP, PRB_CAP_VEL, PRB_SKIP_SAME_BODY, PRB_INERTIA = (
     1, 2, 4, 8, 16, 32)
@@ -42,6 +42,12 @@ def lib():
         L.or_pca_axis.argtypes = [P, sz, P, P]
         L.or_axis_coord_maxvar.restype = ctypes.c_int
         L.or_axis_coord_maxvar.argtypes = [P, sz, P, P]
+        L.or_extents.argtypes = [P, ctypes.c_int, sz, P, P]
+        L.or_slab_axis.argtypes = [P, sz, P, P]
+        L.or_axis_cost.restype = ctypes.c_double
+        L.or_axis_cost.argtypes = [P, P, P, sz, P, P]
+        L.or_axis_auto.restype = ctypes.c_int
+        L.or_axis_auto.argtypes = [P, P, P, sz, ctypes.c_int, P, P]
         L.or_project.argtypes = [P, ctypes.c_int, sz, P, P, P]
         L.or_radix_sort_kv.argtypes = [P, P, sz, ctypes.c_int]
         L.or_radix_sort_pairs.argtypes = [P, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -109,6 +115,38 @@ def coord_axis(mom, n):
     lam = np.zeros(3, np.int32)
     a = lib().or_axis_coord_maxvar(_p(mom), n, _p(ax), _p(lam))
     return a, ax, lam
+
+
+def extents(boxes, kind=KIND_AABB):
+    """(sum, max) of the per-coordinate box extents, fp64."""
+    b = _boxes(boxes, kind)
+    xs, xm = np.zeros(3, np.float64), np.zeros(3, np.float64)
+    lib().or_extents(_p(b), kind, len(b), _p(xs), _p(xm))
+    return xs, xm
+
+
+def slab_axis(mom, n, u):
+    mom = np.ascontiguousarray(mom, np.float64)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.zeros(3, np.float32)
+    lib().or_slab_axis(_p(mom), n, _p(u), _p(v))
+    return v
+
+
+def axis_cost(mom, xs, xm, n, u, v=None):
+    a = [np.ascontiguousarray(x, np.float64) for x in (mom, xs, xm)]
+    u = np.ascontiguousarray(u, np.float32)
+    v = None if v is None else np.ascontiguousarray(v, np.float32)
+    return lib().or_axis_cost(_p(a[0]), _p(a[1]), _p(a[2]), n, _p(u), _p(v))
+
+
+def axis_auto(boxes, kind=KIND_AABB, slabs=False):
+    """(source, u, v): source 0 = PCA kept, 1 = coordinate axis (pair)."""
+    mom = moments(boxes, kind)
+    xs, xm = extents(boxes, kind)
+    u, v = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    src = lib().or_axis_auto(_p(mom), _p(xs), _p(xm), len(boxes), int(slabs), _p(u), _p(v))
+    return src, u, (v if slabs else None)
 
 
 def project(boxes, axis, kind=KIND_AABB):
