@@ -317,6 +317,7 @@ def main():
     pair_gbs = alg_pass / (float(pass_ms.mean()) / 1e3) / 1e9
     pair_roof = {"achieved": round(pair_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(pair_gbs / HBM_PEAK_GBS, 4), "ms_per_pass": round(float(pass_ms.mean()), 4),
+                 "ms_passes": [round(float(x), 4) for x in pass_ms],
                  "passes": len(pass_ms), "bytes_per_pass": alg_pass}
     if tn["pair_local"]:
         loc_gbs = alg_pass / (stages["pair_local"] / 1e3) / 1e9

@@ -237,7 +237,9 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         HIPCHK(h, grow(h->feats, (size_t)nseg * kFeat * 4));
         h->cap_seg = nseg;
     }
-    if (!h->hist_pair.p) HIPCHK(h, grow(h->hist_pair, kMaxPasses * 256 * 4));
+    // pair digit histograms, then the first pass's per-region counts and slot counters
+    if (!h->hist_pair.p)
+        HIPCHK(h, grow(h->hist_pair, (kMaxPasses * 256 + 2 * kMaxRegions * 256) * 4));
     if (slab_mode(h) && !h->slabp.p) {
         HIPCHK(h, grow(h->axis2, 4 * sizeof(float)));
         HIPCHK(h, grow(h->slabp, kSlabParams * sizeof(float)));
@@ -321,6 +323,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.bits_obj = h->obj_bits;
     a.hist_pair = (uint32_t*)h->hist_pair.p;
     a.npass_pair = count_only ? 0 : npass_pair;
+    a.pair_claim = a.hist_pair + kMaxPasses * 256;
     a.keys = (uint32_t*)h->keys[0].p;
     a.vals = (uint32_t*)h->vals[0].p;
     a.keys2 = (uint32_t*)h->keys[1].p;
@@ -410,6 +413,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     // sweep counts the pair sort's digits as it writes the pairs
     const bool sweep_hist = !count_only && local && npass_pair <= kSweepHistPasses;
     w.hist = sweep_hist ? a.hist_pair : nullptr;
+    // with per-XCD pair regions the first pass claims its slots per region
+    // (no look-back); the sweep then also counts that digit per region
+    const bool msplit = sweep_hist && npass_pair > 1 && h->pair_regions == kMaxRegions;
+    w.hist_reg = msplit ? a.pair_claim : nullptr;
     w.hist_npass = (int)npass_pair;
     w.hist_shift0 = (int)shift0;
     w.bbits = (int)bbits;
@@ -433,6 +440,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.nregions = nreg;
         p.shift0 = (int)shift0;
         p.hist = a.hist_pair;
+        p.hist_reg = msplit ? a.pair_claim : nullptr;
+        p.claim = msplit ? a.pair_claim + kMaxRegions * 256 : nullptr;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;
@@ -453,8 +462,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
             uint2* x = (npass_pair & 1u) ? t1p : t0p;
             uint2* y = (npass_pair & 1u) ? t0p : t1p;
             HIPCHK(h, launch_pair_local(x, y, outp, &ctrl->sort_real, &ctrl->num_pairs, cap,
-                                        h->slot_cap, (uint32_t*)h->wstart.p, (int)bbits,
-                                        (int)shift0, st));
+                                        h->slot_cap, (uint32_t*)h->wstart.p, &ctrl->local_big,
+                                        (int)bbits, (int)shift0, st));
             mark(h, st);
         }
         if (sym)

@@ -60,7 +60,8 @@ struct Ctrl {
     uint32_t flags;                  // 1 = capacity overflow, 2 = non-finite input
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
-    uint32_t pad[4];
+    uint32_t local_big;              // windows the pair sort's LDS stage handed on
+    uint32_t pad[3];
 };
 
 enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u };
@@ -98,6 +99,10 @@ struct ObjPipeArgs {
     float* vr_partials;     // [ntiles][kVr] per-tile ranges for the slab parameters
     float* slab;            // [kSlabParams] slab and key-quantisation parameters
     uint32_t* slab_cnt;     // [256] entries per slab (zeroed by k_moments, counted by k_project)
+    // [2][kMaxRegions][256] or null: the pair sort's first-pass digit counts
+    // per pair region (counted by the sweep) and that pass's slot counters;
+    // zeroed by k_moments
+    uint32_t* pair_claim;
 };
 constexpr int kVr = 5;           // max v extent, min lo_v, max hi_v, min lo_u, max hi_u
 constexpr int kSlabParams = 8;   // v0, 1/W, S, u0, 1/du, W, -, -
@@ -136,6 +141,7 @@ struct SweepArgs {
     int hist_npass;
     int hist_shift0;
     int bbits;
+    uint32_t* hist_reg;      // if set: the first digit's counts per region, [kMaxRegions][256]
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
@@ -215,6 +221,11 @@ struct SortArgs {
     uint32_t nregions;
     const unsigned long long* n_dev_rest;  // element count of the passes after the first (or null)
     int shift0;                            // bit offset of the first digit
+    // pair sort, first pass without look-back (an LSD sort's first pass need
+    // not be stable): per-region digit counts [kMaxRegions][256] and slot
+    // counters of the same shape (zero), or null for a onesweep first pass
+    const uint32_t* hist_reg;
+    uint32_t* claim;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
@@ -233,10 +244,11 @@ hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipSt
 constexpr uint32_t kLocW = 2048;   // window: pairs whose bucket starts are one workgroup's
 constexpr uint32_t kLocC = 8192;   // pairs sorted in LDS by one workgroup
 uint64_t pair_local_wstart_words(uint64_t cap);  // u32 words of the window-start table
+// big_ctr: a device word the launch zeroes and counts the handed-on windows in.
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
-                             unsigned long long cap, uint32_t* wstart, int bbits, int L,
-                             hipStream_t st);
+                             unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
+                             int L, hipStream_t st);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
 constexpr uint32_t kSweepHistPasses = 2;   // pair-sort digit histograms the sweep can count

@@ -17,9 +17,11 @@
 //    highest bits in which the window's keys differ puts every key in its
 //    bin; a key's place in the bin is the number of the bin's keys smaller
 //    than it.  Other windows (more than kLocC pairs, buckets too far apart
-//    for a 32-bit key, or a bin of more than kBinMax keys): stable LSD
-//    passes of wave64 ballot ranking through global memory, one chunk at a
-//    time, ping-ponging over the same positions with the free sort buffer.
+//    for a 32-bit key, or a bin of more than kBinMax keys) go to a list.
+//  * k_pair_local_big: the listed windows, by stable LSD passes of wave64
+//    ballot ranking through global memory, one chunk at a time,
+//    ping-ponging over the same positions with the free sort buffer.  A
+//    kernel of its own, so the common path's registers stay few.
 #include "gsap_device.h"
 
 namespace gsap {
@@ -32,6 +34,7 @@ constexpr int kLocItems = kLocC / kLT;  // keys per thread of an LDS sort
 constexpr uint32_t kBinBits = 12;       // counting pass of an LDS sort: 4096 bins
 constexpr uint32_t kBins = 1u << kBinBits;
 constexpr uint32_t kBinMax = 32;        // largest bin ranked by comparisons
+constexpr uint32_t kLocTail = 64;      // LDS words of the key array used for reductions
 constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
 constexpr uint32_t kGlobChunk = kLT * kGlobItems;
 
@@ -98,7 +101,9 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, int bw, ui
 __global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P,
                                                      const unsigned long long* __restrict__ n_dev,
                                                      int bbits, int L,
-                                                     uint32_t* __restrict__ wstart) {
+                                                     uint32_t* __restrict__ wstart,
+                                                     uint32_t* __restrict__ big_ctr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *big_ctr = 0u;  // read by the kernels after this one
     const uint32_t n = (uint32_t)*n_dev;
     const uint32_t J = (n + kLocW - 1) / kLocW;
     const uint32_t lane = lane_id();
@@ -155,23 +160,49 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P
     }
 }
 
-__global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2* __restrict__ out,
-                                                       const unsigned long long* __restrict__ n_dev,
-                                                       const unsigned long long* __restrict__ last_n,
-                                                       unsigned long long last_cap,
-                                                       const uint32_t* __restrict__ wstart,
-                                                       int bbits, int L) {
+// Pair key bits of the 32-bit local view.  khigh = key >> L and klow = key &
+// (2^L - 1) for key = (a << bbits) | b, in 32-bit arithmetic; valid when the
+// key has at most 32 + L bits (2 bbits - L <= 32, i.e. up to 2^28 objects).
+__device__ __forceinline__ uint32_t khigh(uint2 p, int bbits, int L) {
+    return L >= bbits ? (p.x >> (L - bbits)) : ((p.x << (bbits - L)) | (p.y >> L));
+}
+__device__ __forceinline__ uint32_t klow(uint2 p, int bbits, int L) {
+    return L >= bbits ? (((p.x & ((1u << (L - bbits)) - 1u)) << bbits) | p.y)
+                      : (p.y & ((1u << L) - 1u));
+}
+// The pair of key (h << L) | lo.
+__device__ __forceinline__ uint2 kpair(uint32_t h, uint32_t lo, int bbits, int L) {
+    if (L >= bbits)
+        return make_uint2((h << (L - bbits)) | (lo >> bbits), lo & ((1u << bbits) - 1u));
+    return make_uint2(h >> (bbits - L), ((h & ((1u << (bbits - L)) - 1u)) << L) | lo);
+}
+
+// LDS path: one workgroup per window j (grid-stride), sorting the buckets that
+// start in it, positions [wstart[j], wstart[j+1]).  Up to kLocC - kLocTail
+// pairs whose high bits lie within 2^(32-L) of the first one's, as unique
+// 32-bit keys (high - first high) << L | low bits.  One counting pass on the
+// kBinBits highest bits in which the window's keys differ puts every key in
+// its bin; a key's place in its bin is the number of the bin's smaller keys.
+// Windows that do not fit (too many pairs, buckets too far apart, a bin of
+// more than kBinMax keys) are appended to `big` for k_pair_local_big.
+// Software pipeline over the workgroup's windows j, j + G, ...: window starts
+// are read two windows ahead, and the next window's pairs are loaded into
+// registers while this window is sorted.
+__global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pair_local(
+    const uint2* __restrict__ X, uint2* __restrict__ out, const unsigned long long* __restrict__ n_dev,
+    const unsigned long long* __restrict__ last_n, unsigned long long last_cap,
+    const uint32_t* __restrict__ wstart, uint32_t* __restrict__ big, uint32_t* __restrict__ big_ctr,
+    int bbits, int L) {
+    // 48 KB, so that three workgroups fit a CU: the reduction words live in
+    // the last kLocTail words of the key array, which the keys never reach
+    // and which are dead while the keys are scattered and ranked.
     __shared__ uint32_t s_e[kLocC];
-    // bins of the counting pass, or the per-wave digit counters of the LSD passes
-    __shared__ uint32_t s_work[kBins > kLWaves * 256 ? kBins : kLWaves * 256];
-    uint32_t(*const s_wh)[256] = reinterpret_cast<uint32_t(*)[256]>(s_work);
-    static_assert(kBins >= kLWaves * 256, "the digit counters share the bins' memory");
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_tmp[kLWaves];
-    __shared__ uint32_t s_red[3 * kLWaves];
+    __shared__ uint32_t s_bin[kBins];
+    uint32_t* const s_tmp = s_e + (kLocC - kLocTail);
+    uint32_t* const s_red = s_tmp + kLWaves;
+    constexpr uint32_t kCap = kLocC - kLocTail;
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
     const uint32_t n = (uint32_t)*n_dev;
     unsigned long long out_lim = n;
     {
@@ -180,14 +211,11 @@ __global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2
         if (last_cap < out_lim) out_lim = last_cap;
     }
     const uint32_t J = (n + kLocW - 1) / kLocW;
+    const bool fits32 = 2 * bbits - L <= 32;
     const uint32_t lowmask = (1u << L) - 1u;  // 8 <= L <= 24
-
-    // Software pipeline over this workgroup's windows j, j + G, ...: window
-    // starts are read two windows ahead, and the next window's pairs are
-    // loaded into registers while this window is sorted.
     const uint32_t G = gridDim.x;
-    uint32_t j = blockIdx.x;
-    uint32_t s = 0, e = 0, s1 = 0, e1 = 0;
+
+    uint32_t j = blockIdx.x, s = 0, e = 0, s1 = 0, e1 = 0;
     if (j < J) {
         s = wstart[j];
         e = wstart[j + 1];
@@ -196,15 +224,18 @@ __global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2
         s1 = wstart[j + G];
         e1 = wstart[j + G + 1];
     }
-    uint2 raw[kLocItems], rfirst = make_uint2(0u, 0u), rlast = make_uint2(0u, 0u);
-    auto fetch = [&](uint32_t fs, uint32_t fe) {  // an LDS-sized window's pairs
+    // The prefetch holds the first kPre rows of a window (4,096 pairs, more
+    // than most windows have); larger windows load the rest when sorted.
+    constexpr int kPre = kLocItems / 2;
+    uint2 raw[kPre], rfirst = make_uint2(0u, 0u), rlast = make_uint2(0u, 0u);
+    auto fetch = [&](uint32_t fs, uint32_t fe) {  // the pairs of an LDS-sized window
         const uint32_t fS = fe - fs;
-        if (fS == 0 || fS > kLocC) return;
+        if (fS == 0 || fS > kCap || !fits32) return;
         const uint32_t fnk = (fS + kLT - 1) / kLT;
         rfirst = X[fs];
         rlast = X[fe - 1];
 #pragma unroll
-        for (int k = 0; k < kLocItems; ++k) {
+        for (int k = 0; k < kPre; ++k) {
             const uint32_t q = tid + (uint32_t)kLT * k;
             raw[k] = make_uint2(0u, 0u);
             if ((uint32_t)k < fnk && q < fS) raw[k] = X[fs + q];
@@ -220,25 +251,25 @@ __global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2
         bool fetched = false;
         if (s < e) {  // some bucket starts in this window
             const uint32_t S = e - s;
-            // 32-bit keys relative to the window's first bucket: (high - hs) << L | low
-            const unsigned long long hs = pkey(rfirst, bbits) >> L;
-            const unsigned long long he = pkey(rlast, bbits) >> L;
-            bool in_lds = S <= kLocC && (he - hs) < (1ull << (32 - L));
-            if (in_lds) {
+            const uint32_t hs = khigh(rfirst, bbits, L);
+            bool ok = fits32 && S <= kCap && khigh(rlast, bbits, L) - hs < (1u << (32 - L));
+            if (ok) {
                 const uint32_t nk = (S + kLT - 1) / kLT;
                 uint32_t kk[kLocItems], vor = 0u, vand = ~0u;
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < kLocItems; ++k) {
-                    const uint32_t q = tid + (uint32_t)kLT * k;
-                    kk[k] = 0u;
-                    if ((uint32_t)k < nk && q < S) {
-                        const unsigned long long v = pkey(raw[k], bbits);
-                        kk[k] = ((uint32_t)((v >> L) - hs) << L) | ((uint32_t)v & lowmask);
+                    uint2 p = make_uint2(0u, 0u);
+                    if (k < kPre) {
+                        p = raw[k < kPre ? k : 0];
+                    } else if ((uint32_t)k < nk) {
+                        const uint32_t q = tid + (uint32_t)kLT * k;
+                        if (q < S) p = X[s + q];
                     }
+                    kk[k] = ((khigh(p, bbits, L) - hs) << L) | klow(p, bbits, L);
                 }
                 fetch(s1, e1);
                 fetched = true;
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < kLocItems; ++k) {
                     const uint32_t q = tid + (uint32_t)kLT * k;
                     if ((uint32_t)k < nk && q < S) {
@@ -252,153 +283,77 @@ __global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2
                     s_red[wave] = vor;
                     s_red[kLWaves + wave] = vand;
                 }
-                for (uint32_t i = tid; i < kBins; i += kLT) s_work[i] = 0u;
+                for (uint32_t i = tid; i < kBins; i += kLT) s_bin[i] = 0u;
                 __syncthreads();
-                auto emit = [&](uint32_t pos, uint32_t v) {
-                    if (s + pos < out_lim)
-                        out[s + pos] = pkey_pair(((hs + (v >> L)) << L) | (v & lowmask), bbits);
-                };
-                // Counting pass on the 12 highest bits in which the window's keys
-                // differ (the bits above are common to all of them), then each
-                // key's place in its bin = the number of the bin's smaller keys.
                 vor = 0u;
                 vand = ~0u;
-    #pragma unroll
+#pragma unroll
                 for (int w = 0; w < kLWaves; ++w) {
                     vor |= s_red[w];
                     vand &= s_red[kLWaves + w];
                 }
+                // bins: the kBinBits highest bits in which the keys differ (the
+                // bits above are common to all of them)
                 const uint32_t diff = vor ^ vand;
                 const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
                 const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < kLocItems; ++k) {
                     const uint32_t q = tid + (uint32_t)kLT * k;
-                    if ((uint32_t)k < nk && q < S) atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u);
+                    if ((uint32_t)k < nk && q < S) atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u);
                 }
                 __syncthreads();
-                // bin starts (exclusive scan, kBins / kLT bins per thread) and the largest bin
+                // bin starts (exclusive scan, BPT bins per thread) and the largest bin
                 constexpr uint32_t BPT = kBins / kLT;
                 uint32_t cb[BPT], sum = 0, mx = 0, all;
-    #pragma unroll
+#pragma unroll
                 for (uint32_t i = 0; i < BPT; ++i) {
-                    cb[i] = s_work[tid * BPT + i];
+                    cb[i] = s_bin[tid * BPT + i];
                     sum += cb[i];
                     mx = cb[i] > mx ? cb[i] : mx;
                 }
                 mx = wave_max_u32(mx);
                 if (lane == 0) s_red[2 * kLWaves + wave] = mx;
                 uint32_t run = lblock_excl_scan(sum, s_tmp, &all);
-    #pragma unroll
+#pragma unroll
                 for (uint32_t i = 0; i < BPT; ++i) {
-                    s_work[tid * BPT + i] = run;
+                    s_bin[tid * BPT + i] = run;
                     run += cb[i];
                 }
                 mx = 0;
-    #pragma unroll
+#pragma unroll
                 for (int w = 0; w < kLWaves; ++w)
                     mx = s_red[2 * kLWaves + w] > mx ? s_red[2 * kLWaves + w] : mx;
                 __syncthreads();
                 if (mx <= kBinMax) {
-                    // keys to their bins (any order inside a bin); a bin's counter
-                    // ends at the next bin's start
-    #pragma unroll
+                    // keys to their bins (any order inside a bin); a bin's
+                    // counter ends at the next bin's start
+#pragma unroll
                     for (int k = 0; k < kLocItems; ++k) {
                         const uint32_t q = tid + (uint32_t)kLT * k;
                         if ((uint32_t)k < nk && q < S)
-                            s_e[atomicAdd(&s_work[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
+                            s_e[atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
                     }
                     __syncthreads();
                     for (uint32_t pz = tid; pz < S; pz += kLT) {
                         const uint32_t x = s_e[pz];
                         const uint32_t bn = (x >> sd) & (kBins - 1u);
-                        const uint32_t be = s_work[bn];
-                        const uint32_t bs = bn ? s_work[bn - 1u] : 0u;
+                        const uint32_t be = s_bin[bn];
+                        const uint32_t bs = bn ? s_bin[bn - 1u] : 0u;
                         uint32_t rank = 0;
                         for (uint32_t i = bs; i < be; ++i) {
                             const uint32_t y = s_e[i];
                             rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
                         }
-                        emit(bs + rank, x);
+                        const uint32_t pos = s + bs + rank;
+                        if (pos < out_lim) out[pos] = kpair(hs + (x >> L), x & lowmask, bbits, L);
                     }
                 } else {
-                    in_lds = false;  // a crowded bin: the global-memory path
+                    ok = false;  // a crowded bin
                 }
-                __syncthreads();  // s_e and s_work are reused
+                __syncthreads();  // s_e and s_bin are reused by the next window
             }
-            if (!in_lds) {
-                // Through global memory: keys relative to the range's first bucket,
-                // T bits, LSD passes X -> Y -> X ... with the last one into out.
-                const unsigned long long hs = pkey(X[s], bbits) >> L;
-                const unsigned long long he = pkey(X[e - 1], bbits) >> L;
-                const unsigned long long kb = hs << L;
-                const unsigned long long kmax = ((he - hs) << L) | lowmask;
-                const uint32_t T = 64u - (uint32_t)__clzll((long long)kmax);
-                const uint32_t npass = (T + 7u) / 8u;
-                for (uint32_t p = 0; p < npass; ++p) {
-                    const bool last = p + 1 == npass;
-                    const uint2* src = (p & 1u) ? Y : X;
-                    uint2* dst = last ? out : ((p & 1u) ? X : Y);
-                    const uint32_t sh = 8u * p;
-                    const int bw = (int)min(8u, T - sh);
-                    const uint32_t mask = (1u << bw) - 1u;
-                    if (tid < 256u) s_base[tid] = 0u;
-                    __syncthreads();
-                    for (uint32_t q = tid; q < S; q += kLT)
-                        atomicAdd(&s_base[(uint32_t)((pkey(src[s + q], bbits) - kb) >> sh) & mask], 1u);
-                    __syncthreads();
-                    uint32_t all;
-                    const uint32_t c0v = tid < 256u ? s_base[tid] : 0u;
-                    const uint32_t ex = lblock_excl_scan(c0v, s_tmp, &all);
-                    if (tid < 256u) s_base[tid] = s + ex;
-                    __syncthreads();
-                    for (uint32_t c0 = 0; c0 < S; c0 += kGlobChunk) {
-                        const uint32_t cs = min(S - c0, kGlobChunk);
-                        const uint32_t nk = (cs + kLT - 1) / kLT;
-                        const uint32_t w0 = c0 + wave * 64u * nk;
-                        for (uint32_t i = lane; i < 256u; i += 64u) s_wh[wave][i] = 0u;
-                        unsigned long long kk[kGlobItems];
-                        uint32_t rk[kGlobItems];
-    #pragma unroll
-                        for (int k = 0; k < kGlobItems; ++k) {
-                            const uint32_t q = w0 + 64u * k + lane;
-                            kk[k] = 0ull;
-                            if ((uint32_t)k < nk && q < c0 + cs) kk[k] = pkey(src[s + q], bbits) - kb;
-                        }
-    #pragma unroll
-                        for (int k = 0; k < kGlobItems; ++k) {
-                            if ((uint32_t)k < nk) {
-                                const uint32_t q = w0 + 64u * k + lane;
-                                rk[k] = wave_rank((uint32_t)(kk[k] >> sh) & mask, q < c0 + cs, bw,
-                                                  s_wh[wave], gtm);
-                            }
-                        }
-                        __syncthreads();
-                        if (tid < 256u) {
-                            uint32_t run = s_base[tid];
-    #pragma unroll
-                            for (int w = 0; w < kLWaves; ++w) {
-                                const uint32_t cw = s_wh[w][tid];
-                                s_wh[w][tid] = run;
-                                run += cw;
-                            }
-                            s_base[tid] = run;
-                        }
-                        __syncthreads();
-    #pragma unroll
-                        for (int k = 0; k < kGlobItems; ++k) {
-                            if ((uint32_t)k < nk) {
-                                const uint32_t q = w0 + 64u * k + lane;
-                                if (q < c0 + cs) {
-                                    const uint32_t g = s_wh[wave][(uint32_t)(kk[k] >> sh) & mask] + rk[k];
-                                    if (!last || g < out_lim) dst[g] = pkey_pair(kk[k] + kb, bbits);
-                                }
-                            }
-                        }
-                        __syncthreads();
-                    }
-                }
-            }
+            if (!ok && tid == 0) big[atomicAdd(big_ctr, 1u)] = j;
         }
         if (!fetched) fetch(s1, e1);
         s = s1;
@@ -408,24 +363,131 @@ __global__ __launch_bounds__(kLT, 4) void k_pair_local(uint2* X, uint2* Y, uint2
     }
 }
 
-uint64_t pair_local_wstart_words(uint64_t cap) { return cap / kLocW + 2; }
+// The windows k_pair_local handed on, one workgroup each (grid-stride):
+// keys relative to the window's first bucket, T bits, stable LSD passes
+// through global memory X -> Y -> X ... with the last one into out, one chunk
+// of kGlobChunk keys at a time (wave64 ballot ranking).
+__global__ __launch_bounds__(kLT) void k_pair_local_big(uint2* X, uint2* Y, uint2* __restrict__ out,
+                                                        const unsigned long long* __restrict__ n_dev,
+                                                        const unsigned long long* __restrict__ last_n,
+                                                        unsigned long long last_cap,
+                                                        const uint32_t* __restrict__ wstart,
+                                                        const uint32_t* __restrict__ big,
+                                                        const uint32_t* __restrict__ big_ctr, int bbits,
+                                                        int L) {
+    __shared__ uint32_t s_wh[kLWaves][256];
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_tmp[kLWaves];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
+    const uint32_t n = (uint32_t)*n_dev;
+    unsigned long long out_lim = n;
+    {
+        const unsigned long long ln = *last_n;
+        if (ln < out_lim) out_lim = ln;
+        if (last_cap < out_lim) out_lim = last_cap;
+    }
+    const uint32_t lowmask = (1u << L) - 1u;
+    const uint32_t nbig = *big_ctr;
+    for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
+        const uint32_t j = big[bi];
+        const uint32_t s = wstart[j], e = wstart[j + 1], S = e - s;
+        const unsigned long long hs = pkey(X[s], bbits) >> L;
+        const unsigned long long he = pkey(X[e - 1], bbits) >> L;
+        const unsigned long long kb = hs << L;
+        const unsigned long long kmax = ((he - hs) << L) | lowmask;
+        const uint32_t T = 64u - (uint32_t)__clzll((long long)kmax);
+        const uint32_t npass = (T + 7u) / 8u;
+        for (uint32_t p = 0; p < npass; ++p) {
+            const bool last = p + 1 == npass;
+            const uint2* src = (p & 1u) ? Y : X;
+            uint2* dst = last ? out : ((p & 1u) ? X : Y);
+            const uint32_t sh = 8u * p;
+            const int bw = (int)min(8u, T - sh);
+            const uint32_t mask = (1u << bw) - 1u;
+            if (tid < 256u) s_base[tid] = 0u;
+            __syncthreads();
+            for (uint32_t q = tid; q < S; q += kLT)
+                atomicAdd(&s_base[(uint32_t)((pkey(src[s + q], bbits) - kb) >> sh) & mask], 1u);
+            __syncthreads();
+            uint32_t all;
+            const uint32_t c0v = tid < 256u ? s_base[tid] : 0u;
+            const uint32_t ex = lblock_excl_scan(c0v, s_tmp, &all);
+            if (tid < 256u) s_base[tid] = s + ex;
+            __syncthreads();
+            for (uint32_t c0 = 0; c0 < S; c0 += kGlobChunk) {
+                const uint32_t cs = min(S - c0, kGlobChunk);
+                const uint32_t nk = (cs + kLT - 1) / kLT;
+                const uint32_t w0 = c0 + wave * 64u * nk;
+                for (uint32_t i = lane; i < 256u; i += 64u) s_wh[wave][i] = 0u;
+                unsigned long long kk[kGlobItems];
+                uint32_t rk[kGlobItems];
+#pragma unroll
+                for (int k = 0; k < kGlobItems; ++k) {
+                    const uint32_t q = w0 + 64u * k + lane;
+                    kk[k] = 0ull;
+                    if ((uint32_t)k < nk && q < c0 + cs) kk[k] = pkey(src[s + q], bbits) - kb;
+                }
+#pragma unroll
+                for (int k = 0; k < kGlobItems; ++k) {
+                    if ((uint32_t)k < nk) {
+                        const uint32_t q = w0 + 64u * k + lane;
+                        rk[k] = wave_rank((uint32_t)(kk[k] >> sh) & mask, q < c0 + cs, bw, s_wh[wave], gtm);
+                    }
+                }
+                __syncthreads();
+                if (tid < 256u) {
+                    uint32_t run = s_base[tid];
+#pragma unroll
+                    for (int w = 0; w < kLWaves; ++w) {
+                        const uint32_t cw = s_wh[w][tid];
+                        s_wh[w][tid] = run;
+                        run += cw;
+                    }
+                    s_base[tid] = run;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kGlobItems; ++k) {
+                    if ((uint32_t)k < nk) {
+                        const uint32_t q = w0 + 64u * k + lane;
+                        if (q < c0 + cs) {
+                            const uint32_t g = s_wh[wave][(uint32_t)(kk[k] >> sh) & mask] + rk[k];
+                            if (!last || g < out_lim) dst[g] = pkey_pair(kk[k] + kb, bbits);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// Window starts (J + 2 words) followed by the list of handed-on windows.
+static uint64_t wstart_half(uint64_t cap) { return cap / kLocW + 2; }
+uint64_t pair_local_wstart_words(uint64_t cap) { return 2 * wstart_half(cap); }
 
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
-                             unsigned long long cap, uint32_t* wstart, int bbits, int L,
-                             hipStream_t st) {
+                             unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
+                             int L, hipStream_t st) {
     if (L < 8 || L > 24) return hipErrorInvalidValue;
     const unsigned long long J = (cap + kLocW - 1) / kLocW;  // windows, at most
+    uint32_t* big = wstart + wstart_half(cap);
     unsigned long long gw = (J + 1 + kWaves - 1) / kWaves;
     if (gw > 2048) gw = 2048;
     hipLaunchKernelGGL(k_pair_wstart, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits, L,
-                       wstart);
+                       wstart, big_ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     unsigned long long gl = J == 0 ? 1 : J;
-    if (gl > 4096) gl = 4096;
-    hipLaunchKernelGGL(k_pair_local, dim3((uint32_t)gl), dim3(kLT), 0, st, x, y, out, n_dev, last_n,
-                       last_cap, wstart, bbits, L);
+    if (gl > 2048) gl = 2048;
+    hipLaunchKernelGGL(k_pair_local, dim3((uint32_t)gl), dim3(kLT), 0, st, x, out, n_dev, last_n,
+                       last_cap, wstart, big, big_ctr, bbits, L);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pair_local_big, dim3(256), dim3(kLT), 0, st, x, y, out, n_dev, last_n,
+                       last_cap, wstart, big, big_ctr, bbits, L);
     return hipGetLastError();
 }
 

@@ -119,6 +119,8 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
             for (uint32_t i = tid; i < a.npass_pair * 256u; i += kWG) a.hist_pair[i] = 0u;
             if (a.slabs) a.slab_cnt[tid] = 0u;  // kWG == 256 slabs
             if (tid < 2 * kMaxPasses) a.ctrl->tile_ctr[tid] = 0u;
+            if (a.pair_claim)
+                for (uint32_t i = tid; i < 2u * kMaxRegions * 256u; i += kWG) a.pair_claim[i] = 0u;
             if (tid == 0) a.ctrl->epoch += 1u;
         }
     }
@@ -818,13 +820,17 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     const float e1 = r0 * r0;
                     hit = t1 < e1;
                 }
+                // The conditions combine with non-short-circuit & so that they
+                // stay lane masks (scalar ANDs) instead of branches.
                 const uint32_t oorig = SLAB ? (o.orig & ~kFirstBit) : o.orig;
-                if (MASKED) hit = hit && t > o.aux;  // candidates after the object
+                if (MASKED) hit = hit & (t > o.aux);  // candidates after the object
                 if constexpr (SLAB) {
-                    if (MIXED) hit = hit && c.aux <= o.aux;  // candidate in the object's slab
-                    if (!(o.orig & kFirstBit)) hit = hit && cf;  // one of them is a first slab
+                    if (MIXED) hit = hit & (c.aux <= o.aux);  // candidate in the object's slab
+                    // one of them is a first slab (a uniform bool OR a lane mask)
+                    const bool ofirst = (o.orig & kFirstBit) != 0u;
+                    hit = hit & (ofirst | cf);
                 }
-                if (GROUP) hit = hit && a.group[oorig] != gc;
+                if (GROUP) hit = hit & (a.group[oorig] != gc);
                 const unsigned long long m = ballot(hit);
                 if (m == 0ull) return;
                 const uint32_t cnt = (uint32_t)__popcll(m) * mult;
@@ -852,15 +858,19 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 }
             };
             auto objects = [&](auto masked_tag, auto mixed_tag) {
-                uint32_t k = kfirst;
-                for (; k + 4 <= nobj; k += 4) {  // four scalar loads in flight
-                    const Rec o0 = O[k], o1 = O[k + 1], o2 = O[k + 2], o3 = O[k + 3];
+                CRec* p = O + kfirst;
+                CRec* const pe = O + nobj;
+                for (; p + 4 <= pe; p += 4) {  // four scalar loads in flight
+                    const Rec o0 = p[0], o1 = p[1], o2 = p[2], o3 = p[3];
                     process(o0, masked_tag, mixed_tag);
                     process(o1, masked_tag, mixed_tag);
                     process(o2, masked_tag, mixed_tag);
                     process(o3, masked_tag, mixed_tag);
                 }
-                for (; k < nobj; ++k) process(O[k], masked_tag, mixed_tag);
+                for (; p < pe; ++p) {
+                    const Rec o = *p;
+                    process(o, masked_tag, mixed_tag);
+                }
             };
             // an object may sit at or after the chunk start only in the first
             // chunks of the sub-tile
@@ -913,8 +923,14 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             }
         }
         if (a.hist) {  // every wave's digits are counted (they arrived after counting)
-            for (uint32_t i = lane; i < (uint32_t)a.hist_npass * 256u; i += 64u)
-                if (s_ph[i]) atomicAdd(&a.hist[i], s_ph[i]);
+            for (uint32_t i = lane; i < (uint32_t)a.hist_npass * 256u; i += 64u) {
+                const uint32_t c = s_ph[i];
+                if (c) {
+                    atomicAdd(&a.hist[i], c);
+                    // the first digit also per region, for the sort's first pass
+                    if (a.hist_reg && i < 256u) atomicAdd(&a.hist_reg[region * 256u + i], c);
+                }
+            }
         }
     }
 }

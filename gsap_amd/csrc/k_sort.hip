@@ -249,6 +249,106 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     }
 }
 
+// First pass of the pair sort without look-back.  An LSD sort's first pass
+// need not be stable, so a tile does not wait for its predecessors: it counts
+// its digits, claims its slots of each digit with one atomic per digit on the
+// counter of (its pair region, digit), and scatters.  Digit d of region r
+// occupies [start(d) + sum of d's counts in regions < r, + its count in r),
+// from the per-region histograms the sweep counted as it wrote the pairs;
+// regions are the sweep's per-XCD allocation units, so each counter is
+// contended only by one region's tiles.  Tiles are dealt grid-stride; items
+// are staged in LDS in digit order (LDS-atomic ranks) for a coalesced
+// scatter, never past the pair count.
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, uint2* __restrict__ kout,
+                                                SortArgs a) {
+    constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    static_assert(RADIX == kWG, "one digit per thread");
+    __shared__ uint32_t s_cnt[RADIX];
+    __shared__ uint32_t s_tstart[RADIX];
+    __shared__ uint32_t s_gbase[RADIX];
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint2 s_k[TILE];
+
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const int shift = a.shift0;
+    unsigned long long nd = *a.n_dev;
+    const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
+    unsigned long long nr = *a.n_dev_rest;
+    const uint32_t nreal = (uint32_t)(nr > a.cap ? a.cap : nr);
+    const uint32_t ntiles = (n + TILE - 1) / TILE;
+    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
+    // digit tid's start, and its counts in the regions before each region
+    uint32_t tot;
+    const uint32_t dstart = block_excl_scan(a.hist[tid], s_tmp, &tot);
+    uint32_t rpre[kMaxRegions];
+    {
+        uint32_t run = 0;
+#pragma unroll
+        for (int r = 0; r < kMaxRegions; ++r) {
+            rpre[r] = run;
+            if ((uint32_t)r < a.nregions) run += a.hist_reg[r * RADIX + tid];
+        }
+    }
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t lo = tile * TILE;
+        uint32_t hi = min(lo + TILE, n);
+        const uint32_t r = region_of(a.region_start, a.nregions, lo);
+        const uint2* src = kin + ((unsigned long long)r * a.region_stride - a.region_start[r]);
+        const unsigned long long ve = a.region_valid[r];
+        if (ve < hi) hi = (uint32_t)ve;
+        if (lo >= hi) continue;  // a tile of padding (uniform)
+        s_cnt[tid] = 0u;
+        __syncthreads();
+        const uint32_t base = lo + wave * 64u * ITEMS + lane;
+        uint2 key[ITEMS];
+        uint32_t dig[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t idx = base + 64u * k;
+            if (idx < hi) key[k] = src[idx];
+        }
+        // tile digit counts; a run of equal digits in neighbouring lanes is
+        // one LDS atomic (valid lanes are a prefix of each row)
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const bool valid = (base + 64u * k) < hi;
+            const uint32_t d = valid ? KPair::digit(key[k], shift, a.bbits, MASK) : 0u;
+            dig[k] = d;
+            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+            const bool head = valid && (lane == 0u || prev != d);
+            const unsigned long long hm = ballot(head);
+            const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+            if (head) {
+                const unsigned long long nxt = hm & above;
+                const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+                atomicAdd(&s_cnt[d], end - lane);
+            }
+        }
+        __syncthreads();
+        const uint32_t c = s_cnt[tid];
+        uint32_t g = 0;
+        if (c) g = atomicAdd(&a.claim[r * RADIX + tid], c);
+        const uint32_t ts = block_excl_scan(c, s_tmp, &tot);
+        s_gbase[tid] = dstart + rpre[r] + g;
+        s_tstart[tid] = ts;
+        s_cnt[tid] = ts;  // running LDS position of the digit
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if ((base + 64u * k) < hi) s_k[atomicAdd(&s_cnt[dig[k]], 1u)] = key[k];
+        __syncthreads();
+        const uint32_t tn = hi - lo;
+        for (uint32_t i = tid; i < tn; i += kWG) {
+            const uint2 kk = s_k[i];
+            const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
+            const uint32_t gp = s_gbase[d] + (i - s_tstart[d]);
+            if (gp < nreal) kout[gp] = kk;
+        }
+        __syncthreads();
+    }
+}
+
 // Histogram of every digit position (plain sorts and the pair sort).  Lanes
 // hold consecutive keys; equal digits in neighbouring lanes are counted as one
 // run (one LDS atomic per run), so the high digits of nearly sorted or narrow
@@ -356,6 +456,16 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
         typename KT::T* ko = (final_out && p + 1 == a.npass) ? final_out : (cur ? k0 : k1);
         uint32_t* vi = cur ? v1 : v0;
         uint32_t* vo = cur ? v0 : v1;
+        if constexpr (!KT::kHasVals && BITS == 8) {
+            if (p == 0 && a.claim && a.npass > 1) {  // never the final pass: no capacity clamp
+                hipLaunchKernelGGL((k_msplit<BITS, ITEMS>), dim3(grid), dim3(kWG), 0, st, ki, ko, a);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+                if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
+                cur ^= 1;
+                continue;
+            }
+        }
         hipLaunchKernelGGL((k_onesweep<KT, BITS, ITEMS>), dim3(grid), dim3(kWG), 0, st, ki, ko, vi, vo,
                            a, (int)p);
         hipError_t e = hipGetLastError();

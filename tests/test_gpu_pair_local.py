@@ -131,3 +131,23 @@ def test_local_capacity_error_then_retry(dev, loc, regions):
     big = torch.empty((len(ref), 2), dtype=torch.int32, device=dev)
     P, _, _ = s.collide_into(bd, big)
     assert P == len(ref) and np.array_equal(pairs_np(big), ref)
+
+
+@pytest.mark.parametrize("regions", [1, 8])
+def test_local_two_pass_capacity_error_then_retry(dev, regions):
+    # K = 38 (two onesweep passes; with 8 regions the first one claims its
+    # slots per region): an overflowing call reports the capacity and the
+    # retry into a large enough buffer is exact
+    import torch
+    b = W.aniso(300_000, seed=4, sigma=(20.0, 5.0, 1.0))
+    bd = to_dev(b, dev)
+    ref, _, _ = G.Sap().collide(bd)
+    s = G.Sap()
+    s.set_tune(pair_local=1, pair_regions=regions)
+    small = torch.empty((ref.shape[0] // 3, 2), dtype=torch.int32, device=dev)
+    with pytest.raises(G.GsapError) as e:
+        s.collide_into(bd, small)
+    assert e.value.code == G.ERR_CAPACITY
+    big = torch.empty_like(ref)
+    P, _, _ = s.collide_into(bd, big)
+    assert P == ref.shape[0] and torch.equal(big, ref)
