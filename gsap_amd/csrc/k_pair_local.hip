@@ -182,7 +182,9 @@ __device__ __forceinline__ uint2 kpair(uint32_t h, uint32_t lo, int bbits, int L
 // pairs whose high bits lie within 2^(32-L) of the first one's, as unique
 // 32-bit keys (high - first high) << L | low bits.  One counting pass on the
 // kBinBits highest bits in which the window's keys differ puts every key in
-// its bin; a key's place in its bin is the number of the bin's smaller keys.
+// its bin; a key's place in its bin is the number of the bin's smaller keys
+// (bins are uneven -- keys cluster by bucket and index -- so the work is
+// spread over the keys rather than over the bins).
 // Windows that do not fit (too many pairs, buckets too far apart, a bin of
 // more than kBinMax keys) are appended to `big` for k_pair_local_big.
 // Software pipeline over the workgroup's windows j, j + G, ...: window starts
@@ -326,8 +328,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     mx = s_red[2 * kLWaves + w] > mx ? s_red[2 * kLWaves + w] : mx;
                 __syncthreads();
                 if (mx <= kBinMax) {
-                    // keys to their bins (any order inside a bin); a bin's
-                    // counter ends at the next bin's start
+                    // keys to their bins (any order inside a bin)
 #pragma unroll
                     for (int k = 0; k < kLocItems; ++k) {
                         const uint32_t q = tid + (uint32_t)kLT * k;
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     for (uint32_t pz = tid; pz < S; pz += kLT) {
                         const uint32_t x = s_e[pz];
                         const uint32_t bn = (x >> sd) & (kBins - 1u);
-                        const uint32_t be = s_bin[bn];
+                        const uint32_t be = s_bin[bn];  // after the scatter: the bin's end
                         const uint32_t bs = bn ? s_bin[bn - 1u] : 0u;
                         uint32_t rank = 0;
                         for (uint32_t i = bs; i < be; ++i) {

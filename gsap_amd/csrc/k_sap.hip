@@ -790,14 +790,14 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             uint32_t gc = 0;
             if (GROUP) gc = a.group[corig];
             // lanes whose candidate entry is its box's first slab
-            const bool cf = SLAB && (c.orig & kFirstBit) != 0u;
+            const unsigned long long cfm = SLAB ? ballot((c.orig & kFirstBit) != 0u) : 0ull;
 
             // The exact test is the only per-lane arithmetic on the common path;
             // the other conditions are lane masks (scalar ANDs) or uniform branches.
             auto process = [&](const Rec& o, auto masked_tag, auto mixed_tag) {
                 constexpr bool MASKED = decltype(masked_tag)::value;
                 constexpr bool MIXED = decltype(mixed_tag)::value;
-                bool hit;
+                bool hit;  // this lane's test; the wave works with masks below
                 if constexpr (KIND == 0) {
                     // Inclusive AABB test (aabb.h:47-57) on the vector ALU only:
                     // for finite floats with denormals preserved (the kernel's f32
@@ -820,18 +820,19 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     const float e1 = r0 * r0;
                     hit = t1 < e1;
                 }
-                // The conditions combine with non-short-circuit & so that they
-                // stay lane masks (scalar ANDs) instead of branches.
+                // The wave's hit mask is the AND of the ballots of the compares
+                // (scalar ANDs; a ballot of a combined bool would be rebuilt
+                // through VGPRs), and the lanes that stage a pair are selected
+                // by the mask itself (inverse ballot: the mask becomes exec).
                 const uint32_t oorig = SLAB ? (o.orig & ~kFirstBit) : o.orig;
-                if (MASKED) hit = hit & (t > o.aux);  // candidates after the object
+                unsigned long long m = ballot(hit);
+                if (MASKED) m &= ballot(t > o.aux);  // candidates after the object
                 if constexpr (SLAB) {
-                    if (MIXED) hit = hit & (c.aux <= o.aux);  // candidate in the object's slab
-                    // one of them is a first slab (a uniform bool OR a lane mask)
-                    const bool ofirst = (o.orig & kFirstBit) != 0u;
-                    hit = hit & (ofirst | cf);
+                    if (MIXED) m &= ballot(c.aux <= o.aux);  // candidate in the object's slab
+                    // one of them is a first slab
+                    m &= (o.orig & kFirstBit) ? ~0ull : cfm;
                 }
-                if (GROUP) hit = hit & (a.group[oorig] != gc);
-                const unsigned long long m = ballot(hit);
+                if (GROUP) m &= ballot(a.group[oorig] != gc);
                 if (m == 0ull) return;
                 const uint32_t cnt = (uint32_t)__popcll(m) * mult;
                 npairs += cnt;
@@ -842,7 +843,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                         fill = 0;
                         __builtin_amdgcn_wave_barrier();
                     }
-                    if (hit) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(m)) {
                         const uint32_t r = fill + mult * mbcnt(m);
                         if (MODE == 0) {
                             const uint32_t x = oorig < corig ? oorig : corig;
