@@ -75,14 +75,14 @@ def bench_batched(G, W, torch, dev, ns, steps, warmup, cache=None):
     boxes, off = W.scenes(ns, seed=4)
     ntot = int(off[-1])
     bd = torch.from_numpy(boxes).to(dev)
-    sb = G.Sap(max_n=ntot)
+    sb = G.Sap(max_n=ntot, axis_mode=G.AXIS_AUTO)
     tune = sb.autotune(bd, scene_offsets=off, cache=cache)
     P, st = sb.collide_batched(bd, off, None, None)
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     poff = torch.empty(ns + 1, dtype=torch.int64, device=dev)
     ms_b = _time_ms(torch, lambda: sb.collide_batched(bd, off, pairs, poff), steps, warmup)
     po = poff.cpu().numpy()
-    sp = G.Sap(max_n=int(np.max(np.diff(off))))
+    sp = G.Sap(max_n=int(np.max(np.diff(off))), axis_mode=G.AXIS_AUTO)
     lp = torch.empty_like(pairs)
     counts = torch.zeros((ns, 8), dtype=torch.int64, device=dev)
     views = [(bd[off[s]:off[s + 1]], lp[po[s]:po[s + 1]]) for s in range(ns)]
@@ -125,12 +125,13 @@ def bench_learned(G, W, torch, dev, ns):
     boxes, off = W.scenes(ns, seed=5)
     bd = torch.from_numpy(boxes).to(dev)
     ev = L.evaluate(model, bd, off, conf_min=0.95)
-    ms = {}
-    for name, mode in (("pca", G.AXIS_PCA), ("learned", G.AXIS_LEARNED)):
+    ms, tests = {}, {}
+    for name, mode in (("pca", G.AXIS_PCA), ("learned", G.AXIS_LEARNED), ("auto", G.AXIS_AUTO)):
         s = G.Sap(max_n=int(off[-1]), axis_mode=mode)
         if mode == G.AXIS_LEARNED:
             s.set_learned_model(model, 0.95)
-        P, _ = s.collide_batched(bd, off, None, None)
+        P, st = s.collide_batched(bd, off, None, None)
+        tests[name] = int(st.sweep_tests)
         pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
         po = torch.empty(ns + 1, dtype=torch.int64, device=dev)
         ms[name] = _time_ms(torch, lambda: s.collide_batched(bd, off, pairs, po), 5, 2)
@@ -139,7 +140,9 @@ def bench_learned(G, W, torch, dev, ns):
             "pairs": ev["pairs"], "sweep_tests_pca": ev["tests_pca"],
             "sweep_tests_learned": ev["tests_learned"], "reduction": round(ev["reduction"], 4),
             "pairs_identical": ev["pairs_identical"], "ms_batched_pca": round(ms["pca"], 4),
-            "ms_batched_learned": round(ms["learned"], 4)}
+            "ms_batched_learned": round(ms["learned"], 4),
+            # the cost-model choice of PCA or a coordinate axis, for comparison
+            "sweep_tests_auto": tests["auto"], "ms_batched_auto": round(ms["auto"], 4)}
 
 
 def _tune_particles(G, torch, sap, pp, npart, cache=None):

@@ -33,7 +33,7 @@ constexpr int kLWaves = kLT / 64;
 constexpr int kLocItems = kLocC / kLT;  // keys per thread of an LDS sort
 constexpr uint32_t kBinBits = 12;       // counting pass of an LDS sort: 4096 bins
 constexpr uint32_t kBins = 1u << kBinBits;
-constexpr uint32_t kBinMax = 32;        // largest bin ranked by comparisons
+constexpr uint32_t kBinMax = 256;       // largest bin ranked by comparisons (each key walks its bin)
 constexpr uint32_t kLocTail = 64;      // LDS words of the key array used for reductions
 constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
 constexpr uint32_t kGlobChunk = kLT * kGlobItems;
@@ -49,15 +49,17 @@ __device__ __forceinline__ uint32_t wave_and_u32(uint32_t v) {
     return v;
 }
 
-// Exclusive scan over the kLT threads of a workgroup (s_tmp: kLWaves words).
+// Exclusive scan over the LT threads of a workgroup (s_tmp: LT / 64 words).
+template <int LT>
 __device__ __forceinline__ uint32_t lblock_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+    constexpr int NW = LT / 64;
     const uint32_t l = lane_id(), w = threadIdx.x >> 6;
     const uint32_t inc = wave_incl_scan(v);
     if (l == 63) s_tmp[w] = inc;
     __syncthreads();
     uint32_t off = 0, tot = 0;
 #pragma unroll
-    for (int i = 0; i < kLWaves; ++i) {
+    for (int i = 0; i < NW; ++i) {
         const uint32_t t = s_tmp[i];
         off += i < (int)w ? t : 0u;
         tot += t;
@@ -177,16 +179,111 @@ __device__ __forceinline__ uint2 kpair(uint32_t h, uint32_t lo, int bbits, int L
     return make_uint2(h >> (bbits - L), ((h & ((1u << (bbits - L)) - 1u)) << L) | lo);
 }
 
+// Counting sort of one window in LDS by LT threads, ITEMS keys each (kk,
+// the first nk rows valid, S keys in all; s_e holds at least S words).  The
+// keys are unique 32-bit views (high - hs) << L | low bits.  One counting
+// pass on the kBinBits highest bits in which the window's keys differ puts
+// every key in its bin; a key's place in its bin is the number of the bin's
+// smaller keys (bins are uneven -- keys cluster by bucket and index -- so the
+// work is spread over the keys rather than over the bins).  Writes the pairs
+// to out[s + rank] (below out_lim) and returns true, or returns false
+// without writing when a bin holds more than kBinMax keys.  Uniform control
+// flow; ends with a barrier, so the LDS may be reused at once.
+template <int LT, int ITEMS>
+__device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t nk, uint32_t S,
+                                         uint32_t s, uint32_t hs, unsigned long long out_lim,
+                                         uint2* __restrict__ out, uint32_t* s_e, uint32_t* s_bin,
+                                         uint32_t* s_tmp, uint32_t* s_red, int bbits, int L) {
+    constexpr int NW = LT / 64;
+    constexpr uint32_t BPT = kBins / LT;  // bins per thread
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t lowmask = (1u << L) - 1u;
+    uint32_t vor = 0u, vand = ~0u;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t q = tid + (uint32_t)LT * k;
+        if ((uint32_t)k < nk && q < S) {
+            vor |= kk[k];
+            vand &= kk[k];
+        }
+    }
+    vor = wave_or_u32(vor);
+    vand = wave_and_u32(vand);
+    if (lane == 0) {
+        s_red[wave] = vor;
+        s_red[NW + wave] = vand;
+    }
+    for (uint32_t i = tid; i < kBins; i += LT) s_bin[i] = 0u;
+    __syncthreads();
+    vor = 0u;
+    vand = ~0u;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        vor |= s_red[w];
+        vand &= s_red[NW + w];
+    }
+    // bins: the kBinBits highest bits in which the keys differ (the bits
+    // above are common to all of them)
+    const uint32_t diff = vor ^ vand;
+    const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
+    const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t q = tid + (uint32_t)LT * k;
+        if ((uint32_t)k < nk && q < S) atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u);
+    }
+    __syncthreads();
+    // bin starts (exclusive scan, BPT bins per thread) and the largest bin
+    uint32_t cb[BPT], sum = 0, mx = 0, all;
+#pragma unroll
+    for (uint32_t i = 0; i < BPT; ++i) {
+        cb[i] = s_bin[tid * BPT + i];
+        sum += cb[i];
+        mx = cb[i] > mx ? cb[i] : mx;
+    }
+    mx = wave_max_u32(mx);
+    if (lane == 0) s_red[2 * NW + wave] = mx;
+    uint32_t run = lblock_excl_scan<LT>(sum, s_tmp, &all);
+#pragma unroll
+    for (uint32_t i = 0; i < BPT; ++i) {
+        s_bin[tid * BPT + i] = run;
+        run += cb[i];
+    }
+    mx = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) mx = s_red[2 * NW + w] > mx ? s_red[2 * NW + w] : mx;
+    __syncthreads();
+    if (mx > kBinMax) return false;  // uniform; s_e and s_bin are free again
+    // keys to their bins (any order inside a bin); a bin's counter ends at
+    // the next bin's start
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t q = tid + (uint32_t)LT * k;
+        if ((uint32_t)k < nk && q < S) s_e[atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
+    }
+    __syncthreads();
+    for (uint32_t pz = tid; pz < S; pz += LT) {
+        const uint32_t x = s_e[pz];
+        const uint32_t bn = (x >> sd) & (kBins - 1u);
+        const uint32_t be = s_bin[bn];
+        const uint32_t bs = bn ? s_bin[bn - 1u] : 0u;
+        uint32_t rank = 0;
+        for (uint32_t i = bs; i < be; ++i) {
+            const uint32_t y = s_e[i];
+            rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
+        }
+        const uint32_t pos = s + bs + rank;
+        if (pos < out_lim) out[pos] = kpair(hs + (x >> L), x & lowmask, bbits, L);
+    }
+    __syncthreads();
+    return true;
+}
+
 // LDS path: one workgroup per window j (grid-stride), sorting the buckets that
-// start in it, positions [wstart[j], wstart[j+1]).  Up to kLocC - kLocTail
-// pairs whose high bits lie within 2^(32-L) of the first one's, as unique
-// 32-bit keys (high - first high) << L | low bits.  One counting pass on the
-// kBinBits highest bits in which the window's keys differ puts every key in
-// its bin; a key's place in its bin is the number of the bin's smaller keys
-// (bins are uneven -- keys cluster by bucket and index -- so the work is
-// spread over the keys rather than over the bins).
-// Windows that do not fit (too many pairs, buckets too far apart, a bin of
-// more than kBinMax keys) are appended to `big` for k_pair_local_big.
+// start in it, positions [wstart[j], wstart[j+1]), by bin_sort: up to
+// kLocC - kLocTail pairs whose high bits lie within 2^(32-L) of the first
+// one's.  Windows that do not fit (too many pairs, buckets too far apart, a
+// bin of more than kBinMax keys) are appended to `big` for k_pair_local_big.
 // Software pipeline over the workgroup's windows j, j + G, ...: window starts
 // are read two windows ahead, and the next window's pairs are loaded into
 // registers while this window is sorted.
@@ -204,7 +301,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     uint32_t* const s_red = s_tmp + kLWaves;
     constexpr uint32_t kCap = kLocC - kLocTail;
 
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     const uint32_t n = (uint32_t)*n_dev;
     unsigned long long out_lim = n;
     {
@@ -214,7 +311,6 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     }
     const uint32_t J = (n + kLocW - 1) / kLocW;
     const bool fits32 = 2 * bbits - L <= 32;
-    const uint32_t lowmask = (1u << L) - 1u;  // 8 <= L <= 24
     const uint32_t G = gridDim.x;
 
     uint32_t j = blockIdx.x, s = 0, e = 0, s1 = 0, e1 = 0;
@@ -257,7 +353,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             bool ok = fits32 && S <= kCap && khigh(rlast, bbits, L) - hs < (1u << (32 - L));
             if (ok) {
                 const uint32_t nk = (S + kLT - 1) / kLT;
-                uint32_t kk[kLocItems], vor = 0u, vand = ~0u;
+                uint32_t kk[kLocItems];
 #pragma unroll
                 for (int k = 0; k < kLocItems; ++k) {
                     uint2 p = make_uint2(0u, 0u);
@@ -271,88 +367,8 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 }
                 fetch(s1, e1);
                 fetched = true;
-#pragma unroll
-                for (int k = 0; k < kLocItems; ++k) {
-                    const uint32_t q = tid + (uint32_t)kLT * k;
-                    if ((uint32_t)k < nk && q < S) {
-                        vor |= kk[k];
-                        vand &= kk[k];
-                    }
-                }
-                vor = wave_or_u32(vor);
-                vand = wave_and_u32(vand);
-                if (lane == 0) {
-                    s_red[wave] = vor;
-                    s_red[kLWaves + wave] = vand;
-                }
-                for (uint32_t i = tid; i < kBins; i += kLT) s_bin[i] = 0u;
-                __syncthreads();
-                vor = 0u;
-                vand = ~0u;
-#pragma unroll
-                for (int w = 0; w < kLWaves; ++w) {
-                    vor |= s_red[w];
-                    vand &= s_red[kLWaves + w];
-                }
-                // bins: the kBinBits highest bits in which the keys differ (the
-                // bits above are common to all of them)
-                const uint32_t diff = vor ^ vand;
-                const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
-                const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
-#pragma unroll
-                for (int k = 0; k < kLocItems; ++k) {
-                    const uint32_t q = tid + (uint32_t)kLT * k;
-                    if ((uint32_t)k < nk && q < S) atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u);
-                }
-                __syncthreads();
-                // bin starts (exclusive scan, BPT bins per thread) and the largest bin
-                constexpr uint32_t BPT = kBins / kLT;
-                uint32_t cb[BPT], sum = 0, mx = 0, all;
-#pragma unroll
-                for (uint32_t i = 0; i < BPT; ++i) {
-                    cb[i] = s_bin[tid * BPT + i];
-                    sum += cb[i];
-                    mx = cb[i] > mx ? cb[i] : mx;
-                }
-                mx = wave_max_u32(mx);
-                if (lane == 0) s_red[2 * kLWaves + wave] = mx;
-                uint32_t run = lblock_excl_scan(sum, s_tmp, &all);
-#pragma unroll
-                for (uint32_t i = 0; i < BPT; ++i) {
-                    s_bin[tid * BPT + i] = run;
-                    run += cb[i];
-                }
-                mx = 0;
-#pragma unroll
-                for (int w = 0; w < kLWaves; ++w)
-                    mx = s_red[2 * kLWaves + w] > mx ? s_red[2 * kLWaves + w] : mx;
-                __syncthreads();
-                if (mx <= kBinMax) {
-                    // keys to their bins (any order inside a bin)
-#pragma unroll
-                    for (int k = 0; k < kLocItems; ++k) {
-                        const uint32_t q = tid + (uint32_t)kLT * k;
-                        if ((uint32_t)k < nk && q < S)
-                            s_e[atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
-                    }
-                    __syncthreads();
-                    for (uint32_t pz = tid; pz < S; pz += kLT) {
-                        const uint32_t x = s_e[pz];
-                        const uint32_t bn = (x >> sd) & (kBins - 1u);
-                        const uint32_t be = s_bin[bn];  // after the scatter: the bin's end
-                        const uint32_t bs = bn ? s_bin[bn - 1u] : 0u;
-                        uint32_t rank = 0;
-                        for (uint32_t i = bs; i < be; ++i) {
-                            const uint32_t y = s_e[i];
-                            rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
-                        }
-                        const uint32_t pos = s + bs + rank;
-                        if (pos < out_lim) out[pos] = kpair(hs + (x >> L), x & lowmask, bbits, L);
-                    }
-                } else {
-                    ok = false;  // a crowded bin
-                }
-                __syncthreads();  // s_e and s_bin are reused by the next window
+                ok = bin_sort<kLT, kLocItems>(kk, nk, S, s, hs, out_lim, out, s_e, s_bin, s_tmp,
+                                              s_red, bbits, L);
             }
             if (!ok && tid == 0) big[atomicAdd(big_ctr, 1u)] = j;
         }
@@ -364,21 +380,35 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     }
 }
 
-// The windows k_pair_local handed on, one workgroup each (grid-stride):
-// keys relative to the window's first bucket, T bits, stable LSD passes
-// through global memory X -> Y -> X ... with the last one into out, one chunk
-// of kGlobChunk keys at a time (wave64 ballot ranking).
-__global__ __launch_bounds__(kLT) void k_pair_local_big(uint2* X, uint2* Y, uint2* __restrict__ out,
-                                                        const unsigned long long* __restrict__ n_dev,
-                                                        const unsigned long long* __restrict__ last_n,
-                                                        unsigned long long last_cap,
-                                                        const uint32_t* __restrict__ wstart,
-                                                        const uint32_t* __restrict__ big,
-                                                        const uint32_t* __restrict__ big_ctr, int bbits,
-                                                        int L) {
-    __shared__ uint32_t s_wh[kLWaves][256];
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_tmp[kLWaves];
+// The windows k_pair_local handed on, one 1024-thread workgroup each (grid-
+// stride): bin_sort with twice the capacity (kBigC keys) where they fit,
+// otherwise stable LSD passes through global memory X -> Y -> X ... with the
+// last one into out, over keys relative to the window's first bucket, one
+// chunk of kBigT * kGlobItems keys at a time (wave64 ballot ranking).
+constexpr int kBigT = 1024;
+constexpr int kBigItems = 16;
+constexpr uint32_t kBigC = kBigT * kBigItems;
+__global__ __launch_bounds__(kBigT) void k_pair_local_big(uint2* X, uint2* Y, uint2* __restrict__ out,
+                                                          const unsigned long long* __restrict__ n_dev,
+                                                          const unsigned long long* __restrict__ last_n,
+                                                          unsigned long long last_cap,
+                                                          const uint32_t* __restrict__ wstart,
+                                                          const uint32_t* __restrict__ big,
+                                                          const uint32_t* __restrict__ big_ctr, int bbits,
+                                                          int L) {
+    constexpr int NW = kBigT / 64;
+    constexpr uint32_t kChunk = kBigT * kGlobItems;
+    // one LDS block, 80 KB: keys | bins (the LSD path's counters reuse it)
+    __shared__ uint32_t s_mem[kBigC + kBins];
+    uint32_t* const s_e = s_mem;
+    uint32_t* const s_bin = s_mem + kBigC;
+    uint32_t* const s_tmp = s_e + (kBigC - kLocTail);
+    uint32_t* const s_red = s_tmp + NW;
+    uint32_t(*const s_wh)[256] = reinterpret_cast<uint32_t(*)[256]>(s_mem);  // [NW][256]
+    uint32_t* const s_base = s_mem + NW * 256;
+    uint32_t* const s_tmp2 = s_base + 256;
+    constexpr uint32_t kCap = kBigC - kLocTail;
+
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
     const uint32_t n = (uint32_t)*n_dev;
@@ -388,11 +418,29 @@ __global__ __launch_bounds__(kLT) void k_pair_local_big(uint2* X, uint2* Y, uint
         if (ln < out_lim) out_lim = ln;
         if (last_cap < out_lim) out_lim = last_cap;
     }
+    const bool fits32 = 2 * bbits - L <= 32;
     const uint32_t lowmask = (1u << L) - 1u;
     const uint32_t nbig = *big_ctr;
     for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
         const uint32_t j = big[bi];
         const uint32_t s = wstart[j], e = wstart[j + 1], S = e - s;
+        if (fits32 && S <= kCap) {
+            const uint32_t hs = khigh(X[s], bbits, L);
+            if (khigh(X[e - 1], bbits, L) - hs < (1u << (32 - L))) {
+                const uint32_t nk = (S + kBigT - 1) / kBigT;
+                uint32_t kk[kBigItems];
+#pragma unroll
+                for (int k = 0; k < kBigItems; ++k) {
+                    uint2 p = make_uint2(0u, 0u);
+                    const uint32_t q = tid + (uint32_t)kBigT * k;
+                    if ((uint32_t)k < nk && q < S) p = X[s + q];
+                    kk[k] = ((khigh(p, bbits, L) - hs) << L) | klow(p, bbits, L);
+                }
+                if (bin_sort<kBigT, kBigItems>(kk, nk, S, s, hs, out_lim, out, s_e, s_bin, s_tmp,
+                                               s_red, bbits, L))
+                    continue;
+            }
+        }
         const unsigned long long hs = pkey(X[s], bbits) >> L;
         const unsigned long long he = pkey(X[e - 1], bbits) >> L;
         const unsigned long long kb = hs << L;
@@ -408,17 +456,17 @@ __global__ __launch_bounds__(kLT) void k_pair_local_big(uint2* X, uint2* Y, uint
             const uint32_t mask = (1u << bw) - 1u;
             if (tid < 256u) s_base[tid] = 0u;
             __syncthreads();
-            for (uint32_t q = tid; q < S; q += kLT)
+            for (uint32_t q = tid; q < S; q += kBigT)
                 atomicAdd(&s_base[(uint32_t)((pkey(src[s + q], bbits) - kb) >> sh) & mask], 1u);
             __syncthreads();
             uint32_t all;
             const uint32_t c0v = tid < 256u ? s_base[tid] : 0u;
-            const uint32_t ex = lblock_excl_scan(c0v, s_tmp, &all);
+            const uint32_t ex = lblock_excl_scan<kBigT>(c0v, s_tmp2, &all);
             if (tid < 256u) s_base[tid] = s + ex;
             __syncthreads();
-            for (uint32_t c0 = 0; c0 < S; c0 += kGlobChunk) {
-                const uint32_t cs = min(S - c0, kGlobChunk);
-                const uint32_t nk = (cs + kLT - 1) / kLT;
+            for (uint32_t c0 = 0; c0 < S; c0 += kChunk) {
+                const uint32_t cs = min(S - c0, kChunk);
+                const uint32_t nk = (cs + kBigT - 1) / kBigT;
                 const uint32_t w0 = c0 + wave * 64u * nk;
                 for (uint32_t i = lane; i < 256u; i += 64u) s_wh[wave][i] = 0u;
                 unsigned long long kk[kGlobItems];
@@ -440,7 +488,7 @@ __global__ __launch_bounds__(kLT) void k_pair_local_big(uint2* X, uint2* Y, uint
                 if (tid < 256u) {
                     uint32_t run = s_base[tid];
 #pragma unroll
-                    for (int w = 0; w < kLWaves; ++w) {
+                    for (int w = 0; w < NW; ++w) {
                         const uint32_t cw = s_wh[w][tid];
                         s_wh[w][tid] = run;
                         run += cw;
@@ -487,7 +535,7 @@ hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long
                        last_cap, wstart, big, big_ctr, bbits, L);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pair_local_big, dim3(256), dim3(kLT), 0, st, x, y, out, n_dev, last_n,
+    hipLaunchKernelGGL(k_pair_local_big, dim3(256), dim3(kBigT), 0, st, x, y, out, n_dev, last_n,
                        last_cap, wstart, big, big_ctr, bbits, L);
     return hipGetLastError();
 }

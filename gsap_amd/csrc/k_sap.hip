@@ -770,14 +770,16 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         uint32_t kfirst = 0;
         // Candidate chunks are double-buffered: the next chunk's records are
         // requested before this chunk's objects are tested.
-        Rec cn{};
-        if (lo + 1u < wend) cn = R[min(lo + 1u + lane, wend - 1u)];
-        for (uint32_t t0 = lo + 1; t0 < wend; t0 += 64u) {
-            // retire objects whose sweep end is at or before this chunk
-            kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
-            const uint32_t t = t0 + lane;
-            Rec c = cn;
-            if (t0 + 64u < wend) cn = R[min(t + 64u, wend - 1u)];
+        // Candidates are taken two 64-record chunks at a time, so that each
+        // of the wave's object records is read (through the scalar cache)
+        // once per 128 candidates; the next pair of chunks is requested
+        // before this pair's objects are tested.
+        struct Chunk {
+            Rec c;
+            uint32_t t, corig, gc;
+            unsigned long long cfm;  // lanes whose candidate entry is its box's first slab
+        };
+        auto chunk = [&](Rec c, uint32_t t) {
             if (t >= wend) {  // a candidate that overlaps nothing
                 if constexpr (KIND == 0) {
                     c.lo[0] = c.lo[1] = c.lo[2] = __builtin_inff();
@@ -786,17 +788,40 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     c.c[0] = __builtin_inff();
                 }
             }
-            const uint32_t corig = SLAB ? (c.orig & ~kFirstBit) : c.orig;
-            uint32_t gc = 0;
-            if (GROUP) gc = a.group[corig];
-            // lanes whose candidate entry is its box's first slab
-            const unsigned long long cfm = SLAB ? ballot((c.orig & kFirstBit) != 0u) : 0ull;
+            Chunk k;
+            k.c = c;
+            k.t = t;
+            k.corig = SLAB ? (c.orig & ~kFirstBit) : c.orig;
+            k.gc = GROUP ? a.group[k.corig] : 0u;
+            k.cfm = SLAB ? ballot((c.orig & kFirstBit) != 0u) : 0ull;
+            return k;
+        };
+        Rec cn{}, cn2{};
+        if (lo + 1u < wend) {
+            cn = R[min(lo + 1u + lane, wend - 1u)];
+            cn2 = R[min(lo + 65u + lane, wend - 1u)];
+        }
+        for (uint32_t t0 = lo + 1; t0 < wend; t0 += 128u) {
+            // retire objects whose sweep end is at or before this chunk pair
+            kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
+            const uint32_t t = t0 + lane;
+            const Rec c1r = cn, c2r = cn2;
+            if (t0 + 128u < wend) {
+                cn = R[min(t + 128u, wend - 1u)];
+                cn2 = R[min(t + 192u, wend - 1u)];
+            }
+            const bool two = t0 + 64u < wend;  // uniform
+            const Chunk k1 = chunk(c1r, t);
+            const Chunk k2 = chunk(c2r, t + 64u);
 
             // The exact test is the only per-lane arithmetic on the common path;
             // the other conditions are lane masks (scalar ANDs) or uniform branches.
-            auto process = [&](const Rec& o, auto masked_tag, auto mixed_tag) {
+            auto one = [&](const Rec& o, const Chunk& ck, auto masked_tag, auto mixed_tag) {
                 constexpr bool MASKED = decltype(masked_tag)::value;
                 constexpr bool MIXED = decltype(mixed_tag)::value;
+                const Rec& c = ck.c;
+                const uint32_t t = ck.t, corig = ck.corig, gc = ck.gc;
+                const unsigned long long cfm = ck.cfm;
                 bool hit;  // this lane's test; the wave works with masks below
                 if constexpr (KIND == 0) {
                     // Inclusive AABB test (aabb.h:47-57) on the vector ALU only:
@@ -858,6 +883,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     __builtin_amdgcn_wave_barrier();
                 }
             };
+            auto process = [&](const Rec& o, auto masked_tag, auto mixed_tag) {
+                one(o, k1, masked_tag, mixed_tag);
+                if (two) one(o, k2, masked_tag, mixed_tag);
+            };
             auto objects = [&](auto masked_tag, auto mixed_tag) {
                 CRec* p = O + kfirst;
                 CRec* const pe = O + nobj;
@@ -883,8 +912,9 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 else objects(std::false_type{}, std::false_type{});
             }
         }
-        // candidate tests executed: every active object takes whole 64-lane chunks
-        if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 63u) / 64u) * 64u;
+        // candidate tests executed: every active object takes whole pairs of
+        // 64-lane chunks (an upper bound: the wave's last pair may be single)
+        if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 127u) / 128u) * 128u;
         ex = wave_sum(ex);
     }
 
