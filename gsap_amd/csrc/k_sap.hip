@@ -720,21 +720,24 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     // whole grid contending for a single one.
     const uint32_t region = a.nregions > 1 ? blockIdx.x % a.nregions : 0u;
     uint2* const rout = a.out + (size_t)region * a.region_stride;
-    auto flush = [&](uint32_t cnt) {  // cnt > 0, wave-uniform
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&a.ctrl->region_fill[region], (unsigned long long)cnt);
-        base = __builtin_amdgcn_readfirstlane((uint32_t)base) |
-               ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32);
-        return base;
+    // Write the staged pairs st[0, cnt) of this wave (cnt > 0, wave-uniform):
+    // claim cnt slots of the workgroup's region, count the pairs' sort digits
+    // in LDS while the claim is in flight, then store them (within the
+    // capacity).  Canonical mode stages (object, candidate) and orders the
+    // pair here, once per pair rather than once per tested chunk.  (After a
+    // capacity overflow the digit counts include unwritten pairs; the update
+    // then reports the overflow and its pairs are not used.)
+    auto claim = [&](uint32_t cnt) {  // issue the slot claim; the base is read later
+        unsigned long long braw = 0;
+        if (lane == 0) braw = atomicAdd(&a.ctrl->region_fill[region], (unsigned long long)cnt);
+        return braw;
     };
-    // write staged pair i of stage st at base + i (within the capacity) and
-    // count its sort digits
-    auto put = [&](const uint2* st, uint32_t i, unsigned long long base) {
-        if (base + i < cap) {
-            const uint2 pr = st[i];
-            rout[base + i] = pr;
-            if constexpr (MODE != 2) {
-                if (a.hist) {
+    auto count_digits = [&](const uint2* st, uint32_t cnt) {
+        if constexpr (MODE != 2) {
+            if (a.hist) {
+                for (uint32_t i = lane; i < cnt; i += 64u) {
+                    uint2 pr = st[i];
+                    if constexpr (MODE == 0) pr = make_uint2(min(pr.x, pr.y), max(pr.x, pr.y));
                     const unsigned long long k =
                         ((unsigned long long)pr.x << a.bbits) | (unsigned long long)pr.y;
                     for (int q = 0; q < a.hist_npass; ++q)
@@ -742,6 +745,31 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 }
             }
         }
+    };
+    auto store = [&](const uint2* st, uint32_t cnt, unsigned long long base) {
+        for (uint32_t i = lane; i < cnt; i += 64u) {
+            if (base + i < cap) {
+                uint2 pr = st[i];
+                if constexpr (MODE == 0) pr = make_uint2(min(pr.x, pr.y), max(pr.x, pr.y));
+                rout[base + i] = pr;
+            }
+        }
+    };
+    auto first_lane = [](unsigned long long v) {
+        return (unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)v) |
+               ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32);
+    };
+    // Write the staged pairs st[0, cnt) of this wave (cnt > 0, wave-uniform):
+    // claim cnt slots of the workgroup's region, count the pairs' sort digits
+    // in LDS while the claim is in flight, then store them (within the
+    // capacity).  Canonical mode stages (object, candidate) and orders the
+    // pair here, once per pair rather than once per tested chunk.  (After a
+    // capacity overflow the digit counts include unwritten pairs; the update
+    // then reports the overflow and its pairs are not used.)
+    auto write_out = [&](const uint2* st, uint32_t cnt) {
+        const unsigned long long braw = claim(cnt);
+        count_digits(st, cnt);
+        store(st, cnt, first_lane(braw));
     };
 
     if (g0 < hi) {
@@ -860,20 +888,19 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 if (GROUP) m &= ballot(a.group[oorig] != gc);
                 if (m == 0ull) return;
                 const uint32_t cnt = (uint32_t)__popcll(m) * mult;
-                npairs += cnt;
-                if constexpr (MODE != 2) {
+                if constexpr (MODE == 2) {
+                    npairs += cnt;
+                } else {
                     if (fill + cnt > kCap) {  // stage full: write it out
-                        const unsigned long long base = flush(fill);
-                        for (uint32_t i = lane; i < fill; i += 64u) put(stage, i, base);
+                        write_out(stage, fill);
+                        npairs += fill;
                         fill = 0;
                         __builtin_amdgcn_wave_barrier();
                     }
                     if (__builtin_amdgcn_inverse_ballot_w64(m)) {
                         const uint32_t r = fill + mult * mbcnt(m);
                         if (MODE == 0) {
-                            const uint32_t x = oorig < corig ? oorig : corig;
-                            const uint32_t y = oorig < corig ? corig : oorig;
-                            stage[r] = make_uint2(x, y);
+                            stage[r] = make_uint2(oorig, corig);
                         } else {
                             stage[r] = make_uint2(oorig, corig);
                             stage[r + 1] = make_uint2(corig, oorig);
@@ -920,6 +947,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 
     // Hand this wave's remainder and counters to the workgroup; the last wave
     // to arrive writes them all out with one atomic per counter.
+    if constexpr (MODE != 2) npairs += fill;  // the staged remainder
     if (lane == 0) {
         s_fill[w] = fill;
         s_np[w] = npairs;
@@ -945,11 +973,14 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     }
     if constexpr (MODE != 2) {
         if (tot) {
-            unsigned long long base = flush(tot);
+            const unsigned long long braw = claim(tot);
+#pragma unroll
+            for (int v = 0; v < kWaves; ++v) count_digits(s_stage[v], s_fill[v]);
+            unsigned long long base = first_lane(braw);
 #pragma unroll
             for (int v = 0; v < kWaves; ++v) {
                 const uint32_t f = s_fill[v];
-                for (uint32_t i = lane; i < f; i += 64u) put(s_stage[v], i, base);
+                store(s_stage[v], f, base);
                 base += f;
             }
         }
