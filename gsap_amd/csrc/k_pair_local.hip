@@ -502,7 +502,10 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(uint2* X, uint2* Y, ui
                         const uint32_t q = w0 + 64u * k + lane;
                         if (q < c0 + cs) {
                             const uint32_t g = s_wh[wave][(uint32_t)(kk[k] >> sh) & mask] + rk[k];
-                            if (!last || g < out_lim) dst[g] = pkey_pair(kk[k] + kb, bbits);
+                            // g lies in [s, e) when the window's pairs stay put
+                            // between the count and the scatter; the bound keeps
+                            // every store in the window whatever the data
+                            if (g < e && (!last || g < out_lim)) dst[g] = pkey_pair(kk[k] + kb, bbits);
                         }
                     }
                 }

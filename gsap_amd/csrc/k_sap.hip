@@ -720,13 +720,6 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     // whole grid contending for a single one.
     const uint32_t region = a.nregions > 1 ? blockIdx.x % a.nregions : 0u;
     uint2* const rout = a.out + (size_t)region * a.region_stride;
-    // Write the staged pairs st[0, cnt) of this wave (cnt > 0, wave-uniform):
-    // claim cnt slots of the workgroup's region, count the pairs' sort digits
-    // in LDS while the claim is in flight, then store them (within the
-    // capacity).  Canonical mode stages (object, candidate) and orders the
-    // pair here, once per pair rather than once per tested chunk.  (After a
-    // capacity overflow the digit counts include unwritten pairs; the update
-    // then reports the overflow and its pairs are not used.)
     auto claim = [&](uint32_t cnt) {  // issue the slot claim; the base is read later
         unsigned long long braw = 0;
         if (lane == 0) braw = atomicAdd(&a.ctrl->region_fill[region], (unsigned long long)cnt);
@@ -765,7 +758,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     // capacity).  Canonical mode stages (object, candidate) and orders the
     // pair here, once per pair rather than once per tested chunk.  (After a
     // capacity overflow the digit counts include unwritten pairs; the update
-    // then reports the overflow and its pairs are not used.)
+    // then reports the overflow and skips the pair sort.)
     auto write_out = [&](const uint2* st, uint32_t cnt) {
         const unsigned long long braw = claim(cnt);
         count_digits(st, cnt);
@@ -1000,13 +993,13 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 // ---------------------------------------------------------------- CSR entries
 // Row starts of the sorted directed pair list (reference pairEntries).
 __global__ void k_rowflag(const uint2* __restrict__ P, const unsigned long long* n_dev,
-                          unsigned long long cap, uint32_t* rowstart, uint32_t* has) {
+                          unsigned long long cap, uint32_t nobj, uint32_t* rowstart, uint32_t* has) {
     unsigned long long m = *n_dev;
-    if (m > cap) m = cap;
+    if (m > cap) m = 0;  // capacity overflow: the pair sort wrote nothing
     for (unsigned long long k = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; k < m;
          k += (unsigned long long)gridDim.x * blockDim.x) {
         const uint32_t x = P[k].x;
-        if (k == 0 || P[k - 1].x != x) {
+        if (x < nobj && (k == 0 || P[k - 1].x != x)) {
             rowstart[x] = (uint32_t)k;
             has[x] = 1u;
         }
@@ -1075,14 +1068,20 @@ __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
 __global__ void k_pair_regions(Ctrl* ctrl, unsigned long long region_cap, uint32_t nregions,
                                uint32_t tile) {
     if (threadIdx.x != 0) return;
-    // Together the regions hold at most region_cap pairs (the sort buffers'
-    // size); a region is rounded up to whole tiles, so a tile of the first
-    // sort pass lies in one region, and its slots past region_valid are
-    // padding that no pass reads.
+    // A region is rounded up to whole tiles, so a tile of the first sort pass
+    // lies in one region, and its slots past region_valid are padding that no
+    // pass reads.  After a capacity overflow (more pairs found than the
+    // regions hold, region_cap together) the sort is skipped: every region is
+    // empty.  The digit counts then include pairs that were not written, so a
+    // pass would place the written ones by counts that do not match them and
+    // leave an unsorted range, while the local stage's window search relies on
+    // sorted buckets.
+    unsigned long long found = 0;
+    for (uint32_t r = 0; r < nregions; ++r) found += ctrl->region_fill[r];
+    const bool skip = found > region_cap;
     unsigned long long run = 0, real = 0;
     for (uint32_t r = 0; r < nregions; ++r) {
-        unsigned long long f = ctrl->region_fill[r];
-        if (f > region_cap - real) f = region_cap - real;
+        const unsigned long long f = skip ? 0ull : ctrl->region_fill[r];
         const unsigned long long p = nregions > 1 ? (f + tile - 1) / tile * tile : f;
         ctrl->region_start[r] = run;
         ctrl->region_valid[r] = run + f;
@@ -1102,7 +1101,7 @@ __global__ void k_scene_offsets(const uint2* __restrict__ P, const Ctrl* ctrl, u
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nseg) return;
     unsigned long long m = ctrl->num_pairs;
-    if (m > cap) m = cap;
+    if (m > cap) m = 0;  // capacity overflow: the pair sort wrote nothing
     if (s == nseg) { out[s] = m; return; }
     const uint32_t key = seg_start[s];
     unsigned long long b = 0, e = m;
@@ -1168,7 +1167,7 @@ hipError_t launch_entries(const uint2* P, const unsigned long long* n_dev, unsig
                           int32_t* entries, Ctrl* ctrl, hipStream_t st) {
     unsigned long long g = (cap + 255) / 256;
     const uint32_t grid = (uint32_t)(g > 4096 ? 4096 : (g == 0 ? 1 : g));
-    hipLaunchKernelGGL(k_rowflag, dim3(grid), dim3(256), 0, st, P, n_dev, cap, rowstart, has);
+    hipLaunchKernelGGL(k_rowflag, dim3(grid), dim3(256), 0, st, P, n_dev, cap, nobj, rowstart, has);
     const uint32_t nb = (nobj + 2047) / 2048;
     if (nb == 0) return hipGetLastError();
     hipLaunchKernelGGL(k_entries_count, dim3(nb), dim3(kWG), 0, st, has, nobj, block_sums);

@@ -151,3 +151,43 @@ def test_local_two_pass_capacity_error_then_retry(dev, regions):
     big = torch.empty_like(ref)
     P, _, _ = s.collide_into(bd, big)
     assert P == ref.shape[0] and torch.equal(big, ref)
+
+
+def test_local_40bit_keys_big_windows(dev):
+    # 600,000 boxes: K = 40 (a per-region first pass, then one onesweep pass
+    # before the local stage).  Three boxes span the scene: their buckets hold
+    # ~600,000 pairs, far over the LDS sorts, so they take the global-memory
+    # path.  Reference: the whole-key sort.
+    import torch
+    b = W.scene(600_000, np.random.default_rng(12), "uniform").copy()
+    lo, hi = b[:, :3].min(0), b[:, 3:].max(0)
+    for i in (0, 1, 2):
+        b[i, :3] = lo - 1.0
+        b[i, 3:] = hi + 1.0
+    bd = to_dev(b, dev)
+    s0 = G.Sap()
+    s0.set_tune(pair_local=0)
+    ref, _, _ = s0.collide(bd)
+    for slabs in (0, 1):
+        s = G.Sap()
+        s.set_tune(pair_local=1, pair_regions=8, slabs=slabs)
+        got, _, _ = s.collide(bd)
+        assert torch.equal(got, ref), slabs
+
+
+def test_local_40bit_keys_capacity_error_then_retry(dev):
+    # an overflowing update skips the pair sort (its digit counts include the
+    # pairs that were not written); the retry is exact
+    import torch
+    b = W.aniso(700_000, seed=6, sigma=(20.0, 5.0, 1.0))
+    bd = to_dev(b, dev)
+    ref, _, _ = G.Sap().collide(bd)
+    s = G.Sap()
+    s.set_tune(pair_local=1, pair_regions=8)
+    small = torch.empty((ref.shape[0] // 2, 2), dtype=torch.int32, device=dev)
+    with pytest.raises(G.GsapError) as e:
+        s.collide_into(bd, small)
+    assert e.value.code == G.ERR_CAPACITY
+    big = torch.empty_like(ref)
+    P, _, _ = s.collide_into(bd, big)
+    assert P == ref.shape[0] and torch.equal(big, ref)

@@ -196,6 +196,29 @@ def test_capacity_error_then_retry(dev):
     assert P == len(ref) and np.array_equal(pairs_np(big), ref)
 
 
+def test_sym_csr_capacity_error_then_retry(dev):
+    # an overflowing SYM_CSR update writes no row starts (the pair sort is
+    # skipped, and the row-start pass reads no pairs); the retry is exact
+    import torch
+    rng = np.random.default_rng(9)
+    s = np.concatenate([rng.uniform(0, 30, (20000, 3)), rng.uniform(0.1, 0.7, (20000, 1))], 1)
+    s = s.astype(np.float32)
+    sym_ref, ent_ref = O.sym_csr(O.brute(s, kind=O.KIND_SPHERE))
+    sap = G.Sap(input_kind=G.INPUT_SPHERE, axis_mode=G.AXIS_COORD_MAXVAR, output_kind=G.OUT_SYM_CSR)
+    sd = to_dev(s, dev)
+    small = torch.full((len(sym_ref) // 2, 2), -1, dtype=torch.int32, device=dev)
+    E = torch.full((20000,), -7, dtype=torch.int32, device=dev)
+    with pytest.raises(G.GsapError) as e:
+        sap.collide_into(sd, small, E)
+    assert e.value.code == G.ERR_CAPACITY
+    assert bool((E == -7).all())
+    P = torch.empty((len(sym_ref), 2), dtype=torch.int32, device=dev)
+    npairs, nent, _ = sap.collide_into(sd, P, E)
+    assert npairs == len(sym_ref) and nent == len(ent_ref)
+    assert np.array_equal(P.cpu().numpy(), sym_ref)
+    assert np.array_equal(E[:nent].cpu().numpy(), ent_ref)
+
+
 def test_batched_equals_plain_per_scene(dev):
     import torch
     boxes, off = W.scenes(24, seed=7, lo=200, hi=5000)
