@@ -241,7 +241,7 @@ hipError_t sort_pairs(const SortArgs& a, uint2* p0, uint2* p1, uint2* out, hipSt
 // equal key >> L ("bucket") by the low L bits (8 <= L <= 24).  x holds the
 // bucket-sorted pairs (*n_dev of them), y is free scratch of the same size;
 // out receives positions < min(*n_dev, *last_n, last_cap).
-constexpr uint32_t kLocW = 2048;   // window: pairs whose bucket starts are one workgroup's
+constexpr uint32_t kLocW = 3072;   // window: pairs whose bucket starts are one workgroup's
 constexpr uint32_t kLocC = 8192;   // pairs sorted in LDS by one workgroup
 uint64_t pair_local_wstart_words(uint64_t cap);  // u32 words of the window-start table
 // big_ctr: a device word the launch zeroes and counts the handed-on windows in.
