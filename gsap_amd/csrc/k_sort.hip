@@ -256,9 +256,14 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
 // occupies [start(d) + sum of d's counts in regions < r, + its count in r),
 // from the per-region histograms the sweep counted as it wrote the pairs;
 // regions are the sweep's per-XCD allocation units, so each counter is
-// contended only by one region's tiles.  Tiles are dealt grid-stride; items
-// are staged in LDS in digit order (LDS-atomic ranks) for a coalesced
-// scatter, never past the pair count.
+// contended only by one region's tiles.  Region r's tiles go to the
+// workgroups of XCD r (workgroup w runs on XCD w mod 8, so those with
+// w mod nregions == r; the grid is a multiple of nregions): a digit's runs of
+// one region then lie side by side in one L2, which merges the 128-byte lines
+// that two neighbouring runs share before writing them back.  Lines written
+// in parts from two L2s cost HBM about twice their bytes.  Items are staged
+// in LDS in digit order (LDS-atomic ranks) for a coalesced scatter, never
+// past the pair count.
 template <int BITS, int ITEMS>
 __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, uint2* __restrict__ kout,
                                                 SortArgs a) {
@@ -290,12 +295,16 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
             if ((uint32_t)r < a.nregions) run += a.hist_reg[r * RADIX + tid];
         }
     }
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // this workgroup's region and its tiles (regions are tile-aligned)
+    const uint32_t r = blockIdx.x % a.nregions;
+    const uint32_t rt0 = (uint32_t)(a.region_start[r] / TILE);
+    const uint32_t rt1 = min((uint32_t)((a.region_start[r + 1] + TILE - 1) / TILE), ntiles);
+    const uint2* src = kin + ((unsigned long long)r * a.region_stride - a.region_start[r]);
+    const unsigned long long ve = a.region_valid[r];
+    const uint32_t gstep = gridDim.x / a.nregions;
+    for (uint32_t tile = rt0 + blockIdx.x / a.nregions; tile < rt1; tile += gstep) {
         const uint32_t lo = tile * TILE;
         uint32_t hi = min(lo + TILE, n);
-        const uint32_t r = region_of(a.region_start, a.nregions, lo);
-        const uint2* src = kin + ((unsigned long long)r * a.region_stride - a.region_start[r]);
-        const unsigned long long ve = a.region_valid[r];
         if (ve < hi) hi = (uint32_t)ve;
         if (lo >= hi) continue;  // a tile of padding (uniform)
         s_cnt[tid] = 0u;
@@ -458,7 +467,9 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
         uint32_t* vo = cur ? v0 : v1;
         if constexpr (!KT::kHasVals && BITS == 8) {
             if (p == 0 && a.claim && a.npass > 1) {  // never the final pass: no capacity clamp
-                hipLaunchKernelGGL((k_msplit<BITS, ITEMS>), dim3(grid), dim3(kWG), 0, st, ki, ko, a);
+                // a whole number of workgroups per region
+                const uint32_t mg = (grid + a.nregions - 1) / a.nregions * a.nregions;
+                hipLaunchKernelGGL((k_msplit<BITS, ITEMS>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
