@@ -239,7 +239,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     }
     // pair digit histograms, then the first pass's per-region counts and slot counters
     if (!h->hist_pair.p)
-        HIPCHK(h, grow(h->hist_pair, (kMaxPasses * 256 + 2 * kMaxRegions * 256) * 4));
+        HIPCHK(h, grow(h->hist_pair, (kMaxPasses * 256 + kPairClaimWords) * 4));
     if (slab_mode(h) && !h->slabp.p) {
         HIPCHK(h, grow(h->axis2, 4 * sizeof(float)));
         HIPCHK(h, grow(h->slabp, kSlabParams * sizeof(float)));
@@ -442,6 +442,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.hist = a.hist_pair;
         p.hist_reg = msplit ? a.pair_claim : nullptr;
         p.claim = msplit ? a.pair_claim + kMaxRegions * 256 : nullptr;
+        // two passes: the second one runs in parts, one per XCD, from the
+        // per-part digit counts the first one takes
+        p.joint = msplit && npass_pair == 2 ? a.pair_claim + 2 * kMaxRegions * 256 : nullptr;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;

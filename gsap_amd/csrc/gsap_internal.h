@@ -99,11 +99,13 @@ struct ObjPipeArgs {
     float* vr_partials;     // [ntiles][kVr] per-tile ranges for the slab parameters
     float* slab;            // [kSlabParams] slab and key-quantisation parameters
     uint32_t* slab_cnt;     // [256] entries per slab (zeroed by k_moments, counted by k_project)
-    // [2][kMaxRegions][256] or null: the pair sort's first-pass digit counts
-    // per pair region (counted by the sweep) and that pass's slot counters;
+    // [3][kMaxRegions][256] or null: the pair sort's first-pass digit counts
+    // per pair region (counted by the sweep), that pass's slot counters, and
+    // the second-pass digit counts per part (counted by the first pass);
     // zeroed by k_moments
     uint32_t* pair_claim;
 };
+constexpr uint32_t kPairClaimWords = 3u * kMaxRegions * 256u;
 constexpr int kVr = 5;           // max v extent, min lo_v, max hi_v, min lo_u, max hi_u
 constexpr int kSlabParams = 8;   // v0, 1/W, S, u0, 1/du, W, -, -
 constexpr uint32_t kSlabNull = 0xFFFFFFFFu;  // key of an unused second entry
@@ -226,6 +228,10 @@ struct SortArgs {
     // counters of the same shape (zero), or null for a onesweep first pass
     const uint32_t* hist_reg;
     uint32_t* claim;
+    // pair sort of two passes after a per-region first pass: [kMaxRegions][256]
+    // counts of the second digit per part (part = first digit >> 5), taken
+    // by the first pass (zero on entry); the second pass then runs per part
+    uint32_t* joint;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.

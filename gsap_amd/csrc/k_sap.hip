@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
             if (a.slabs) a.slab_cnt[tid] = 0u;  // kWG == 256 slabs
             if (tid < 2 * kMaxPasses) a.ctrl->tile_ctr[tid] = 0u;
             if (a.pair_claim)
-                for (uint32_t i = tid; i < 2u * kMaxRegions * 256u; i += kWG) a.pair_claim[i] = 0u;
+                for (uint32_t i = tid; i < kPairClaimWords; i += kWG) a.pair_claim[i] = 0u;
             if (tid == 0) a.ctrl->epoch += 1u;
         }
     }

@@ -274,6 +274,12 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
     __shared__ uint32_t s_gbase[RADIX];
     __shared__ uint32_t s_tmp[kWaves];
     __shared__ uint2 s_k[TILE];
+    // with a.joint: the second digit's counts per part (part = digit >> 5),
+    // two 16-bit counters per word, added to a.joint every kJointFlush tiles
+    // (at most kJointFlush * TILE < 2^16 counts in between; 4 KB, so that
+    // four workgroups still fit a CU)
+    constexpr uint32_t kJointFlush = 65535u / TILE;
+    __shared__ uint32_t s_j[kMaxRegions * RADIX / 2];
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0;
@@ -281,6 +287,21 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
     const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
     unsigned long long nr = *a.n_dev_rest;
     const uint32_t nreal = (uint32_t)(nr > a.cap ? a.cap : nr);
+    uint32_t* const joint = a.joint;
+    for (uint32_t i = tid; i < kMaxRegions * RADIX / 2; i += kWG) s_j[i] = 0u;
+    auto flush_joint = [&]() {  // uniform; s_j zero again afterwards
+        __syncthreads();
+        for (uint32_t i = tid; i < kMaxRegions * RADIX / 2; i += kWG) {
+            const uint32_t w = s_j[i];
+            if (w) {
+                if (w & 0xFFFFu) atomicAdd(&joint[2 * i], w & 0xFFFFu);
+                if (w >> 16) atomicAdd(&joint[2 * i + 1], w >> 16);
+                s_j[i] = 0u;
+            }
+        }
+        __syncthreads();
+    };
+    uint32_t since = 0;  // tiles counted into s_j since the last flush
     const uint32_t ntiles = (n + TILE - 1) / TILE;
     const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
     // digit tid's start, and its counts in the regions before each region
@@ -353,6 +374,218 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
             const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
             const uint32_t gp = s_gbase[d] + (i - s_tstart[d]);
             if (gp < nreal) kout[gp] = kk;
+            if (joint) {
+                const uint32_t j = ((d >> 5) << BITS) | KPair::digit(kk, shift + BITS, a.bbits, MASK);
+                atomicAdd(&s_j[j >> 1], 1u << ((j & 1u) << 4));
+            }
+        }
+        __syncthreads();
+        if (joint && ++since == kJointFlush) {
+            flush_joint();
+            since = 0;
+        }
+    }
+    if (joint && since) flush_joint();
+}
+
+// Second (last) onesweep pass of a two-pass pair sort after k_msplit, in
+// kMaxRegions parts: part x holds the keys whose first digit d1 has
+// d1 >> 5 == x, a contiguous range of the first pass's output (which is
+// ordered by d1).  Digit d's output range is laid out part after part: part
+// x's keys of digit d start at start(d) + sum over x' < x of J[x'][d] (J: the
+// per-part digit counts k_msplit took), so the result is ordered by (d, d1)
+// as after one stable pass, while each part runs its own look-back chain on
+// the workgroups of one XCD (w mod kMaxRegions == x, the grid a multiple of
+// kMaxRegions).  A digit's neighbouring runs then lie in one L2, which merges
+// the lines they share before writing them back.  Within a part: tiles
+// claimed through the part's counter, stable ballot ranking, look-back to
+// the part's first tile (inclusive), LDS-staged coalesced scatter.
+template <int ITEMS>
+__global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, uint2* __restrict__ kout,
+                                               SortArgs a) {
+    constexpr uint32_t BITS = 8, RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    constexpr uint32_t NP = kMaxRegions;
+    static_assert(RADIX == kWG, "one digit per thread");
+    __shared__ uint32_t s_wh[kWaves][RADIX];
+    __shared__ uint32_t s_tstart[RADIX];
+    __shared__ uint32_t s_gbase[RADIX];
+    __shared__ uint32_t s_tmp[2 * kWaves];
+    __shared__ uint32_t s_cnt[RADIX];
+    __shared__ uint2 s_k[TILE];
+    __shared__ uint32_t s_q[NP + 1];   // part starts in the input
+    __shared__ uint32_t s_t0[NP];      // each part's first look-back slot
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const int shift = a.shift0 + (int)BITS;
+    unsigned long long nd = *a.n_dev_rest;
+    const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
+    const uint32_t x = blockIdx.x % NP;
+    const uint32_t ctr_idx = (uint32_t)a.ctr_base + x;
+    const uint32_t tag = ((a.ctrl->epoch << 4) | ((uint32_t)a.ctr_base + 1u)) & 0x7FFFFFFFu;
+    const unsigned long long ltm = lanemask_lt();
+    const unsigned long long gtm = ~ltm & ~(1ull << lane);
+    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
+    // part starts: the first pass's digit starts at d1 = 32 x
+    uint32_t tot;
+    const uint32_t d1start = block_excl_scan(a.hist[tid], s_tmp, &tot);
+    if ((tid & 31u) == 0u) s_q[tid >> 5] = min(d1start, n);
+    if (tid == 0) s_q[NP] = min(tot, n);
+    // digit tid's base for this part: its start over all parts plus its
+    // counts in the parts before this one
+    uint32_t all = 0, pre = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < NP; ++q) {
+        const uint32_t c = a.joint[q * RADIX + tid];
+        all += c;
+        pre += q < x ? c : 0u;
+    }
+    const uint32_t dbase = block_excl_scan(all, s_tmp, &tot) + pre;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+        for (uint32_t q = 0; q < NP; ++q) {
+            s_t0[q] = t;
+            t += (s_q[q + 1] - s_q[q] + TILE - 1) / TILE;
+        }
+    }
+    s_cnt[tid] = 0u;
+    const uint32_t plo = s_q[x], phi = s_q[x + 1];
+
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_ctr[ctr_idx], 1u);
+        for (uint32_t i = tid; i < kWaves * RADIX; i += kWG) (&s_wh[0][0])[i] = 0u;
+        __syncthreads();
+        const uint32_t ti = s_tile;
+        const unsigned long long lo64 = (unsigned long long)plo + (unsigned long long)ti * TILE;
+        if (lo64 >= phi) break;  // uniform
+        const uint32_t lo = (uint32_t)lo64, hi = min(lo + TILE, phi);
+        const bool first = ti == 0u;
+        const uint32_t gtile = s_t0[x] + ti;
+
+        uint2 key[ITEMS];
+        uint32_t rank[ITEMS], dig[ITEMS];
+        const uint32_t base = lo + wave * 64u * ITEMS + lane;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t idx = base + 64u * k;
+            if (idx < hi) key[k] = kin[idx];
+        }
+        // tile digit counts, published at once; a run of equal digits in
+        // neighbouring lanes is one LDS atomic
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const bool valid = (base + 64u * k) < hi;
+            const uint32_t d = valid ? KPair::digit(key[k], shift, a.bbits, MASK) : 0u;
+            dig[k] = d;
+            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+            const bool head = valid && (lane == 0u || prev != d);
+            const unsigned long long hm = ballot(head);
+            const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+            if (head) {
+                const unsigned long long nxt = hm & above;
+                const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+                atomicAdd(&s_cnt[d], end - lane);
+            }
+        }
+        __syncthreads();
+        const uint32_t tcount = s_cnt[tid];
+        s_cnt[tid] = 0u;
+        st_rel(a.lookback + (size_t)gtile * RADIX + tid, lb_word(tag, first, tcount));
+        // The order within a digit need only follow the first digit d1 (keys
+        // of equal d1 and d are ordered by the local stage).  The input is
+        // ordered by d1, so a tile whose first and last keys share d1 holds
+        // one d1 value: its items take their places in any order (one LDS
+        // atomic each); a tile across a d1 boundary ranks them stably
+        // (items in tile order (wave, item, lane), peers by BITS ballots).
+        const bool one_d1 = KPair::digit(kin[lo], a.shift0, a.bbits, MASK) ==
+                            KPair::digit(kin[hi - 1], a.shift0, a.bbits, MASK);  // uniform
+        __syncthreads();  // s_cnt is zero
+        if (one_d1) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k)
+                rank[k] = (base + 64u * k) < hi ? atomicAdd(&s_cnt[dig[k]], 1u) : 0u;
+        } else {
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const bool valid = (base + 64u * k) < hi;
+                const uint32_t d = dig[k];
+                unsigned long long m = ballot(valid);
+#pragma unroll
+                for (uint32_t b = 0; b < BITS; ++b) {
+                    const bool bit = (d >> b) & 1u;
+                    const unsigned long long bb = ballot(bit);
+                    m &= bit ? bb : ~bb;
+                }
+                const uint32_t before = s_wh[wave][d];
+                rank[k] = before + mbcnt(m);
+                if (valid && (m & gtm) == 0ull) s_wh[wave][d] = before + (uint32_t)__popcll(m);
+            }
+        }
+        __syncthreads();
+        if (one_d1) s_cnt[tid] = 0u;  // the atomic ranks leave the tile counts
+        uint32_t excl = 0;
+        {
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = s_wh[w][tid];
+                s_wh[w][tid] = run;
+                run += c;
+            }
+            if (!first) {
+                // predecessors four at a time, down to the nearest inclusive
+                // prefix (the part's first tile is one)
+                constexpr int WIN = 4;
+                int p = (int)gtile - 1;
+                uint32_t spins = 0;
+                for (;;) {
+                    unsigned long long v[WIN];
+#pragma unroll
+                    for (int k = 0; k < WIN; ++k)
+                        v[k] = p - k >= 0 ? ld_acq(a.lookback + (size_t)(p - k) * RADIX + tid) : 0ull;
+                    uint32_t add = 0;
+                    int used = 0;
+                    bool done = false;
+#pragma unroll
+                    for (int k = 0; k < WIN; ++k) {
+                        if (done || used != k || (uint32_t)(v[k] >> 33) != tag) continue;
+                        add += (uint32_t)v[k];
+                        used = k + 1;
+                        done = (v[k] >> 32) & 1ull;
+                    }
+                    excl += add;
+                    if (done) break;
+                    if (used == 0) {
+                        if (++spins > (1u << 22)) {  // bounded wait (see k_onesweep)
+                            atomicOr(&a.ctrl->flags, kFlagTimeout);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    p -= used;
+                }
+                st_rel(a.lookback + (size_t)gtile * RADIX + tid, lb_word(tag, true, excl + tcount));
+            }
+        }
+        const uint32_t tex = block_excl_scan(tcount, s_tmp + kWaves, &tot);
+        s_gbase[tid] = dbase + excl;
+        s_tstart[tid] = tex;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            if ((base + 64u * k) < hi) {
+                const uint32_t d = dig[k];
+                s_k[s_tstart[d] + s_wh[wave][d] + rank[k]] = key[k];
+            }
+        }
+        __syncthreads();
+        const uint32_t tn = hi - lo;
+        for (uint32_t i = tid; i < tn; i += kWG) {
+            const uint2 kk = s_k[i];
+            const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
+            const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
+            if (g < n) kout[g] = kk;
         }
         __syncthreads();
     }
@@ -470,6 +703,17 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
                 // a whole number of workgroups per region
                 const uint32_t mg = (grid + a.nregions - 1) / a.nregions * a.nregions;
                 hipLaunchKernelGGL((k_msplit<BITS, ITEMS>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+                if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
+                cur ^= 1;
+                continue;
+            }
+        }
+        if constexpr (!KT::kHasVals && BITS == 8) {
+            if (p == 1 && a.joint && a.npass == 2) {  // the last pass, per part
+                const uint32_t mg = (grid + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
+                hipLaunchKernelGGL((k_xpass<ITEMS>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
