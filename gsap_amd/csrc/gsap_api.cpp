@@ -417,6 +417,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     // (no look-back); the sweep then also counts that digit per region
     const bool msplit = sweep_hist && npass_pair > 1 && h->pair_regions == kMaxRegions;
     w.hist_reg = msplit ? a.pair_claim : nullptr;
+    // two passes: the second one runs in parts (one per XCD), from the
+    // per-part digit counts the sweep takes in place of its histogram
+    uint32_t* const joint = msplit && npass_pair == 2 ? a.pair_claim + 2 * kMaxRegions * 256 : nullptr;
+    w.joint = joint;
     w.hist_npass = (int)npass_pair;
     w.hist_shift0 = (int)shift0;
     w.bbits = (int)bbits;
@@ -442,9 +446,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.hist = a.hist_pair;
         p.hist_reg = msplit ? a.pair_claim : nullptr;
         p.claim = msplit ? a.pair_claim + kMaxRegions * 256 : nullptr;
-        // two passes: the second one runs in parts, one per XCD, from the
-        // per-part digit counts the first one takes
-        p.joint = msplit && npass_pair == 2 ? a.pair_claim + 2 * kMaxRegions * 256 : nullptr;
+        p.joint = joint;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;

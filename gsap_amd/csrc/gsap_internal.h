@@ -101,8 +101,8 @@ struct ObjPipeArgs {
     uint32_t* slab_cnt;     // [256] entries per slab (zeroed by k_moments, counted by k_project)
     // [3][kMaxRegions][256] or null: the pair sort's first-pass digit counts
     // per pair region (counted by the sweep), that pass's slot counters, and
-    // the second-pass digit counts per part (counted by the first pass);
-    // zeroed by k_moments
+    // the second-pass digit counts per part (counted by the sweep); zeroed
+    // by k_moments
     uint32_t* pair_claim;
 };
 constexpr uint32_t kPairClaimWords = 3u * kMaxRegions * 256u;
@@ -144,6 +144,9 @@ struct SweepArgs {
     int hist_shift0;
     int bbits;
     uint32_t* hist_reg;      // if set: the first digit's counts per region, [kMaxRegions][256]
+    // if set (with hist, two digits): the second digit's counts per part
+    // (part = first digit >> 5), [kMaxRegions][256], in place of its histogram
+    uint32_t* joint;
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 
@@ -230,7 +233,7 @@ struct SortArgs {
     uint32_t* claim;
     // pair sort of two passes after a per-region first pass: [kMaxRegions][256]
     // counts of the second digit per part (part = first digit >> 5), taken
-    // by the first pass (zero on entry); the second pass then runs per part
+    // by the sweep; the second pass then runs per part
     uint32_t* joint;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which

@@ -669,7 +669,7 @@ constexpr uint32_t kStage = kSweepStage;
 // entries to be its box's first slab (the pair's slab: that of the larger
 // lower bound), so each pair is found exactly once.
 template <int KIND, bool GROUP, int MODE, bool SLAB>  // MODE 0 canonical, 1 symmetric, 2 count
-__global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_sweep(SweepArgs a) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
     // The object records are read-only here; the constant address space lets
     // their wave-uniform loads go through the scalar cache.
@@ -678,13 +678,19 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 #else
     typedef const Rec CRec;
 #endif
-    // the slab sweep emits denser hits, so it gets the larger stage (8
-    // workgroups of 4 waves still fit a CU's LDS either way)
+    // the slab sweep emits denser hits, so it gets the larger stage (seven
+    // workgroups of 4 waves, as many as the registers allow, still fit a
+    // CU's LDS either way)
     constexpr uint32_t kCap = SLAB ? kSweepStageSlab : kStage;
     constexpr uint32_t kSt = MODE == 2 ? 1u : kCap;
     __shared__ uint2 s_stage[kWaves][kSt];
-    // digit histograms of the pairs written, for the pair sort's first passes
-    __shared__ uint32_t s_ph[MODE == 2 ? 1 : kSweepHistPasses * 256];
+    // digit histograms of the pairs written, for the pair sort's first
+    // passes; with a.joint: the first digit's histogram, then the per-part
+    // second-digit counts as 16-bit halves of kJointWords words
+    constexpr uint32_t kJointWords = kMaxRegions * 256u / 2u;
+    constexpr uint32_t kPhWords = 256u + kJointWords;
+    static_assert(kPhWords >= kSweepHistPasses * 256u, "histogram words");
+    __shared__ uint32_t s_ph[MODE == 2 ? 1 : kPhWords];
     __shared__ uint32_t s_fill[kWaves];
     __shared__ uint32_t s_np[kWaves];
     __shared__ unsigned long long s_ex[kWaves];
@@ -700,7 +706,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     if (threadIdx.x == 0) s_done = 0u;
     if constexpr (MODE != 2) {
         if (a.hist)
-            for (uint32_t i = threadIdx.x; i < kSweepHistPasses * 256u; i += kWG) s_ph[i] = 0u;
+            for (uint32_t i = threadIdx.x; i < kPhWords; i += kWG) s_ph[i] = 0u;
     }
     __syncthreads();  // the only block-level barrier
     const uint32_t hi = min(lo + kSweepTile, ti.hi);
@@ -725,6 +731,22 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         if (lane == 0) braw = atomicAdd(&a.ctrl->region_fill[region], (unsigned long long)cnt);
         return braw;
     };
+    // Per-part counts: each wave adds at most kJointWave + kCap to a 16-bit
+    // half before it flushes them all (atomic exchanges, so the other waves'
+    // concurrent adds are kept), so no half exceeds kWaves * (kJointWave +
+    // kCap) < 2^16 between flushes.
+    constexpr uint32_t kJointWave = 15000u;
+    static_assert(kWaves * (kJointWave + kSweepStageSlab) < 65536u, "16-bit joint counts");
+    uint32_t jcount = 0;  // pairs this wave counted since it last flushed them
+    auto flush_joint = [&]() {
+        if constexpr (MODE != 2) {
+            for (uint32_t i = lane; i < kJointWords; i += 64u) {
+                const uint32_t v = atomicExch(&s_ph[256u + i], 0u);
+                if (v & 0xFFFFu) atomicAdd(&a.joint[2u * i], v & 0xFFFFu);
+                if (v >> 16) atomicAdd(&a.joint[2u * i + 1u], v >> 16);
+            }
+        }
+    };
     auto count_digits = [&](const uint2* st, uint32_t cnt) {
         if constexpr (MODE != 2) {
             if (a.hist) {
@@ -733,8 +755,22 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     if constexpr (MODE == 0) pr = make_uint2(min(pr.x, pr.y), max(pr.x, pr.y));
                     const unsigned long long k =
                         ((unsigned long long)pr.x << a.bbits) | (unsigned long long)pr.y;
-                    for (int q = 0; q < a.hist_npass; ++q)
-                        atomicAdd(&s_ph[q * 256 + (uint32_t)((k >> (a.hist_shift0 + 8 * q)) & 255u)], 1u);
+                    if (a.joint) {
+                        const uint32_t d1 = (uint32_t)(k >> a.hist_shift0) & 255u;
+                        const uint32_t j = ((d1 >> 5) << 8) | ((uint32_t)(k >> (a.hist_shift0 + 8)) & 255u);
+                        atomicAdd(&s_ph[d1], 1u);
+                        atomicAdd(&s_ph[256u + (j >> 1)], 1u << ((j & 1u) << 4));
+                    } else {
+                        for (int q = 0; q < a.hist_npass; ++q)
+                            atomicAdd(&s_ph[q * 256 + (uint32_t)((k >> (a.hist_shift0 + 8 * q)) & 255u)], 1u);
+                    }
+                }
+                if (a.joint) {
+                    jcount += cnt;
+                    if (jcount > kJointWave) {
+                        flush_joint();
+                        jcount = 0;
+                    }
                 }
             }
         }
@@ -978,7 +1014,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             }
         }
         if (a.hist) {  // every wave's digits are counted (they arrived after counting)
-            for (uint32_t i = lane; i < (uint32_t)a.hist_npass * 256u; i += 64u) {
+            const uint32_t nh = a.joint ? 256u : (uint32_t)a.hist_npass * 256u;
+            for (uint32_t i = lane; i < nh; i += 64u) {
                 const uint32_t c = s_ph[i];
                 if (c) {
                     atomicAdd(&a.hist[i], c);
@@ -986,6 +1023,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     if (a.hist_reg && i < 256u) atomicAdd(&a.hist_reg[region * 256u + i], c);
                 }
             }
+            if (a.joint) flush_joint();
         }
     }
 }

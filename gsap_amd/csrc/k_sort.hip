@@ -274,12 +274,6 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
     __shared__ uint32_t s_gbase[RADIX];
     __shared__ uint32_t s_tmp[kWaves];
     __shared__ uint2 s_k[TILE];
-    // with a.joint: the second digit's counts per part (part = digit >> 5),
-    // two 16-bit counters per word, added to a.joint every kJointFlush tiles
-    // (at most kJointFlush * TILE < 2^16 counts in between; 4 KB, so that
-    // four workgroups still fit a CU)
-    constexpr uint32_t kJointFlush = 65535u / TILE;
-    __shared__ uint32_t s_j[kMaxRegions * RADIX / 2];
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0;
@@ -287,21 +281,6 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
     const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
     unsigned long long nr = *a.n_dev_rest;
     const uint32_t nreal = (uint32_t)(nr > a.cap ? a.cap : nr);
-    uint32_t* const joint = a.joint;
-    for (uint32_t i = tid; i < kMaxRegions * RADIX / 2; i += kWG) s_j[i] = 0u;
-    auto flush_joint = [&]() {  // uniform; s_j zero again afterwards
-        __syncthreads();
-        for (uint32_t i = tid; i < kMaxRegions * RADIX / 2; i += kWG) {
-            const uint32_t w = s_j[i];
-            if (w) {
-                if (w & 0xFFFFu) atomicAdd(&joint[2 * i], w & 0xFFFFu);
-                if (w >> 16) atomicAdd(&joint[2 * i + 1], w >> 16);
-                s_j[i] = 0u;
-            }
-        }
-        __syncthreads();
-    };
-    uint32_t since = 0;  // tiles counted into s_j since the last flush
     const uint32_t ntiles = (n + TILE - 1) / TILE;
     const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
     // digit tid's start, and its counts in the regions before each region
@@ -374,18 +353,9 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
             const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
             const uint32_t gp = s_gbase[d] + (i - s_tstart[d]);
             if (gp < nreal) kout[gp] = kk;
-            if (joint) {
-                const uint32_t j = ((d >> 5) << BITS) | KPair::digit(kk, shift + BITS, a.bbits, MASK);
-                atomicAdd(&s_j[j >> 1], 1u << ((j & 1u) << 4));
-            }
         }
         __syncthreads();
-        if (joint && ++since == kJointFlush) {
-            flush_joint();
-            since = 0;
-        }
     }
-    if (joint && since) flush_joint();
 }
 
 // Second (last) onesweep pass of a two-pass pair sort after k_msplit, in
@@ -393,7 +363,7 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
 // d1 >> 5 == x, a contiguous range of the first pass's output (which is
 // ordered by d1).  Digit d's output range is laid out part after part: part
 // x's keys of digit d start at start(d) + sum over x' < x of J[x'][d] (J: the
-// per-part digit counts k_msplit took), so the result is ordered by (d, d1)
+// per-part digit counts the sweep took as it wrote the pairs), so the result is ordered by (d, d1)
 // as after one stable pass, while each part runs its own look-back chain on
 // the workgroups of one XCD (w mod kMaxRegions == x, the grid a multiple of
 // kMaxRegions).  A digit's neighbouring runs then lie in one L2, which merges
