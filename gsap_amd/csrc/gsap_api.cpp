@@ -91,6 +91,7 @@ struct gsap_sap {
     uint64_t region_stride = 0;  // slots between region starts
     uint64_t sort_cap = 0;       // capacity of the second pair buffer (padded total)
     bool timing = false;
+    uint32_t cus = 256;  // compute units of the device the handle was created on
     static constexpr int kEv = 20;
     hipEvent_t ev[kEv] = {};
     int nev = 0;
@@ -163,6 +164,12 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_counts, 8 * sizeof(unsigned long long), 0);
     for (int i = 0; i < gsap_sap::kEv && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
     if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) {
+        int dev = 0, cus = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess && cus > 0) h->cus = (uint32_t)cus;
+    }
     if (e != hipSuccess) {
         std::string msg = std::string("gsap_sap_create: ") + hipGetErrorString(e);
         gsap_sap_destroy(h);
@@ -447,6 +454,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.hist_reg = msplit ? a.pair_claim : nullptr;
         p.claim = msplit ? a.pair_claim + kMaxRegions * 256 : nullptr;
         p.joint = joint;
+        // the per-part pass runs best at three workgroups per CU (more tiles
+        // in flight per XCD evict the lines neighbouring runs share)
+        p.xgrid = 3 * h->cus;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;

@@ -43,23 +43,35 @@ __device__ __forceinline__ T wave_sum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
+// Inclusive scan of the wave's 64 lanes under an associative op with
+// identity id, by DPP moves (VALU only; no LDS crossbar round trips):
+// row_shr 1..3 of the inputs, row_shr 4 and 8 of the partial sums (upper
+// banks), then row_bcast:15 and row_bcast:31 carry the row totals into the
+// rows above.  Lanes a move does not write (and sources outside the row)
+// take the identity.  All 64 lanes must be active.
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t id, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)src, CTRL, ROWS, BANKS, false);
+}
+template <class Op>
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t x, uint32_t id, Op op) {
+    uint32_t v = op(x, dpp_mov<0x111, 0xF, 0xF>(id, x));  // row_shr:1
+    v = op(v, dpp_mov<0x112, 0xF, 0xF>(id, x));           // row_shr:2
+    v = op(v, dpp_mov<0x113, 0xF, 0xF>(id, x));           // row_shr:3
+    v = op(v, dpp_mov<0x114, 0xF, 0xE>(id, v));           // row_shr:4, banks 1-3
+    v = op(v, dpp_mov<0x118, 0xF, 0xC>(id, v));           // row_shr:8, banks 2-3
+    v = op(v, dpp_mov<0x142, 0xA, 0xF>(id, v));           // row_bcast:15 into rows 1, 3
+    v = op(v, dpp_mov<0x143, 0xC, 0xF>(id, v));           // row_bcast:31 into rows 2, 3
     return v;
 }
-// inclusive scan within the wave
+// the wave's maximum, in every lane (all lanes active)
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    const uint32_t m = wave_scan_dpp(v, 0u, [](uint32_t p, uint32_t q) { return p > q ? p : q; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
+}
+// inclusive scan within the wave (all lanes active)
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    uint32_t l = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t w = __shfl_up(v, o, 64);
-        if (l >= (uint32_t)o) v += w;
-    }
-    return v;
+    return wave_scan_dpp(v, 0u, [](uint32_t p, uint32_t q) { return p + q; });
 }
 
 // Exclusive scan over a 256-thread workgroup; s_tmp needs kWaves words.

@@ -233,8 +233,9 @@ struct SortArgs {
     uint32_t* claim;
     // pair sort of two passes after a per-region first pass: [kMaxRegions][256]
     // counts of the second digit per part (part = first digit >> 5), taken
-    // by the sweep; the second pass then runs per part
+    // by the sweep; the second pass then runs per part, on xgrid workgroups
     uint32_t* joint;
+    uint32_t xgrid;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.

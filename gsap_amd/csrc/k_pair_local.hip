@@ -38,15 +38,14 @@ constexpr uint32_t kLocTail = 64;      // LDS words of the key array used for re
 constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
 constexpr uint32_t kGlobChunk = kLT * kGlobItems;
 
+// the wave's OR / AND, in every lane (all lanes active)
 __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
-    return v;
+    const uint32_t m = wave_scan_dpp(v, 0u, [](uint32_t p, uint32_t q) { return p | q; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
 }
 __device__ __forceinline__ uint32_t wave_and_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v &= (uint32_t)__shfl_xor((int)v, o, 64);
-    return v;
+    const uint32_t m = wave_scan_dpp(v, ~0u, [](uint32_t p, uint32_t q) { return p & q; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
 }
 
 // Exclusive scan over the LT threads of a workgroup (s_tmp: LT / 64 words).

@@ -682,7 +682,8 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
         }
         if constexpr (!KT::kHasVals && BITS == 8) {
             if (p == 1 && a.joint && a.npass == 2) {  // the last pass, per part
-                const uint32_t mg = (grid + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
+                const uint32_t xg = a.xgrid ? a.xgrid : grid;
+                const uint32_t mg = (xg + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
                 hipLaunchKernelGGL((k_xpass<ITEMS>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
