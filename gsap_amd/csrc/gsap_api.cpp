@@ -489,7 +489,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
             HIPCHK(h, launch_scene_offsets(outp, ctrl, cap, seg.seg_start, seg.nseg, scene_out, st));
     }
     mark(h, st);
-    HIPCHK(h, launch_finish(ctrl, count_only ? ~0ull : cap, d_counts, sym, st));
+    // the synchronous entry points read the counts and the axis back in one copy
+    unsigned long long* const own = (unsigned long long*)h->counts.p;
+    HIPCHK(h, launch_finish(ctrl, count_only ? ~0ull : cap, d_counts, sym, (const float*)h->axis.p,
+                            d_counts == own ? own + 5 : nullptr, st));
     mark(h, st);
     return GSAP_OK;
 }
@@ -537,15 +540,15 @@ extern "C" int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void*
 static int32_t read_counts(gsap_sap* h, hipStream_t st, uint64_t cap, bool count_only,
                            uint64_t* h_num_pairs, uint32_t* h_num_entries, gsap_stats* stats,
                            uint32_t n) {
-    HIPCHK(h, hipMemcpyAsync(h->h_counts, h->counts.p, 5 * sizeof(unsigned long long),
+    // counts [0, 5) and the axis words [5, 7) (written by the finish kernel)
+    HIPCHK(h, hipMemcpyAsync(h->h_counts, h->counts.p, 7 * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost, st));
-    if (stats) HIPCHK(h, hipMemcpyAsync(stats->axis, h->axis.p, 3 * sizeof(float),
-                                        hipMemcpyDeviceToHost, st));
-    float src = 0.0f;
-    if (stats) HIPCHK(h, hipMemcpyAsync(&src, (float*)h->axis.p + 3, sizeof(float),
-                                        hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
     const unsigned long long* c = h->h_counts;
+    float ax[4];
+    memcpy(ax, c + 5, sizeof ax);
+    if (stats) memcpy(stats->axis, ax, 3 * sizeof(float));
+    const float src = ax[3];
     if (h->timing && h->nev >= 2) {
         for (int i = 0; i + 1 < h->nev; ++i)
             (void)hipEventElapsedTime(&h->stage_ms[i], h->ev[i], h->ev[i + 1]);
@@ -887,7 +890,8 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     HIPCHK(h, launch_features(a, (float*)h->tile_sums.p, d_features, st));
     // the moments kernel bumped the epoch and cleared counters it owns; reset
     // the accumulators the finish kernel normally resets
-    HIPCHK(h, launch_finish((Ctrl*)h->ctrl.p, ~0ull, (unsigned long long*)h->counts.p, 0, st));
+    HIPCHK(h, launch_finish((Ctrl*)h->ctrl.p, ~0ull, (unsigned long long*)h->counts.p, 0, nullptr, nullptr,
+                            st));
     HIPCHK(h, hipStreamSynchronize(st));
     return GSAP_OK;
 }

@@ -171,8 +171,10 @@ hipError_t launch_scene_offsets(const uint2* P, Ctrl* ctrl, unsigned long long c
                                 const uint32_t* seg_start, uint32_t nseg, unsigned long long* out,
                                 hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
-hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts,
-                         int sym, hipStream_t st);
+// d_counts[0..4] = pairs, entries, sweep tests, flags, executed tests; with
+// axis_out: axis_out[0..1] = the float bits of axis[0..3]
+hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts, int sym,
+                         const float* axis, unsigned long long* axis_out, hipStream_t st);
 
 // ---- learned sweep axis (k_mlp.hip) ----
 constexpr int kFeat = 64, kHid = 128, kOut = 4, kCand = 64;

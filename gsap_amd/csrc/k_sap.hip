@@ -669,7 +669,7 @@ constexpr uint32_t kStage = kSweepStage;
 // entries to be its box's first slab (the pair's slab: that of the larger
 // lower bound), so each pair is found exactly once.
 template <int KIND, bool GROUP, int MODE, bool SLAB>  // MODE 0 canonical, 1 symmetric, 2 count
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_sweep(SweepArgs a) {
+__global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
     // The object records are read-only here; the constant address space lets
     // their wave-uniform loads go through the scalar cache.
@@ -1083,7 +1083,8 @@ __global__ __launch_bounds__(kWG) void k_entries_write(uint32_t* has, const uint
     }
 }
 
-__global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts) {
+__global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts,
+                         const float* axis, unsigned long long* axis_out) {
     if (threadIdx.x == 0) {
         const unsigned long long np = ctrl->num_pairs;
         uint32_t fl = ctrl->flags;
@@ -1093,6 +1094,12 @@ __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
         d_counts[2] = ctrl->tests;
         d_counts[3] = fl;
         d_counts[4] = ctrl->exec_tests;
+        if (axis_out) {  // the (first) scene's axis and its source, as float bits
+            axis_out[0] = (unsigned long long)__float_as_uint(axis[0]) |
+                          ((unsigned long long)__float_as_uint(axis[1]) << 32);
+            axis_out[1] = (unsigned long long)__float_as_uint(axis[2]) |
+                          ((unsigned long long)__float_as_uint(axis[3]) << 32);
+        }
         ctrl->num_pairs = 0;
         ctrl->num_entries = 0;
         ctrl->tests = 0;
@@ -1215,9 +1222,9 @@ hipError_t launch_entries(const uint2* P, const unsigned long long* n_dev, unsig
 }
 
 hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts, int sym,
-                         hipStream_t st) {
+                         const float* axis, unsigned long long* axis_out, hipStream_t st) {
     (void)sym;
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, ctrl, cap, d_counts);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, ctrl, cap, d_counts, axis, axis_out);
     return hipGetLastError();
 }
 
