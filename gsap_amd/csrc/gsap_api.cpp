@@ -162,7 +162,10 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
     if (e == hipSuccess) e = hipMemset(h->ctrl.p, 0, sizeof(Ctrl));
     if (e == hipSuccess) e = grow(h->counts, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_counts, 8 * sizeof(unsigned long long), 0);
-    for (int i = 0; i < gsap_sap::kEv && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
+    // stage events only time the work (read after the stream is synchronised),
+    // so they skip the system-scope fence (an L2 write-back on every record)
+    for (int i = 0; i < gsap_sap::kEv && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&h->ev[i], hipEventDisableSystemFence);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) {
         int dev = 0, cus = 0;
