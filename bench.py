@@ -242,8 +242,9 @@ def main():
                     help="run one extra section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "multi-GPU sharding is design-only", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
+              "the broadphase runs on a single GPU and its sharding is not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
