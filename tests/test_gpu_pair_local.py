@@ -5,7 +5,9 @@ list must be bit-identical to the brute-force oracle and to the whole-key
 sort, across key widths (local bits L = 8 ... 24, one to three onesweep
 passes), bucket shapes (one bucket larger than a workgroup's LDS sort, windows
 with more buckets than a 32-bit LDS key can rank), output kinds, batched
-scenes, pair regions with padding, and a capacity overflow."""
+scenes, pair regions with padding, a capacity overflow, and the two-pass sort
+whose second pass runs in per-XCD parts (34- to 40-bit keys, all keys in one
+part)."""
 import numpy as np
 import pytest
 
@@ -191,3 +193,38 @@ def test_local_40bit_keys_capacity_error_then_retry(dev):
     big = torch.empty_like(ref)
     P, _, _ = s.collide_into(bd, big)
     assert P == ref.shape[0] and torch.equal(big, ref)
+
+
+# Two onesweep passes with 8 pair regions: the second runs in 8 parts by the
+# first digit's top 3 bits.  n -> K: 100,000 -> 34, 200,000 -> 36.
+@pytest.mark.parametrize("n", [100_000, 200_000])
+def test_local_two_pass_parts_key_widths(dev, n):
+    b = W.aniso(n, seed=n % 97, sigma=(12.0, 4.0, 1.0))
+    bd = to_dev(b, dev)
+    s0 = G.Sap()
+    s0.set_tune(pair_local=0)
+    ref, _, _ = s0.collide(bd)
+    for slabs in (0, 1):
+        s = G.Sap()
+        s.set_tune(pair_local=1, pair_regions=8, slabs=slabs)
+        got, _, _ = s.collide(bd)
+        assert np.array_equal(pairs_np(got), pairs_np(ref)), slabs
+
+
+def test_local_two_pass_one_part(dev):
+    # 256 identical boxes overlap each other and nothing else; 300,000 small
+    # boxes on a grid overlap nothing.  Every pair has i < j < 256, so all
+    # keys fall in the first part (seven parts empty); 32,640 pairs.
+    n = 300_000
+    g = np.arange(n, dtype=np.float64)
+    lo = np.stack([(g % 100) * 10.0, ((g // 100) % 100) * 10.0, (g // 10000) * 10.0], 1)
+    b = np.concatenate([lo, lo + 1.0], 1).astype(np.float32)
+    b[:256, :3] = -50.0
+    b[:256, 3:] = -40.0
+    i, j = np.triu_indices(256, 1)
+    expect = np.stack([i, j], 1).astype(np.uint32)
+    for slabs in (0, 1):
+        s = G.Sap()
+        s.set_tune(pair_local=1, pair_regions=8, slabs=slabs)
+        got, _, _ = s.collide(to_dev(b, dev))
+        assert np.array_equal(pairs_np(got), expect), slabs
