@@ -362,14 +362,15 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
 // kMaxRegions parts: part x holds the keys whose first digit d1 has
 // d1 >> 5 == x, a contiguous range of the first pass's output (which is
 // ordered by d1).  Digit d's output range is laid out part after part: part
-// x's keys of digit d start at start(d) + sum over x' < x of J[x'][d] (J: the
-// per-part digit counts the sweep took as it wrote the pairs), so the result is ordered by (d, d1)
-// as after one stable pass, while each part runs its own look-back chain on
-// the workgroups of one XCD (w mod kMaxRegions == x, the grid a multiple of
-// kMaxRegions).  A digit's neighbouring runs then lie in one L2, which merges
-// the lines they share before writing them back.  Within a part: tiles
-// claimed through the part's counter, stable ballot ranking, look-back to
-// the part's first tile (inclusive), LDS-staged coalesced scatter.
+// x's keys of digit d start at start(d) + sum over x' < x of J[x'][d] (J:
+// the per-part digit counts the sweep took as it wrote the pairs), so the
+// result is ordered by (d, d1) as after one stable pass, while each part
+// runs its own look-back chain on the workgroups of one XCD (w mod
+// kMaxRegions == x, the grid a multiple of kMaxRegions).  A digit's
+// neighbouring runs then lie in one L2, which merges the lines they share
+// before writing them back.  Within a part: tiles claimed through the
+// part's counter, ranking (below), look-back to the part's first tile
+// (inclusive), LDS-staged coalesced scatter.
 template <int ITEMS>
 __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, uint2* __restrict__ kout,
                                                SortArgs a) {
