@@ -555,7 +555,7 @@ template <int KIND>
 __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* __restrict__ K,
                                                 const uint32_t* __restrict__ V) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
-    __shared__ uint32_t s_end[kWG];
+    __shared__ unsigned long long s_key[kWG];  // (sweep end, lane): the grouping keys
     __shared__ uint32_t s_sb[257];  // slab mode: first sorted position of each slab (+ end)
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
@@ -622,13 +622,14 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
             }
             reinterpret_cast<Rec*>(a.rec)[s] = r;
         }
-        s_end[tid] = end;
+        // place in the sub-tile ordered by (sweep end, position): the number
+        // of smaller keys (keys are unique)
+        const unsigned long long mk = ((unsigned long long)end << 8) | tid;
+        s_key[tid] = mk;
         __syncthreads();
         uint32_t rank = 0;
-        for (uint32_t j = 0; j < kWG; ++j) {
-            const uint32_t ej = s_end[j];
-            rank += (ej < end) | ((ej == end) & (j < tid));
-        }
+#pragma unroll 8
+        for (uint32_t j = 0; j < kWG; ++j) rank += s_key[j] < mk ? 1u : 0u;
         if (valid) {
             r.aux = s;
             reinterpret_cast<Rec*>(a.grp)[sub + rank] = r;
