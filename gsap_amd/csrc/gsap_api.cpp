@@ -249,7 +249,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     }
     // pair digit histograms, then the first pass's per-region counts and slot counters
     if (!h->hist_pair.p)
-        HIPCHK(h, grow(h->hist_pair, (kMaxPasses * 256 + kPairClaimWords) * 4));
+        HIPCHK(h, grow(h->hist_pair, (kMaxPasses * 256 + kPairClaimWords + kPairDigitStartWords) * 4));
     if (slab_mode(h) && !h->slabp.p) {
         HIPCHK(h, grow(h->axis2, 4 * sizeof(float)));
         HIPCHK(h, grow(h->slabp, kSlabParams * sizeof(float)));
@@ -457,6 +457,13 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.hist_reg = msplit ? a.pair_claim : nullptr;
         p.claim = msplit ? a.pair_claim + kMaxRegions * 256 : nullptr;
         p.joint = joint;
+        // keys of at most 40 bits after the per-part pass: 32-bit items for
+        // the local stage (it reads 4 bytes per pair; the top bits are the
+        // pass's digit, implied by the position)
+        const bool out32 = joint != nullptr && kbits <= 40;
+        uint32_t* const dstarts = a.pair_claim + kPairClaimWords;
+        p.out32 = out32 ? 1 : 0;
+        p.st_out = out32 ? dstarts : nullptr;
         // the per-part pass runs best at three workgroups per CU (more tiles
         // in flight per XCD evict the lines neighbouring runs share)
         p.xgrid = 3 * h->cus;
@@ -481,7 +488,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
             uint2* y = (npass_pair & 1u) ? t0p : t1p;
             HIPCHK(h, launch_pair_local(x, y, outp, &ctrl->sort_real, &ctrl->num_pairs, cap,
                                         h->slot_cap, (uint32_t*)h->wstart.p, &ctrl->local_big,
-                                        (int)bbits, (int)shift0, st));
+                                        (int)bbits, (int)shift0, out32 ? dstarts : nullptr, st));
             mark(h, st);
         }
         if (sym)

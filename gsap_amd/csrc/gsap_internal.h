@@ -106,6 +106,7 @@ struct ObjPipeArgs {
     uint32_t* pair_claim;
 };
 constexpr uint32_t kPairClaimWords = 3u * kMaxRegions * 256u;
+constexpr uint32_t kPairDigitStartWords = 260u;  // digit starts of the 32-bit items (257 used)
 constexpr int kVr = 5;           // max v extent, min lo_v, max hi_v, min lo_u, max hi_u
 constexpr int kSlabParams = 8;   // v0, 1/W, S, u0, 1/du, W, -, -
 constexpr uint32_t kSlabNull = 0xFFFFFFFFu;  // key of an unused second entry
@@ -238,6 +239,11 @@ struct SortArgs {
     // by the sweep; the second pass then runs per part, on xgrid workgroups
     uint32_t* joint;
     uint32_t xgrid;
+    // with joint and keys of at most 40 bits: the per-part pass writes each
+    // key's low 32 bits (uint32 items; the bits above are its digit's) and
+    // the digit starts st_out[0..256] (the exclusive scan of its counts)
+    int out32;
+    uint32_t* st_out;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
@@ -257,10 +263,15 @@ constexpr uint32_t kLocW = 3072;   // window: pairs whose bucket starts are one 
 constexpr uint32_t kLocC = 8192;   // pairs sorted in LDS by one workgroup
 uint64_t pair_local_wstart_words(uint64_t cap);  // u32 words of the window-start table
 // big_ctr: a device word the launch zeroes and counts the handed-on windows in.
+// st non-null: x holds 32-bit items (each key's low 32 bits; keys of at
+// most 40 bits after a two-pass sort whose second pass ran per part, the
+// bits above implied by the position: that pass's digit d occupies
+// [st[d], st[d + 1])), and x has room for 2 cap pairs (the upper half is
+// scratch for oversized windows).
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
-                             int L, hipStream_t st);
+                             int L, const uint32_t* st, hipStream_t stream);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
 constexpr uint32_t kSweepHistPasses = 2;   // pair-sort digit histograms the sweep can count

@@ -36,7 +36,6 @@ constexpr uint32_t kBins = 1u << kBinBits;
 constexpr uint32_t kBinMax = 256;       // largest bin ranked by comparisons (each key walks its bin)
 constexpr uint32_t kLocTail = 64;      // LDS words of the key array used for reductions
 constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
-constexpr uint32_t kGlobChunk = kLT * kGlobItems;
 
 // the wave's OR / AND, in every lane (all lanes active)
 __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
@@ -96,15 +95,82 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, int bw, ui
 
 }  // namespace
 
+// Pair key bits of the 32-bit local view.  khigh = key >> L and klow = key &
+// (2^L - 1) for key = (a << bbits) | b, in 32-bit arithmetic; valid when the
+// key has at most 32 + L bits (2 bbits - L <= 32, i.e. up to 2^28 objects).
+__device__ __forceinline__ uint32_t khigh(uint2 p, int bbits, int L) {
+    return L >= bbits ? (p.x >> (L - bbits)) : ((p.x << (bbits - L)) | (p.y >> L));
+}
+__device__ __forceinline__ uint32_t klow(uint2 p, int bbits, int L) {
+    return L >= bbits ? (((p.x & ((1u << (L - bbits)) - 1u)) << bbits) | p.y)
+                      : (p.y & ((1u << L) - 1u));
+}
+// The pair of key (h << L) | lo.
+__device__ __forceinline__ uint2 kpair(uint32_t h, uint32_t lo, int bbits, int L) {
+    if (L >= bbits)
+        return make_uint2((h << (L - bbits)) | (lo >> bbits), lo & ((1u << bbits) - 1u));
+    return make_uint2(h >> (bbits - L), ((h & ((1u << (bbits - L)) - 1u)) << L) | lo);
+}
+
+// 32-bit items: after a two-pass sort of keys of at most 40 bits whose
+// second pass ran per part, an item is the key's low 32 bits.  The bits
+// above are the top 8 - hb bits of the second digit d (hb = 40 - K: the
+// digit's bits below bit 32), and d is known from the position: digit d
+// occupies [st[d], st[d + 1]) (st, 257 words, written by that pass).
+// load_st copies st into LDS (ends with a barrier).
+template <int LT>
+__device__ __forceinline__ void load_st(const uint32_t* __restrict__ st_g, uint32_t* st) {
+    for (uint32_t i = threadIdx.x; i <= 256u; i += LT) st[i] = st_g[i];
+    __syncthreads();
+}
+// The digit of position q < st[256]: the largest d with st[d] <= q.
+__device__ __forceinline__ uint32_t d2_of(const uint32_t* st, uint32_t q) {
+    uint32_t lo = 0, hi = 256;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (st[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// key >> L and key & (2^L - 1) of an item with second digit d (L <= 24).
+__device__ __forceinline__ uint32_t ihigh(uint32_t v, uint32_t d, int hb, int L) {
+    return ((d >> hb) << (32 - L)) | (v >> L);
+}
+__device__ __forceinline__ uint32_t ilow(uint32_t v, int L) { return v & ((1u << L) - 1u); }
+
+// One item of the local stage: an 8-byte pair, or (X32) a 32-bit item with
+// its second digit d.
+template <bool X32>
+struct LItem {
+    using T = typename std::conditional<X32, uint32_t, uint2>::type;
+    __device__ static __forceinline__ uint32_t high(T v, uint32_t d, int bbits, int hb, int L) {
+        if constexpr (X32) return ihigh(v, d, hb, L);
+        else return khigh(v, bbits, L);
+    }
+    __device__ static __forceinline__ uint32_t low(T v, int bbits, int L) {
+        if constexpr (X32) return ilow(v, L);
+        else return klow(v, bbits, L);
+    }
+};
+
 // One wave per window boundary j in [0, J]: the smallest q >= j * kLocW with
 // q == n or key[q] >> L != key[q-1] >> L.  The pairs are sorted by key >> L,
-// so "key[q] >> L differs from key[t-1] >> L" is monotone in q >= t.
-__global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P,
+// so "key[q] >> L differs from key[t-1] >> L" is monotone in q >= t.  With
+// 32-bit items (second digits from st) it holds from the end of key[t-1]'s
+// digit bucket on, and before it when the items' bits from L up differ.
+template <bool X32>
+__global__ __launch_bounds__(kWG) void k_pair_wstart(const void* __restrict__ Pv,
                                                      const unsigned long long* __restrict__ n_dev,
                                                      int bbits, int L,
                                                      uint32_t* __restrict__ wstart,
-                                                     uint32_t* __restrict__ big_ctr) {
+                                                     uint32_t* __restrict__ big_ctr,
+                                                     const uint32_t* __restrict__ st_g) {
+    __shared__ uint32_t s_st[X32 ? 257 : 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) *big_ctr = 0u;  // read by the kernels after this one
+    if constexpr (X32) load_st<kWG>(st_g, s_st);
+    const uint2* const P = reinterpret_cast<const uint2*>(Pv);
+    const uint32_t* const P32 = reinterpret_cast<const uint32_t*>(Pv);
     const uint32_t n = (uint32_t)*n_dev;
     const uint32_t J = (n + kLocW - 1) / kLocW;
     const uint32_t lane = lane_id();
@@ -115,12 +181,23 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P
         if (t >= n) {
             res = n;
         } else if (t > 0) {
-            const unsigned long long h0 = pkey(P[t - 1], bbits) >> L;
+            unsigned long long h0 = 0, qend = n;
+            if constexpr (X32) {
+                const uint32_t d0 = d2_of(s_st, (uint32_t)(t - 1));
+                if (s_st[d0 + 1] < qend) qend = s_st[d0 + 1];
+                h0 = P32[t - 1] >> L;
+            } else {
+                h0 = pkey(P[t - 1], bbits) >> L;
+            }
+            auto differs = [&](unsigned long long q) {  // q < n
+                if constexpr (X32) return q >= qend || (unsigned long long)(P32[q] >> L) != h0;
+                else return (pkey(P[q], bbits) >> L) != h0;
+            };
             // gallop: probes lo + lane * step, the step growing 64x per round
             unsigned long long lo = t, hi = n, step = 1;
             for (;;) {
                 const unsigned long long q = lo + (unsigned long long)lane * step;
-                const bool pr = q >= n || (pkey(P[q], bbits) >> L) != h0;
+                const bool pr = q >= n || differs(q);
                 const unsigned long long m = ballot(pr);
                 if (m) {
                     const uint32_t i = (uint32_t)__builtin_ctzll(m);
@@ -140,7 +217,7 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P
             while (lo < hi) {
                 const unsigned long long stride = (hi - lo + 63) / 64;
                 const unsigned long long q = lo + (unsigned long long)lane * stride;
-                const bool pr = q >= hi || (pkey(P[q], bbits) >> L) != h0;
+                const bool pr = q >= hi || differs(q);
                 const unsigned long long m = ballot(pr);
                 if (m == 0) {
                     lo += 63ull * stride + 1;
@@ -159,23 +236,6 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const uint2* __restrict__ P
         }
         if (lane == 0) wstart[j] = (uint32_t)res;
     }
-}
-
-// Pair key bits of the 32-bit local view.  khigh = key >> L and klow = key &
-// (2^L - 1) for key = (a << bbits) | b, in 32-bit arithmetic; valid when the
-// key has at most 32 + L bits (2 bbits - L <= 32, i.e. up to 2^28 objects).
-__device__ __forceinline__ uint32_t khigh(uint2 p, int bbits, int L) {
-    return L >= bbits ? (p.x >> (L - bbits)) : ((p.x << (bbits - L)) | (p.y >> L));
-}
-__device__ __forceinline__ uint32_t klow(uint2 p, int bbits, int L) {
-    return L >= bbits ? (((p.x & ((1u << (L - bbits)) - 1u)) << bbits) | p.y)
-                      : (p.y & ((1u << L) - 1u));
-}
-// The pair of key (h << L) | lo.
-__device__ __forceinline__ uint2 kpair(uint32_t h, uint32_t lo, int bbits, int L) {
-    if (L >= bbits)
-        return make_uint2((h << (L - bbits)) | (lo >> bbits), lo & ((1u << bbits) - 1u));
-    return make_uint2(h >> (bbits - L), ((h & ((1u << (bbits - L)) - 1u)) << L) | lo);
 }
 
 // Counting sort of one window in LDS by LT threads, ITEMS keys each (kk,
@@ -286,21 +346,28 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
 // Software pipeline over the workgroup's windows j, j + G, ...: window starts
 // are read two windows ahead, and the next window's pairs are loaded into
 // registers while this window is sorted.
+template <bool X32>
 __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pair_local(
-    const uint2* __restrict__ X, uint2* __restrict__ out, const unsigned long long* __restrict__ n_dev,
+    const void* __restrict__ Xv, uint2* __restrict__ out, const unsigned long long* __restrict__ n_dev,
     const unsigned long long* __restrict__ last_n, unsigned long long last_cap,
     const uint32_t* __restrict__ wstart, uint32_t* __restrict__ big, uint32_t* __restrict__ big_ctr,
-    int bbits, int L) {
-    // 48 KB, so that three workgroups fit a CU: the reduction words live in
-    // the last kLocTail words of the key array, which the keys never reach
-    // and which are dead while the keys are scattered and ranked.
+    int bbits, int L, const uint32_t* __restrict__ st_g, int hb) {
+    using IT = LItem<X32>;
+    using XT = typename IT::T;
+    const XT* __restrict__ X = reinterpret_cast<const XT*>(Xv);
+    // 48 KB (+1 KB with 32-bit items), so that three workgroups fit a CU: the
+    // reduction words live in the last kLocTail words of the key array, which
+    // the keys never reach and which are dead while the keys are scattered
+    // and ranked.
     __shared__ uint32_t s_e[kLocC];
     __shared__ uint32_t s_bin[kBins];
+    __shared__ uint32_t s_st[X32 ? 257 : 1];
     uint32_t* const s_tmp = s_e + (kLocC - kLocTail);
     uint32_t* const s_red = s_tmp + kLWaves;
     constexpr uint32_t kCap = kLocC - kLocTail;
 
     const uint32_t tid = threadIdx.x;
+    if constexpr (X32) load_st<kLT>(st_g, s_st);
     const uint32_t n = (uint32_t)*n_dev;
     unsigned long long out_lim = n;
     {
@@ -324,7 +391,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // The prefetch holds the first kPre rows of a window (4,096 pairs, more
     // than most windows have); larger windows load the rest when sorted.
     constexpr int kPre = kLocItems / 2;
-    uint2 raw[kPre], rfirst = make_uint2(0u, 0u), rlast = make_uint2(0u, 0u);
+    XT raw[kPre], rfirst{}, rlast{};
     auto fetch = [&](uint32_t fs, uint32_t fe) {  // the pairs of an LDS-sized window
         const uint32_t fS = fe - fs;
         if (fS == 0 || fS > kCap || !fits32) return;
@@ -334,7 +401,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
 #pragma unroll
         for (int k = 0; k < kPre; ++k) {
             const uint32_t q = tid + (uint32_t)kLT * k;
-            raw[k] = make_uint2(0u, 0u);
+            raw[k] = XT{};
             if ((uint32_t)k < fnk && q < fS) raw[k] = X[fs + q];
         }
     };
@@ -348,21 +415,36 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         bool fetched = false;
         if (s < e) {  // some bucket starts in this window
             const uint32_t S = e - s;
-            const uint32_t hs = khigh(rfirst, bbits, L);
-            bool ok = fits32 && S <= kCap && khigh(rlast, bbits, L) - hs < (1u << (32 - L));
+            // 32-bit items: a window inside one second-digit bucket (all but
+            // the few that cross a bucket start, which are handed on) has
+            // keys (d << 32 | item) with d fixed, so its keys relative to the
+            // first bucket are item - (first item with its low L bits clear)
+            uint32_t ds = 0, hs = 0;
+            bool ok = false;
+            if constexpr (X32) {
+                ds = d2_of(s_st, s);
+                hs = ihigh(rfirst, ds, hb, L);
+                ok = S <= kCap && d2_of(s_st, e - 1) == ds;
+            } else {
+                hs = khigh(rfirst, bbits, L);
+                ok = fits32 && S <= kCap && khigh(rlast, bbits, L) - hs < (1u << (32 - L));
+            }
             if (ok) {
                 const uint32_t nk = (S + kLT - 1) / kLT;
                 uint32_t kk[kLocItems];
 #pragma unroll
                 for (int k = 0; k < kLocItems; ++k) {
-                    uint2 p = make_uint2(0u, 0u);
+                    const uint32_t q = tid + (uint32_t)kLT * k;
+                    XT p{};
                     if (k < kPre) {
                         p = raw[k < kPre ? k : 0];
                     } else if ((uint32_t)k < nk) {
-                        const uint32_t q = tid + (uint32_t)kLT * k;
                         if (q < S) p = X[s + q];
                     }
-                    kk[k] = ((khigh(p, bbits, L) - hs) << L) | klow(p, bbits, L);
+                    if constexpr (X32)
+                        kk[k] = p - (rfirst & ~((1u << L) - 1u));
+                    else
+                        kk[k] = ((khigh(p, bbits, L) - hs) << L) | klow(p, bbits, L);
                 }
                 fetch(s1, e1);
                 fetched = true;
@@ -387,18 +469,27 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
 constexpr int kBigT = 1024;
 constexpr int kBigItems = 16;
 constexpr uint32_t kBigC = kBigT * kBigItems;
-__global__ __launch_bounds__(kBigT) void k_pair_local_big(uint2* X, uint2* Y, uint2* __restrict__ out,
+template <bool X32>
+__global__ __launch_bounds__(kBigT) void k_pair_local_big(const void* Xv, uint2* Z, uint2* Y,
+                                                          uint2* __restrict__ out,
                                                           const unsigned long long* __restrict__ n_dev,
                                                           const unsigned long long* __restrict__ last_n,
                                                           unsigned long long last_cap,
                                                           const uint32_t* __restrict__ wstart,
                                                           const uint32_t* __restrict__ big,
                                                           const uint32_t* __restrict__ big_ctr, int bbits,
-                                                          int L) {
+                                                          int L, const uint32_t* __restrict__ st_g, int hb) {
+    using IT = LItem<X32>;
+    using XT = typename IT::T;
+    const XT* const Xin = reinterpret_cast<const XT*>(Xv);
+    // the global path's 8-byte pairs: the input itself, or (32-bit items) the
+    // window expanded into the same positions of a free buffer Z
+    uint2* const X = X32 ? Z : const_cast<uint2*>(reinterpret_cast<const uint2*>(Xv));
     constexpr int NW = kBigT / 64;
     constexpr uint32_t kChunk = kBigT * kGlobItems;
     // one LDS block, 80 KB: keys | bins (the LSD path's counters reuse it)
     __shared__ uint32_t s_mem[kBigC + kBins];
+    __shared__ uint32_t s_st[X32 ? 257 : 1];
     uint32_t* const s_e = s_mem;
     uint32_t* const s_bin = s_mem + kBigC;
     uint32_t* const s_tmp = s_e + (kBigC - kLocTail);
@@ -409,6 +500,7 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(uint2* X, uint2* Y, ui
     constexpr uint32_t kCap = kBigC - kLocTail;
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    if constexpr (X32) load_st<kBigT>(st_g, s_st);
     const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
     const uint32_t n = (uint32_t)*n_dev;
     unsigned long long out_lim = n;
@@ -423,22 +515,37 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(uint2* X, uint2* Y, ui
     for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
         const uint32_t j = big[bi];
         const uint32_t s = wstart[j], e = wstart[j + 1], S = e - s;
+        uint32_t ds = 0;  // 32-bit items: the second digit of the window's first item
+        if constexpr (X32) ds = d2_of(s_st, s);
         if (fits32 && S <= kCap) {
-            const uint32_t hs = khigh(X[s], bbits, L);
-            if (khigh(X[e - 1], bbits, L) - hs < (1u << (32 - L))) {
+            uint32_t de = 0;
+            if constexpr (X32) de = d2_of(s_st, e - 1);
+            const uint32_t hs = IT::high(Xin[s], ds, bbits, hb, L);
+            if (IT::high(Xin[e - 1], de, bbits, hb, L) - hs < (1u << (32 - L))) {
                 const uint32_t nk = (S + kBigT - 1) / kBigT;
-                uint32_t kk[kBigItems];
+                uint32_t kk[kBigItems], d = ds;
 #pragma unroll
                 for (int k = 0; k < kBigItems; ++k) {
-                    uint2 p = make_uint2(0u, 0u);
+                    XT p{};
                     const uint32_t q = tid + (uint32_t)kBigT * k;
-                    if ((uint32_t)k < nk && q < S) p = X[s + q];
-                    kk[k] = ((khigh(p, bbits, L) - hs) << L) | klow(p, bbits, L);
+                    if ((uint32_t)k < nk && q < S) p = Xin[s + q];
+                    if constexpr (X32)  // positions rise with k: advance the digit
+                        while (d < 255u && s + q >= s_st[d + 1]) ++d;
+                    kk[k] = ((IT::high(p, d, bbits, hb, L) - hs) << L) | IT::low(p, bbits, L);
                 }
                 if (bin_sort<kBigT, kBigItems>(kk, nk, S, s, hs, out_lim, out, s_e, s_bin, s_tmp,
                                                s_red, bbits, L))
                     continue;
             }
+        }
+        if constexpr (X32) {  // expand the window to 8-byte pairs at X[s, e)
+            for (uint32_t q = tid; q < S; q += kBigT) {
+                const uint32_t dq = d2_of(s_st, s + q);
+                const unsigned long long key =
+                    ((unsigned long long)(dq >> hb) << 32) | (unsigned long long)Xin[s + q];
+                X[s + q] = pkey_pair(key, bbits);
+            }
+            __syncthreads();
         }
         const unsigned long long hs = pkey(X[s], bbits) >> L;
         const unsigned long long he = pkey(X[e - 1], bbits) >> L;
@@ -521,24 +628,34 @@ uint64_t pair_local_wstart_words(uint64_t cap) { return 2 * wstart_half(cap); }
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
-                             int L, hipStream_t st) {
+                             int L, const uint32_t* stg, hipStream_t st) {
     if (L < 8 || L > 24) return hipErrorInvalidValue;
+    const bool x32 = stg != nullptr;
+    const int kbits = 2 * bbits;
+    if (x32 && (kbits > 40 || kbits - L != 16)) return hipErrorInvalidValue;
+    const int hb = x32 ? 40 - kbits : 0;  // second-digit bits below bit 32
     const unsigned long long J = (cap + kLocW - 1) / kLocW;  // windows, at most
     uint32_t* big = wstart + wstart_half(cap);
+    uint2* const z = x + cap;  // 32-bit items: expansion space of oversized windows
     unsigned long long gw = (J + 1 + kWaves - 1) / kWaves;
     if (gw > 2048) gw = 2048;
-    hipLaunchKernelGGL(k_pair_wstart, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits, L,
-                       wstart, big_ctr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     unsigned long long gl = J == 0 ? 1 : J;
     if (gl > 2048) gl = 2048;
-    hipLaunchKernelGGL(k_pair_local, dim3((uint32_t)gl), dim3(kLT), 0, st, x, out, n_dev, last_n,
-                       last_cap, wstart, big, big_ctr, bbits, L);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pair_local_big, dim3(256), dim3(kBigT), 0, st, x, y, out, n_dev, last_n,
-                       last_cap, wstart, big, big_ctr, bbits, L);
+    if (x32) {
+        hipLaunchKernelGGL(k_pair_wstart<true>, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits, L,
+                           wstart, big_ctr, stg);
+        hipLaunchKernelGGL(k_pair_local<true>, dim3((uint32_t)gl), dim3(kLT), 0, st, x, out, n_dev,
+                           last_n, last_cap, wstart, big, big_ctr, bbits, L, stg, hb);
+        hipLaunchKernelGGL(k_pair_local_big<true>, dim3(256), dim3(kBigT), 0, st, x, z, y, out, n_dev,
+                           last_n, last_cap, wstart, big, big_ctr, bbits, L, stg, hb);
+    } else {
+        hipLaunchKernelGGL(k_pair_wstart<false>, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits,
+                           L, wstart, big_ctr, stg);
+        hipLaunchKernelGGL(k_pair_local<false>, dim3((uint32_t)gl), dim3(kLT), 0, st, x, out, n_dev,
+                           last_n, last_cap, wstart, big, big_ctr, bbits, L, stg, hb);
+        hipLaunchKernelGGL(k_pair_local_big<false>, dim3(256), dim3(kBigT), 0, st, x, x, y, out, n_dev,
+                           last_n, last_cap, wstart, big, big_ctr, bbits, L, stg, hb);
+    }
     return hipGetLastError();
 }
 

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
 // before writing them back.  Within a part: tiles claimed through the
 // part's counter, ranking (below), look-back to the part's first tile
 // (inclusive), LDS-staged coalesced scatter.
-template <int ITEMS>
+template <int ITEMS, bool OUT32>
 __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, uint2* __restrict__ kout,
                                                SortArgs a) {
     constexpr uint32_t BITS = 8, RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
@@ -411,7 +411,12 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
         all += c;
         pre += q < x ? c : 0u;
     }
-    const uint32_t dbase = block_excl_scan(all, s_tmp, &tot) + pre;
+    const uint32_t dstart = block_excl_scan(all, s_tmp, &tot);
+    const uint32_t dbase = dstart + pre;
+    if (OUT32 && blockIdx.x == 0) {  // the digit starts, for the readers of the 32-bit items
+        a.st_out[tid] = dstart;
+        if (tid == 0) a.st_out[RADIX] = tot;
+    }
     __syncthreads();
     if (tid == 0) {
         uint32_t t = 0;
@@ -556,7 +561,12 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
             const uint2 kk = s_k[i];
             const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
             const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
-            if (g < n) kout[g] = kk;
+            if (g < n) {
+                if constexpr (OUT32)  // the key's low 32 bits; the bits above are d's
+                    reinterpret_cast<uint32_t*>(kout)[g] = (kk.x << a.bbits) | kk.y;
+                else
+                    kout[g] = kk;
+            }
         }
         __syncthreads();
     }
@@ -685,7 +695,10 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
             if (p == 1 && a.joint && a.npass == 2) {  // the last pass, per part
                 const uint32_t xg = a.xgrid ? a.xgrid : grid;
                 const uint32_t mg = (xg + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
-                hipLaunchKernelGGL((k_xpass<ITEMS>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
+                if (a.out32)
+                    hipLaunchKernelGGL((k_xpass<ITEMS, true>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
+                else
+                    hipLaunchKernelGGL((k_xpass<ITEMS, false>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
