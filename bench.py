@@ -304,10 +304,17 @@ def main():
     obj_passes = -(-32 // tn["sort_bits"])
     alg_sweep = 24.0 * n + 8.0 * P
     alg_pass = 16.0 * P
+    # Keys of at most 40 bits sorted in two passes with 8 pair regions: the
+    # second pass writes 4-byte items (8 B read + 4 B written per pair) and
+    # the local stage reads them (4 B read + 8 B written).
+    x32 = bool(tn["pair_local"]) and tn["pair_regions"] == 8 and 2 * max(1, (n - 1).bit_length()) <= 40 \
+        and 2 * max(1, (n - 1).bit_length()) > 32
+    pass_bytes = [alg_pass, 12.0 * P if x32 else alg_pass]
+    alg_local = 12.0 * P if x32 else alg_pass
     cands = [("k_sweep", stages["sweep"], stages["sweep"], alg_sweep),
              ("k_onesweep<pair>", float(pass_ms.sum()), float(pass_ms.mean()), alg_pass)]
     if tn["pair_local"]:
-        cands.append(("k_pair_local", stages["pair_local"], stages["pair_local"], alg_pass))
+        cands.append(("k_pair_local", stages["pair_local"], stages["pair_local"], alg_local))
     roof_k, _, roof_ms, roof_b = max(cands, key=lambda c: c[1])
     achieved = roof_b / (roof_ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -318,15 +325,17 @@ def main():
     sort_roof = {"achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "ms": round(stages["object_sort"], 4),
                  "bytes": obj_bytes, "passes": obj_passes, "entries": entries}
-    pair_gbs = alg_pass / (float(pass_ms.mean()) / 1e3) / 1e9
+    pb = [pass_bytes[min(i, 1)] if len(pass_ms) == 2 else alg_pass for i in range(len(pass_ms))]
+    pair_gbs = sum(pb) / (float(pass_ms.sum()) / 1e3) / 1e9
     pair_roof = {"achieved": round(pair_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(pair_gbs / HBM_PEAK_GBS, 4), "ms_per_pass": round(float(pass_ms.mean()), 4),
                  "ms_passes": [round(float(x), 4) for x in pass_ms],
-                 "passes": len(pass_ms), "bytes_per_pass": alg_pass}
+                 "frac_passes": [round(b / (float(t) / 1e3) / 1e9 / HBM_PEAK_GBS, 4) for b, t in zip(pb, pass_ms)],
+                 "passes": len(pass_ms), "bytes_passes": pb}
     if tn["pair_local"]:
-        loc_gbs = alg_pass / (stages["pair_local"] / 1e3) / 1e9
+        loc_gbs = alg_local / (stages["pair_local"] / 1e3) / 1e9
         pair_roof["local"] = {"achieved": round(loc_gbs, 1), "frac": round(loc_gbs / HBM_PEAK_GBS, 4),
-                              "ms": round(stages["pair_local"], 4), "bytes": alg_pass}
+                              "ms": round(stages["pair_local"], 4), "bytes": alg_local}
         pair_roof["ms_whole_sort"] = round(stages["pair_sort"], 4)
 
     cpu = None
