@@ -14,9 +14,9 @@
 //    it, positions [wstart[j], wstart[j+1]).  Up to kLocC pairs whose high
 //    bits lie within 2^(32-L) of the first one's: in LDS, on unique 32-bit
 //    keys (high - first high) << L | low bits.  One counting pass on the 12
-//    highest bits in which the window's keys differ puts every key in its
-//    bin; a key's place in the bin is the number of the bin's keys smaller
-//    than it.  Other windows (more than kLocC pairs, buckets too far apart
+//    bits below the top bit of the window's largest possible key (its last
+//    bucket with all low bits set) puts every key in its bin; a key's place
+//    in the bin is the number of the bin's keys smaller than it.  Other windows (more than kLocC pairs, buckets too far apart
 //    for a 32-bit key, or a bin of more than kBinMax keys) go to a list.
 //  * k_pair_local_big: the listed windows, by stable LSD passes of wave64
 //    ballot ranking through global memory, one chunk at a time,
@@ -36,16 +36,6 @@ constexpr uint32_t kBins = 1u << kBinBits;
 constexpr uint32_t kBinMax = 256;       // largest bin ranked by comparisons (each key walks its bin)
 constexpr uint32_t kLocTail = 64;      // LDS words of the key array used for reductions
 constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
-
-// the wave's OR / AND, in every lane (all lanes active)
-__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
-    const uint32_t m = wave_scan_dpp(v, 0u, [](uint32_t p, uint32_t q) { return p | q; });
-    return (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
-}
-__device__ __forceinline__ uint32_t wave_and_u32(uint32_t v) {
-    const uint32_t m = wave_scan_dpp(v, ~0u, [](uint32_t p, uint32_t q) { return p & q; });
-    return (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
-}
 
 // Exclusive scan over the LT threads of a workgroup (s_tmp: LT / 64 words).
 template <int LT>
@@ -240,52 +230,33 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const void* __restrict__ Pv
 
 // Counting sort of one window in LDS by LT threads, ITEMS keys each (kk,
 // the first nk rows valid, S keys in all; s_e holds at least S words).  The
-// keys are unique 32-bit views (high - hs) << L | low bits.  One counting
-// pass on the kBinBits highest bits in which the window's keys differ puts
-// every key in its bin; a key's place in its bin is the number of the bin's
+// keys are unique 32-bit views (high - hs) << L | low bits, none above khi.
+// One counting pass on the kBinBits bits below khi's top bit (the bits above
+// are zero in every key, so the bins are in key order) puts every key in its
+// bin; a key's place in its bin is the number of the bin's
 // smaller keys (bins are uneven -- keys cluster by bucket and index -- so the
-// work is spread over the keys rather than over the bins).  Writes the pairs
-// to out[s + rank] (below out_lim) and returns true, or returns false
-// without writing when a bin holds more than kBinMax keys.  Uniform control
-// flow; ends with a barrier, so the LDS may be reused at once.
+// work is spread over the keys rather than over the bins; the pairs, and so
+// the keys, are unique, so no tie-break is needed).  Writes the pairs to
+// out[s + rank] (below out_lim) and returns true, or returns false without
+// writing when a bin holds more than kBinMax keys.  s_bin holds kBins + 1
+// words: counts in [0, kBins), then bin b's start in word b + 1 (word 0 is
+// 0), so after the scatter bin b is [s_bin[b], s_bin[b + 1]).  Uniform
+// control flow; ends with a barrier, so the LDS may be reused at once.
 template <int LT, int ITEMS>
 __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t nk, uint32_t S,
-                                         uint32_t s, uint32_t hs, unsigned long long out_lim,
+                                         uint32_t khi, uint32_t s, uint32_t hs,
+                                         unsigned long long out_lim,
                                          uint2* __restrict__ out, uint32_t* s_e, uint32_t* s_bin,
                                          uint32_t* s_tmp, uint32_t* s_red, int bbits, int L) {
     constexpr int NW = LT / 64;
     constexpr uint32_t BPT = kBins / LT;  // bins per thread
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const uint32_t lowmask = (1u << L) - 1u;
-    uint32_t vor = 0u, vand = ~0u;
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t q = tid + (uint32_t)LT * k;
-        if ((uint32_t)k < nk && q < S) {
-            vor |= kk[k];
-            vand &= kk[k];
-        }
-    }
-    vor = wave_or_u32(vor);
-    vand = wave_and_u32(vand);
-    if (lane == 0) {
-        s_red[wave] = vor;
-        s_red[NW + wave] = vand;
-    }
     for (uint32_t i = tid; i < kBins; i += LT) s_bin[i] = 0u;
-    __syncthreads();
-    vor = 0u;
-    vand = ~0u;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        vor |= s_red[w];
-        vand &= s_red[NW + w];
-    }
-    // bins: the kBinBits highest bits in which the keys differ (the bits
-    // above are common to all of them)
-    const uint32_t diff = vor ^ vand;
-    const uint32_t hb = diff ? 31u - (uint32_t)__clz(diff) : 0u;
+    // bins: the kBinBits bits below khi's top bit hb (x >> sd < kBins)
+    const uint32_t hb = khi ? 31u - (uint32_t)__clz(khi) : 0u;
     const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t q = tid + (uint32_t)LT * k;
@@ -302,12 +273,13 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
     }
     mx = wave_max_u32(mx);
     if (lane == 0) s_red[2 * NW + wave] = mx;
-    uint32_t run = lblock_excl_scan<LT>(sum, s_tmp, &all);
+    uint32_t run = lblock_excl_scan<LT>(sum, s_tmp, &all);  // every count read before the writes
 #pragma unroll
     for (uint32_t i = 0; i < BPT; ++i) {
-        s_bin[tid * BPT + i] = run;
+        s_bin[tid * BPT + i + 1u] = run;
         run += cb[i];
     }
+    if (tid == 0) s_bin[0] = 0u;
     mx = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) mx = s_red[2 * NW + w] > mx ? s_red[2 * NW + w] : mx;
@@ -318,21 +290,25 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t q = tid + (uint32_t)LT * k;
-        if ((uint32_t)k < nk && q < S) s_e[atomicAdd(&s_bin[(kk[k] >> sd) & (kBins - 1u)], 1u)] = kk[k];
+        if ((uint32_t)k < nk && q < S)
+            s_e[atomicAdd(&s_bin[((kk[k] >> sd) & (kBins - 1u)) + 1u], 1u)] = kk[k];
     }
+    // places below rlim are written (out_lim - s, clamped)
+    const uint32_t rlim = out_lim <= s ? 0u
+                        : out_lim - s >= 0xffffffffull ? 0xffffffffu : (uint32_t)(out_lim - s);
     __syncthreads();
     for (uint32_t pz = tid; pz < S; pz += LT) {
         const uint32_t x = s_e[pz];
         const uint32_t bn = (x >> sd) & (kBins - 1u);
-        const uint32_t be = s_bin[bn];
-        const uint32_t bs = bn ? s_bin[bn - 1u] : 0u;
+        const uint32_t bs = s_bin[bn], be = s_bin[bn + 1u];
+        // two keys per step; the word after the bin's last (i + 1 <= S <
+        // the key array's size) is read but not counted
         uint32_t rank = 0;
-        for (uint32_t i = bs; i < be; ++i) {
-            const uint32_t y = s_e[i];
-            rank += (y < x || (y == x && i < pz)) ? 1u : 0u;
+        for (uint32_t i = bs; i < be; i += 2u) {
+            const uint32_t y0 = s_e[i], y1 = s_e[i + 1u];
+            rank += (y0 < x ? 1u : 0u) + (i + 1u < be && y1 < x ? 1u : 0u);
         }
-        const uint32_t pos = s + bs + rank;
-        if (pos < out_lim) out[pos] = kpair(hs + (x >> L), x & lowmask, bbits, L);
+        if (bs + rank < rlim) out[s + bs + rank] = kpair(hs + (x >> L), x & lowmask, bbits, L);
     }
     __syncthreads();
     return true;
@@ -360,7 +336,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // the keys never reach and which are dead while the keys are scattered
     // and ranked.
     __shared__ uint32_t s_e[kLocC];
-    __shared__ uint32_t s_bin[kBins];
+    __shared__ uint32_t s_bin[kBins + 1];
     __shared__ uint32_t s_st[X32 ? 257 : 1];
     uint32_t* const s_tmp = s_e + (kLocC - kLocTail);
     uint32_t* const s_red = s_tmp + kLWaves;
@@ -446,10 +422,16 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     else
                         kk[k] = ((khigh(p, bbits, L) - hs) << L) | klow(p, bbits, L);
                 }
+                // every key is at most the last bucket's with all low bits set
+                uint32_t khi;
+                if constexpr (X32)
+                    khi = (rlast | ((1u << L) - 1u)) - (rfirst & ~((1u << L) - 1u));
+                else
+                    khi = ((khigh(rlast, bbits, L) - hs) << L) | ((1u << L) - 1u);
                 fetch(s1, e1);
                 fetched = true;
-                ok = bin_sort<kLT, kLocItems>(kk, nk, S, s, hs, out_lim, out, s_e, s_bin, s_tmp,
-                                              s_red, bbits, L);
+                ok = bin_sort<kLT, kLocItems>(kk, nk, S, khi, s, hs, out_lim, out, s_e, s_bin,
+                                              s_tmp, s_red, bbits, L);
             }
             if (!ok && tid == 0) big[atomicAdd(big_ctr, 1u)] = j;
         }
@@ -488,7 +470,7 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(const void* Xv, uint2*
     constexpr int NW = kBigT / 64;
     constexpr uint32_t kChunk = kBigT * kGlobItems;
     // one LDS block, 80 KB: keys | bins (the LSD path's counters reuse it)
-    __shared__ uint32_t s_mem[kBigC + kBins];
+    __shared__ uint32_t s_mem[kBigC + kBins + 1];
     __shared__ uint32_t s_st[X32 ? 257 : 1];
     uint32_t* const s_e = s_mem;
     uint32_t* const s_bin = s_mem + kBigC;
@@ -521,7 +503,8 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(const void* Xv, uint2*
             uint32_t de = 0;
             if constexpr (X32) de = d2_of(s_st, e - 1);
             const uint32_t hs = IT::high(Xin[s], ds, bbits, hb, L);
-            if (IT::high(Xin[e - 1], de, bbits, hb, L) - hs < (1u << (32 - L))) {
+            const uint32_t hspan = IT::high(Xin[e - 1], de, bbits, hb, L) - hs;
+            if (hspan < (1u << (32 - L))) {
                 const uint32_t nk = (S + kBigT - 1) / kBigT;
                 uint32_t kk[kBigItems], d = ds;
 #pragma unroll
@@ -533,8 +516,8 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(const void* Xv, uint2*
                         while (d < 255u && s + q >= s_st[d + 1]) ++d;
                     kk[k] = ((IT::high(p, d, bbits, hb, L) - hs) << L) | IT::low(p, bbits, L);
                 }
-                if (bin_sort<kBigT, kBigItems>(kk, nk, S, s, hs, out_lim, out, s_e, s_bin, s_tmp,
-                                               s_red, bbits, L))
+                if (bin_sort<kBigT, kBigItems>(kk, nk, S, (hspan << L) | lowmask, s, hs, out_lim,
+                                               out, s_e, s_bin, s_tmp, s_red, bbits, L))
                     continue;
             }
         }
