@@ -95,11 +95,13 @@ __device__ __forceinline__ uint32_t klow(uint2 p, int bbits, int L) {
     return L >= bbits ? (((p.x & ((1u << (L - bbits)) - 1u)) << bbits) | p.y)
                       : (p.y & ((1u << L) - 1u));
 }
-// The pair of key (h << L) | lo.
-__device__ __forceinline__ uint2 kpair(uint32_t h, uint32_t lo, int bbits, int L) {
-    if (L >= bbits)
-        return make_uint2((h << (L - bbits)) | (lo >> bbits), lo & ((1u << bbits) - 1u));
-    return make_uint2(h >> (bbits - L), ((h & ((1u << (bbits - L)) - 1u)) << L) | lo);
+// The pair of key kbase + x: (key >> bbits, key's low bbits bits).  The key
+// has at most 2 bbits <= 62 bits, so key >> bbits fits 32 bits and is the
+// low word of the 64-bit shift (alignbit).
+__device__ __forceinline__ uint2 kpair_at(unsigned long long kbase, uint32_t x, int bbits) {
+    const unsigned long long k = kbase + x;
+    return make_uint2(__builtin_amdgcn_alignbit((uint32_t)(k >> 32), (uint32_t)k, (uint32_t)bbits),
+                      (uint32_t)k & ((1u << bbits) - 1u));
 }
 
 // 32-bit items: after a two-pass sort of keys of at most 40 bits whose
@@ -122,6 +124,15 @@ __device__ __forceinline__ uint32_t d2_of(const uint32_t* st, uint32_t q) {
         else hi = mid;
     }
     return lo;
+}
+// d2_of for a q that every lane of a full wave holds: lane l compares
+// st[4l .. 4l + 3] with q (one 16-byte LDS read; st 16-byte aligned), and the
+// digit is the number of d in [0, 256) with st[d] <= q, less one (st[0] = 0).
+__device__ __forceinline__ uint32_t d2_of_wave(const uint32_t* st, uint32_t q) {
+    const uint4 v = reinterpret_cast<const uint4*>(st)[lane_id()];
+    return (uint32_t)(__builtin_popcountll(ballot(v.x <= q)) + __builtin_popcountll(ballot(v.y <= q)) +
+                      __builtin_popcountll(ballot(v.z <= q)) + __builtin_popcountll(ballot(v.w <= q))) -
+           1u;
 }
 // key >> L and key & (2^L - 1) of an item with second digit d (L <= 24).
 __device__ __forceinline__ uint32_t ihigh(uint32_t v, uint32_t d, int hb, int L) {
@@ -251,7 +262,7 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
     constexpr int NW = LT / 64;
     constexpr uint32_t BPT = kBins / LT;  // bins per thread
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const uint32_t lowmask = (1u << L) - 1u;
+    const unsigned long long kbase = (unsigned long long)hs << L;
     for (uint32_t i = tid; i < kBins; i += LT) s_bin[i] = 0u;
     // bins: the kBinBits bits below khi's top bit hb (x >> sd < kBins)
     const uint32_t hb = khi ? 31u - (uint32_t)__clz(khi) : 0u;
@@ -308,7 +319,8 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
             const uint32_t y0 = s_e[i], y1 = s_e[i + 1u];
             rank += (y0 < x ? 1u : 0u) + (i + 1u < be && y1 < x ? 1u : 0u);
         }
-        if (bs + rank < rlim) out[s + bs + rank] = kpair(hs + (x >> L), x & lowmask, bbits, L);
+        // the key is (hs + (x >> L)) << L | (x's low L bits) = (hs << L) + x
+        if (bs + rank < rlim) out[s + bs + rank] = kpair_at(kbase, x, bbits);
     }
     __syncthreads();
     return true;
@@ -337,7 +349,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // and ranked.
     __shared__ uint32_t s_e[kLocC];
     __shared__ uint32_t s_bin[kBins + 1];
-    __shared__ uint32_t s_st[X32 ? 257 : 1];
+    __shared__ __align__(16) uint32_t s_st[X32 ? 260 : 1];
     uint32_t* const s_tmp = s_e + (kLocC - kLocTail);
     uint32_t* const s_red = s_tmp + kLWaves;
     constexpr uint32_t kCap = kLocC - kLocTail;
@@ -398,9 +410,9 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             uint32_t ds = 0, hs = 0;
             bool ok = false;
             if constexpr (X32) {
-                ds = d2_of(s_st, s);
+                ds = d2_of_wave(s_st, s);  // s, e: the same in every lane
                 hs = ihigh(rfirst, ds, hb, L);
-                ok = S <= kCap && d2_of(s_st, e - 1) == ds;
+                ok = S <= kCap && d2_of_wave(s_st, e - 1) == ds;
             } else {
                 hs = khigh(rfirst, bbits, L);
                 ok = fits32 && S <= kCap && khigh(rlast, bbits, L) - hs < (1u << (32 - L));
