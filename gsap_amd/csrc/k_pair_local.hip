@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const void* __restrict__ Pv
                                                      uint32_t* __restrict__ wstart,
                                                      uint32_t* __restrict__ big_ctr,
                                                      const uint32_t* __restrict__ st_g) {
-    __shared__ uint32_t s_st[X32 ? 257 : 1];
+    __shared__ __align__(16) uint32_t s_st[X32 ? 260 : 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) *big_ctr = 0u;  // read by the kernels after this one
     if constexpr (X32) load_st<kWG>(st_g, s_st);
     const uint2* const P = reinterpret_cast<const uint2*>(Pv);
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kWG) void k_pair_wstart(const void* __restrict__ Pv
         } else if (t > 0) {
             unsigned long long h0 = 0, qend = n;
             if constexpr (X32) {
-                const uint32_t d0 = d2_of(s_st, (uint32_t)(t - 1));
+                const uint32_t d0 = d2_of_wave(s_st, (uint32_t)(t - 1));  // t: per wave
                 if (s_st[d0 + 1] < qend) qend = s_st[d0 + 1];
                 h0 = P32[t - 1] >> L;
             } else {
