@@ -259,11 +259,11 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
                                          unsigned long long out_lim,
                                          uint2* __restrict__ out, uint32_t* s_e, uint32_t* s_bin,
                                          uint32_t* s_tmp, uint32_t* s_red, int bbits, int L) {
-    constexpr int NW = LT / 64;
     constexpr uint32_t BPT = kBins / LT;  // bins per thread
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     const unsigned long long kbase = (unsigned long long)hs << L;
     for (uint32_t i = tid; i < kBins; i += LT) s_bin[i] = 0u;
+    if (tid == 0) s_red[0] = 0u;  // set by the threads that see a bin over kBinMax
     // bins: the kBinBits bits below khi's top bit hb (x >> sd < kBins)
     const uint32_t hb = khi ? 31u - (uint32_t)__clz(khi) : 0u;
     const uint32_t sd = hb >= kBinBits - 1u ? hb - (kBinBits - 1u) : 0u;
@@ -282,8 +282,7 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
         sum += cb[i];
         mx = cb[i] > mx ? cb[i] : mx;
     }
-    mx = wave_max_u32(mx);
-    if (lane == 0) s_red[2 * NW + wave] = mx;
+    if (mx > kBinMax) s_red[0] = 1u;
     uint32_t run = lblock_excl_scan<LT>(sum, s_tmp, &all);  // every count read before the writes
 #pragma unroll
     for (uint32_t i = 0; i < BPT; ++i) {
@@ -291,11 +290,9 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
         run += cb[i];
     }
     if (tid == 0) s_bin[0] = 0u;
-    mx = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) mx = s_red[2 * NW + w] > mx ? s_red[2 * NW + w] : mx;
+    const bool over = s_red[0] != 0u;  // read before the barrier: the next call clears it after
     __syncthreads();
-    if (mx > kBinMax) return false;  // uniform; s_e and s_bin are free again
+    if (over) return false;  // uniform; s_e and s_bin are free again
     // keys to their bins (any order inside a bin); a bin's counter ends at
     // the next bin's start
 #pragma unroll
