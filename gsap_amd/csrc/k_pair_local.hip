@@ -38,7 +38,9 @@ constexpr uint32_t kLocTail = 64;      // LDS words of the key array used for re
 constexpr int kGlobItems = 8;           // keys per thread per chunk of the global path
 
 // Exclusive scan over the LT threads of a workgroup (s_tmp: LT / 64 words).
-template <int LT>
+// Without TAIL_SYNC it ends without the barrier that lets s_tmp be written
+// again at once: for callers that pass another barrier first.
+template <int LT, bool TAIL_SYNC = true>
 __device__ __forceinline__ uint32_t lblock_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
     constexpr int NW = LT / 64;
     const uint32_t l = lane_id(), w = threadIdx.x >> 6;
@@ -52,7 +54,7 @@ __device__ __forceinline__ uint32_t lblock_excl_scan(uint32_t v, uint32_t* s_tmp
         off += i < (int)w ? t : 0u;
         tot += t;
     }
-    __syncthreads();
+    if constexpr (TAIL_SYNC) __syncthreads();
     *total = tot;
     return off + inc - v;
 }
@@ -283,7 +285,9 @@ __device__ __forceinline__ bool bin_sort(const uint32_t (&kk)[ITEMS], uint32_t n
         mx = cb[i] > mx ? cb[i] : mx;
     }
     if (mx > kBinMax) s_red[0] = 1u;
-    uint32_t run = lblock_excl_scan<LT>(sum, s_tmp, &all);  // every count read before the writes
+    // its first barrier orders every count read (and flag write) before the
+    // writes below; s_tmp is next written after several more barriers
+    uint32_t run = lblock_excl_scan<LT, false>(sum, s_tmp, &all);
 #pragma unroll
     for (uint32_t i = 0; i < BPT; ++i) {
         s_bin[tid * BPT + i + 1u] = run;
