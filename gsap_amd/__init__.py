@@ -43,6 +43,7 @@ EXPORTS = [
     "gsap_mlp_get_params", "gsap_mlp_forward", "gsap_mlp_loss_grad", "gsap_mlp_train",
     "gsap_axis_candidates", "gsap_sap_set_learned_model", "gsap_sap_features_batched",
     "gsap_sap_axis_tests_batched", "gsap_sap_get_tune", "gsap_sap_set_tune", "gsap_sap_autotune",
+    "gsap_sap_query_workspace", "gsap_sap_set_workspace",
 ]
 N_FEATURES, N_PARAMS, N_CANDIDATES = 64, 25348, 64
 
@@ -90,7 +91,8 @@ class PrbPtrs(ctypes.Structure):
         "rb_pos", "rb_quat", "rb_linvel", "rb_angvel", "rb_inv_inertia", "p_pos", "p_vel",
         "p_force", "p_mass", "p_body", "pairs", "pair_entries")] + [
         ("num_bodies", ctypes.c_int32), ("num_particles", ctypes.c_int32),
-        ("num_pairs", ctypes.c_int64), ("num_pair_entries", ctypes.c_int64)]
+        ("num_pairs", ctypes.c_int64), ("num_pair_entries", ctypes.c_int64),
+        ("rb_shape_idx", ctypes.c_void_p), ("rb_num_particles", ctypes.c_void_p)]
 
 
 _lib = None
@@ -153,6 +155,8 @@ def lib():
         "gsap_sap_get_tune": (i32, [P, P]),
         "gsap_sap_set_tune": (i32, [P, P]),
         "gsap_sap_autotune": (i32, [P, P, P, u32, P, P, u32, ctypes.c_char_p, P]),
+        "gsap_sap_query_workspace": (i32, [P, u32, u64, P]),
+        "gsap_sap_set_workspace": (i32, [P, P, u64]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -247,8 +251,9 @@ class Sap:
         rc = lib().gsap_sap_collide(self.h, _stream(stream), _ptr(boxes), n, _ptr(group),
                                     _ptr(pairs), cap, _ptr(entries), ctypes.byref(npairs),
                                     ctypes.byref(nent), ctypes.byref(st))
-        if rc == ERR_CAPACITY:
-            raise GsapError(rc, f"capacity {cap} < {npairs.value}")
+        if rc == ERR_CAPACITY:  # the pair list or the caller's workspace is too small
+            raise GsapError(rc, f"{lib().gsap_last_error().decode()} (pair capacity {cap}, "
+                                f"pairs reported {npairs.value})")
         check(rc)
         return int(npairs.value), int(nent.value), st
 
@@ -317,6 +322,20 @@ class Sap:
             None if cache is None else os.fsencode(cache), ctypes.byref(t)))
         return t.as_dict()
 
+    def query_workspace(self, n, pair_capacity=0):
+        """Device bytes an update of n objects with room for pair_capacity
+        pairs needs under the current configuration."""
+        b = ctypes.c_uint64(0)
+        check(lib().gsap_sap_query_workspace(self.h, n, pair_capacity, ctypes.byref(b)))
+        return int(b.value)
+
+    def set_workspace(self, ws):
+        """Carve every later update's scratch from the device tensor `ws`
+        (kept referenced here); None returns to internal allocation."""
+        self._ws = ws
+        nbytes = 0 if ws is None else ws.numel() * ws.element_size()
+        check(lib().gsap_sap_set_workspace(self.h, _ptr(ws), nbytes))
+
     def set_learned_model(self, model, conf_min=0.95):
         check(lib().gsap_sap_set_learned_model(self.h, model.h, conf_min))
 
@@ -355,7 +374,8 @@ class Sap:
             self.h, _stream(stream), _ptr(boxes), off.ctypes.data_as(ctypes.c_void_p), ns,
             _ptr(pairs), cap, _ptr(scene_pair_offsets), ctypes.byref(npairs), ctypes.byref(st))
         if rc == ERR_CAPACITY:
-            raise GsapError(rc, f"capacity {cap} < {npairs.value}")
+            raise GsapError(rc, f"{lib().gsap_last_error().decode()} (pair capacity {cap}, "
+                                f"pairs reported {npairs.value})")
         check(rc)
         return int(npairs.value), st
 

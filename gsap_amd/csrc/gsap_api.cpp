@@ -29,13 +29,23 @@ thread_local std::string g_err;
 struct Buf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = false;  // allocated here (false: carved from a caller workspace)
 };
 
-hipError_t grow(Buf& b, size_t bytes) {
-    if (bytes <= b.bytes && b.p) return hipSuccess;
-    if (b.p) (void)hipFree(b.p);
+void release(Buf& b) {
+    if (b.p && b.owned) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+    b.owned = false;
+}
+
+// Grows b to at least `bytes`; contents are not kept.  *gen is bumped when
+// the buffer moves, so anything that captured the old address (a recorded
+// graph) can tell it is stale.
+hipError_t grow(Buf& b, size_t bytes, uint64_t* gen) {
+    if (bytes <= b.bytes && b.p) return hipSuccess;
+    release(b);
+    ++*gen;
     size_t want = bytes < 256 ? 256 : bytes;
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
@@ -43,13 +53,8 @@ hipError_t grow(Buf& b, size_t bytes) {
         return e;
     }
     b.bytes = want;
+    b.owned = true;
     return hipSuccess;
-}
-
-void release(Buf& b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
 }
 
 uint32_t bits_for(uint64_t n) {
@@ -96,7 +101,19 @@ struct gsap_sap {
     hipEvent_t ev[kEv] = {};
     int nev = 0;
     float stage_ms[kEv] = {};
+    // Buffer generation: bumped whenever a buffer an update uses moves (grown,
+    // re-carved) or a scene table is rewritten.  The captured batched graph
+    // holds raw buffer addresses and is replayed only at the generation it was
+    // recorded at.
+    uint64_t buf_gen = 1;
+    // caller-provided workspace (gsap_sap_set_workspace): the update buffers
+    // are carved from it instead of allocated; ws_size[] is the current carve
+    void* ws_base = nullptr;
+    uint64_t ws_bytes = 0;
+    size_t ws_size[32] = {};
+    bool ws_carved = false;
     // batched graph cache
+    uint64_t g_gen = 0;
     hipGraphExec_t gexec = nullptr;
     hipStream_t cap_stream = nullptr;
     std::vector<uint32_t> g_offsets;
@@ -158,9 +175,9 @@ extern "C" int32_t gsap_sap_create(const gsap_sap_desc* desc, gsap_sap** out) {
     if (desc->sort_items) h->items = desc->sort_items;
     if (desc->digit_bits) h->obj_bits = h->pair_bits = desc->digit_bits;
     h->slabs = (desc->flags & GSAP_FLAG_SLABS) ? 1 : 0;
-    hipError_t e = grow(h->ctrl, sizeof(Ctrl));
+    hipError_t e = grow(h->ctrl, sizeof(Ctrl), &h->buf_gen);
     if (e == hipSuccess) e = hipMemset(h->ctrl.p, 0, sizeof(Ctrl));
-    if (e == hipSuccess) e = grow(h->counts, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = grow(h->counts, 8 * sizeof(unsigned long long), &h->buf_gen);
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_counts, 8 * sizeof(unsigned long long), 0);
     // stage events only time the work (read after the stream is synchronised),
     // so they skip the system-scope fence (an L2 write-back on every record)
@@ -204,58 +221,99 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
 
 static bool slab_mode(const gsap_sap* h) { return h->slabs != 0; }
 
-// Object workspace for n objects, nseg scenes, ntiles tiles.  In slab mode
-// the sort, gather and sweep buffers hold two entries per object.
-static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntiles) {
-    const uint32_t em = slab_mode(h) ? 2 : 1;
-    if (em > h->cap_em) {  // sort buffers sized for one entry per object: regrow
-        h->cap_n = 0;
-        h->cap_tiles = 0;
+// ---------------------------------------------------------------- workspace
+// The device buffers of one update (besides the caller's input and output,
+// the control block and the counts), in carve order.
+enum WsId : int {
+    W_KEYS0, W_KEYS1, W_VALS0, W_VALS1, W_REC, W_GRP, W_GRP_END, W_ROWSTART, W_HAS,
+    W_BLOCK_SUMS, W_PARTIALS, W_LB_OBJ, W_VR_PARTIALS, W_TILE_SEG, W_TILE_SUMS, W_AXIS,
+    W_HIST_OBJ, W_SEG_START, W_SEG_N, W_SEG_TILE0, W_SEG_MOM, W_FEATS, W_HIST_PAIR, W_AXIS2,
+    W_SLABP, W_SLAB_CNT, W_PTMP0, W_PTMP1, W_WSTART, W_LB_PAIR, W_COUNT
+};
+static_assert(W_COUNT <= 32, "gsap_sap::ws_size holds 32 entries");
+
+static Buf* ws_buf(gsap_sap* h, int id) {
+    switch (id) {
+        case W_KEYS0: return &h->keys[0];
+        case W_KEYS1: return &h->keys[1];
+        case W_VALS0: return &h->vals[0];
+        case W_VALS1: return &h->vals[1];
+        case W_REC: return &h->rec;
+        case W_GRP: return &h->grp;
+        case W_GRP_END: return &h->grp_end;
+        case W_ROWSTART: return &h->rowstart;
+        case W_HAS: return &h->has;
+        case W_BLOCK_SUMS: return &h->block_sums;
+        case W_PARTIALS: return &h->partials;
+        case W_LB_OBJ: return &h->lb_obj;
+        case W_VR_PARTIALS: return &h->vr_partials;
+        case W_TILE_SEG: return &h->tile_seg;
+        case W_TILE_SUMS: return &h->tile_sums;
+        case W_AXIS: return &h->axis;
+        case W_HIST_OBJ: return &h->hist_obj;
+        case W_SEG_START: return &h->seg_start;
+        case W_SEG_N: return &h->seg_n;
+        case W_SEG_TILE0: return &h->seg_tile0;
+        case W_SEG_MOM: return &h->seg_mom;
+        case W_FEATS: return &h->feats;
+        case W_HIST_PAIR: return &h->hist_pair;
+        case W_AXIS2: return &h->axis2;
+        case W_SLABP: return &h->slabp;
+        case W_SLAB_CNT: return &h->slab_cnt;
+        case W_PTMP0: return &h->ptmp[0];
+        case W_PTMP1: return &h->ptmp[1];
+        case W_WSTART: return &h->wstart;
+        default: return &h->lb_pair;
     }
-    if (n > h->cap_n) {
-        uint32_t want = std::max(n, h->desc.max_n);
-        for (int i = 0; i < 2; ++i) {
-            HIPCHK(h, grow(h->keys[i], (size_t)want * 4 * em));
-            HIPCHK(h, grow(h->vals[i], (size_t)want * 4 * em));
-        }
-        HIPCHK(h, grow(h->rec, (size_t)want * 32 * em));
-        HIPCHK(h, grow(h->grp, (size_t)want * 32 * em));
-        HIPCHK(h, grow(h->grp_end, (size_t)want * 4 * em));
-        HIPCHK(h, grow(h->rowstart, (size_t)want * 4));
-        HIPCHK(h, grow(h->has, (size_t)want * 4));
-        HIPCHK(h, hipMemset(h->has.p, 0, (size_t)want * 4));
-        HIPCHK(h, grow(h->block_sums, ((size_t)want / 2048 + 2) * 4));
-        h->cap_n = want;
-        h->cap_em = std::max(h->cap_em, em);
-    }
-    if (ntiles > h->cap_tiles) {
-        HIPCHK(h, grow(h->partials, (size_t)ntiles * kMomStride * sizeof(double)));
-        HIPCHK(h, grow(h->lb_obj, ((size_t)ntiles * em + 1) * 256 * 8));
-        HIPCHK(h, hipMemset(h->lb_obj.p, 0, ((size_t)ntiles * em + 1) * 256 * 8));
-        if (slab_mode(h)) HIPCHK(h, grow(h->vr_partials, (size_t)ntiles * kVr * 4));
-        HIPCHK(h, grow(h->tile_seg, (size_t)ntiles * 4));
-        HIPCHK(h, grow(h->tile_sums, (size_t)ntiles * kFeatSums * 4));
-        h->cap_tiles = ntiles;
-    }
-    if (nseg > h->cap_seg) {
-        HIPCHK(h, grow(h->axis, (size_t)nseg * 16));
-        HIPCHK(h, grow(h->hist_obj, (size_t)nseg * kMaxPasses * 256 * 4));
-        HIPCHK(h, grow(h->seg_start, (size_t)nseg * 4));
-        HIPCHK(h, grow(h->seg_n, (size_t)nseg * 4));
-        HIPCHK(h, grow(h->seg_tile0, (size_t)(nseg + 1) * 4));
-        HIPCHK(h, grow(h->seg_mom, (size_t)nseg * 9 * 8));
-        HIPCHK(h, grow(h->feats, (size_t)nseg * kFeat * 4));
-        h->cap_seg = nseg;
-    }
+}
+
+// Buffers that must start zeroed whenever their memory is (re)assigned:
+// `has` (every row-start pass leaves it zeroed again) and the look-back
+// arrays (their words are tagged with the handle's update epoch, so reused
+// memory must not hold a tag an update could match).
+static bool ws_zero_init(int id) { return id == W_HAS || id == W_LB_OBJ || id == W_LB_PAIR; }
+
+static size_t ws_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct WsReq {
+    size_t b[W_COUNT] = {};
+};
+
+static size_t ws_total(const size_t* b) {
+    size_t t = 0;
+    for (int i = 0; i < W_COUNT; ++i) t += ws_align(b[i]);
+    return t;
+}
+
+// Object side of an update of n objects in nseg scenes over ntiles sort
+// tiles.  In slab mode the sort, gather and sweep buffers hold two entries
+// per object.
+static void req_obj(const gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntiles, WsReq& r) {
+    const size_t em = slab_mode(h) ? 2 : 1;
+    const size_t want = std::max(n, h->desc.max_n);
+    r.b[W_KEYS0] = r.b[W_KEYS1] = r.b[W_VALS0] = r.b[W_VALS1] = want * 4 * em;
+    r.b[W_REC] = r.b[W_GRP] = want * 32 * em;
+    r.b[W_GRP_END] = want * 4 * em;
+    r.b[W_ROWSTART] = r.b[W_HAS] = want * 4;
+    r.b[W_BLOCK_SUMS] = (want / 2048 + 2) * 4;
+    r.b[W_PARTIALS] = (size_t)ntiles * kMomStride * sizeof(double);
+    r.b[W_LB_OBJ] = ((size_t)ntiles * em + 1) * 256 * 8;
+    r.b[W_VR_PARTIALS] = slab_mode(h) ? (size_t)ntiles * kVr * 4 : 0;
+    r.b[W_TILE_SEG] = (size_t)ntiles * 4;
+    r.b[W_TILE_SUMS] = (size_t)ntiles * kFeatSums * 4;
+    r.b[W_AXIS] = (size_t)nseg * 16;
+    r.b[W_HIST_OBJ] = (size_t)nseg * kMaxPasses * 256 * 4;
+    r.b[W_SEG_START] = r.b[W_SEG_N] = (size_t)nseg * 4;
+    r.b[W_SEG_TILE0] = (size_t)(nseg + 1) * 4;
+    r.b[W_SEG_MOM] = (size_t)nseg * 9 * 8;
+    r.b[W_FEATS] = (size_t)nseg * kFeat * 4;
     // pair digit histograms, then the first pass's per-region counts and slot counters
-    if (!h->hist_pair.p)
-        HIPCHK(h, grow(h->hist_pair, (kMaxPasses * 256 + kPairClaimWords + kPairDigitStartWords) * 4));
-    if (slab_mode(h) && !h->slabp.p) {
-        HIPCHK(h, grow(h->axis2, 4 * sizeof(float)));
-        HIPCHK(h, grow(h->slabp, kSlabParams * sizeof(float)));
-        HIPCHK(h, grow(h->slab_cnt, 256 * sizeof(uint32_t)));
+    r.b[W_HIST_PAIR] = (kMaxPasses * 256 + kPairClaimWords + kPairDigitStartWords) * 4;
+    if (slab_mode(h)) {
+        r.b[W_AXIS2] = 4 * sizeof(float);
+        r.b[W_SLABP] = kSlabParams * sizeof(float);
+        r.b[W_SLAB_CNT] = 256 * sizeof(uint32_t);
     }
-    return GSAP_OK;
 }
 
 // Internal pair buffers.  The sweep writes the pairs it finds densely into
@@ -263,9 +321,7 @@ static int32_t ensure_obj(gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
 // capacity; the sort reads them as one array (regions padded to a whole tile)
 // and ping-pongs with the second buffer, which holds the padded total.
 static constexpr uint64_t kPairTileMax = 256 * 16;  // largest pair-sort tile
-static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg) {
-    (void)n;
-    (void)nseg;
+static int32_t req_pairs(gsap_sap* h, uint64_t cap, WsReq& r) {
     const uint64_t rcap = cap > 0 ? cap : 1;
     // pair positions are 32-bit; the margin keeps every sort index (padding,
     // unrolled histogram rounds) below 2^32
@@ -274,18 +330,82 @@ static int32_t ensure_pairs(gsap_sap* h, uint64_t cap, uint32_t n, uint32_t nseg
     const uint64_t nreg = (uint64_t)h->pair_regions;
     const uint64_t stride = (rcap + kPairTileMax - 1) / kPairTileMax * kPairTileMax + kPairTileMax;
     const uint64_t sorted = rcap + (nreg + 1) * kPairTileMax;  // padded total, upper bound
-    HIPCHK(h, grow(h->ptmp[0], (size_t)(nreg * stride) * 8));
-    HIPCHK(h, grow(h->ptmp[1], (size_t)sorted * 8));
-    HIPCHK(h, grow(h->wstart, (size_t)pair_local_wstart_words(rcap) * 4));
+    const uint64_t tiles = (sorted + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
+    r.b[W_PTMP0] = (size_t)(nreg * stride) * 8;
+    r.b[W_PTMP1] = (size_t)sorted * 8;
+    r.b[W_WSTART] = (size_t)pair_local_wstart_words(rcap) * 4;
+    r.b[W_LB_PAIR] = (size_t)tiles * 256 * 8;
     h->slot_cap = rcap;
     h->region_stride = stride;
     h->sort_cap = sorted;
-    const uint64_t tiles = (sorted + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
-    if (tiles > h->cap_pair_tiles) {
-        HIPCHK(h, grow(h->lb_pair, (size_t)tiles * 256 * 8));
-        HIPCHK(h, hipMemset(h->lb_pair.p, 0, (size_t)tiles * 256 * 8));
-        h->cap_pair_tiles = tiles;
+    return GSAP_OK;
+}
+
+// Zero the zero-initialised buffers of the current carve on st (a fill
+// kernel, so a caller may record the call into a graph).
+static int32_t ws_clear(gsap_sap* h, hipStream_t st) {
+    for (int id = 0; id < W_COUNT; ++id) {
+        Buf* b = ws_buf(h, id);
+        if (ws_zero_init(id) && b->p && b->bytes)
+            HIPCHK(h, launch_zero_u64((unsigned long long*)b->p, (uint32_t)((b->bytes + 7) / 8), st));
     }
+    return GSAP_OK;
+}
+
+// Assigns memory to every buffer `base` points into, from sizes sz[], and
+// clears the zero-initialised ones on st.
+static int32_t ws_carve(gsap_sap* h, char* base, const size_t* sz, hipStream_t st) {
+    size_t off = 0;
+    for (int id = 0; id < W_COUNT; ++id) {
+        Buf* b = ws_buf(h, id);
+        release(*b);
+        b->p = sz[id] ? base + off : nullptr;
+        b->bytes = sz[id];
+        h->ws_size[id] = sz[id];
+        off += ws_align(sz[id]);
+    }
+    ++h->buf_gen;
+    return ws_clear(h, st);
+}
+
+// Makes every buffer of req available.  Internal mode grows allocations
+// (they never shrink); with a caller workspace the current carve is kept when
+// it covers req, else the workspace is carved again for the union of the two
+// (or for req alone when the union does not fit).
+static int32_t ws_ensure(gsap_sap* h, const WsReq& req, hipStream_t st) {
+    if (!h->ws_base) {
+        for (int id = 0; id < W_COUNT; ++id) {
+            Buf* b = ws_buf(h, id);
+            if (req.b[id] == 0 || (b->p && req.b[id] <= b->bytes)) continue;
+            HIPCHK(h, grow(*b, req.b[id], &h->buf_gen));
+            if (ws_zero_init(id)) HIPCHK(h, hipMemset(b->p, 0, b->bytes));
+        }
+        return GSAP_OK;
+    }
+    bool covers = h->ws_carved;
+    size_t merged[W_COUNT];
+    for (int id = 0; id < W_COUNT; ++id) {
+        covers = covers && req.b[id] <= h->ws_size[id];
+        merged[id] = std::max(req.b[id], h->ws_carved ? h->ws_size[id] : (size_t)0);
+    }
+    if (covers) return GSAP_OK;
+    const size_t* use = ws_total(merged) <= h->ws_bytes ? merged : req.b;
+    const size_t need = ws_total(use);
+    if (need > h->ws_bytes) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "workspace of %llu bytes too small: %llu required",
+                 (unsigned long long)h->ws_bytes, (unsigned long long)ws_total(req.b));
+        return fail(h, GSAP_ERR_CAPACITY, buf);
+    }
+    h->ws_carved = true;
+    return ws_carve(h, (char*)h->ws_base, use, st);
+}
+
+// Requirement of a single-scene update of n objects (pairs: cap > 0).
+static int32_t req_single(gsap_sap* h, uint32_t n, uint64_t cap, bool count_only, WsReq& r) {
+    const uint32_t tile = sort_tile(h->items);
+    req_obj(h, n, 1, (n + tile - 1) / tile, r);
+    if (!count_only) return req_pairs(h, cap, r);
     return GSAP_OK;
 }
 
@@ -535,12 +655,9 @@ extern "C" int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void*
     }
     const uint32_t tile = sort_tile(h->items);
     const uint32_t ntiles = (n + tile - 1) / tile;
-    rc = ensure_obj(h, n, 1, ntiles);
-    if (rc) return rc;
-    if (!count_only) {
-        rc = ensure_pairs(h, pair_capacity, n, 1);
-        if (rc) return rc;
-    }
+    WsReq req;
+    if ((rc = req_single(h, n, pair_capacity, count_only, req))) return rc;
+    if ((rc = ws_ensure(h, req, st))) return rc;
     SegTable seg{};
     seg.nseg = 1;
     return enqueue_update(h, st, d_in, n, seg, ntiles, d_group, d_pairs, pair_capacity, d_entries,
@@ -606,30 +723,95 @@ extern "C" int32_t gsap_sap_collide(gsap_sap* h, void* stream, const void* d_in,
     return read_counts(h, st, pair_capacity, count_only, h_num_pairs, h_num_entries, stats, n);
 }
 
+extern "C" int32_t gsap_sap_query_workspace(gsap_sap* h, uint32_t n, uint64_t pair_capacity,
+                                            uint64_t* bytes) {
+    if (!h || !bytes) return fail(h, GSAP_ERR_INVALID_ARG, "null argument");
+    if (n > 0x7FFFFFFFu) return fail(h, GSAP_ERR_INVALID_ARG, "n exceeds 2^31-1");
+    WsReq req;
+    // req_pairs records the layout of that capacity in the handle; keep the
+    // layout of the last update
+    const uint64_t sc = h->slot_cap, rs = h->region_stride, so = h->sort_cap;
+    const int32_t rc = req_single(h, n, pair_capacity, pair_capacity == 0, req);
+    h->slot_cap = sc;
+    h->region_stride = rs;
+    h->sort_cap = so;
+    if (rc) return rc;
+    *bytes = ws_total(req.b);
+    return GSAP_OK;
+}
+
+extern "C" int32_t gsap_sap_set_workspace(gsap_sap* h, void* d_ws, uint64_t bytes) {
+    if (!h) return fail(nullptr, GSAP_ERR_INVALID_ARG, "null handle");
+    if (d_ws && ((uintptr_t)d_ws & 255u))
+        return fail(h, GSAP_ERR_INVALID_ARG, "workspace must be 256-byte aligned");
+    if (d_ws && bytes == 0) return fail(h, GSAP_ERR_INVALID_ARG, "workspace of 0 bytes");
+    // updates still in flight may use the current buffers
+    HIPCHK(h, hipDeviceSynchronize());
+    for (int id = 0; id < W_COUNT; ++id) {
+        release(*ws_buf(h, id));
+        h->ws_size[id] = 0;
+    }
+    h->ws_base = d_ws;
+    h->ws_bytes = d_ws ? bytes : 0;
+    h->ws_carved = false;
+    ++h->buf_gen;
+    return GSAP_OK;
+}
+
 // Scene tables for the batched path: tiles never straddle scenes.
-static int32_t upload_scenes(gsap_sap* h, hipStream_t st, const uint32_t* off, uint32_t ns,
-                             uint32_t* ntiles_out) {
+struct SceneTables {
+    std::vector<uint32_t> start, cnt, t0, tseg;
+    uint32_t ntiles = 0;
+};
+
+static int32_t scene_tables(gsap_sap* h, const uint32_t* off, uint32_t ns, SceneTables& t) {
     const uint32_t tile = sort_tile(h->items);
-    std::vector<uint32_t> start(ns), cnt(ns), t0(ns + 1), tseg;
+    t.start.resize(ns);
+    t.cnt.resize(ns);
+    t.t0.resize(ns + 1);
+    t.tseg.clear();
     uint32_t nt = 0;
     for (uint32_t s = 0; s < ns; ++s) {
         if (off[s + 1] < off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
-        start[s] = off[s];
-        cnt[s] = off[s + 1] - off[s];
-        t0[s] = nt;
-        const uint32_t k = (cnt[s] + tile - 1) / tile;
-        for (uint32_t i = 0; i < k; ++i) tseg.push_back(s);
+        t.start[s] = off[s];
+        t.cnt[s] = off[s + 1] - off[s];
+        t.t0[s] = nt;
+        const uint32_t k = (t.cnt[s] + tile - 1) / tile;
+        for (uint32_t i = 0; i < k; ++i) t.tseg.push_back(s);
         nt += k;
     }
-    t0[ns] = nt;
-    int32_t rc = ensure_obj(h, off[ns], ns, std::max(nt, 1u));
+    t.t0[ns] = nt;
+    t.ntiles = nt;
+    return GSAP_OK;
+}
+
+// Writes the scene tables into the (already ensured) device buffers.  The
+// tables' contents change, so a captured batched update is stale.
+static int32_t upload_scenes(gsap_sap* h, const SceneTables& t) {
+    const size_t ns = t.start.size();
+    HIPCHK(h, hipMemcpy(h->seg_start.p, t.start.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->seg_n.p, t.cnt.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->seg_tile0.p, t.t0.data(), (ns + 1) * 4, hipMemcpyHostToDevice));
+    if (t.ntiles) HIPCHK(h, hipMemcpy(h->tile_seg.p, t.tseg.data(), (size_t)t.ntiles * 4, hipMemcpyHostToDevice));
+    ++h->buf_gen;
+    return GSAP_OK;
+}
+
+// Scene tables of `off` on the device with the object buffers (and, when
+// cap > 0, the pair buffers) of that batch ensured in the same step.
+static int32_t prepare_scenes(gsap_sap* h, hipStream_t st, const uint32_t* off, uint32_t ns,
+                              uint64_t cap, bool with_pairs, uint32_t* ntiles_out) {
+    SceneTables t;
+    int32_t rc = scene_tables(h, off, ns, t);
     if (rc) return rc;
-    HIPCHK(h, hipMemcpy(h->seg_start.p, start.data(), ns * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->seg_n.p, cnt.data(), ns * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->seg_tile0.p, t0.data(), (ns + 1) * 4, hipMemcpyHostToDevice));
-    if (nt) HIPCHK(h, hipMemcpy(h->tile_seg.p, tseg.data(), nt * 4, hipMemcpyHostToDevice));
-    (void)st;
-    *ntiles_out = nt;
+    WsReq req;
+    req_obj(h, off[ns], ns, std::max(t.ntiles, 1u), req);
+    if (with_pairs && (rc = req_pairs(h, cap, req))) return rc;
+    if ((rc = ws_ensure(h, req, st))) return rc;
+    // the tables are written by synchronous copies after the carve's clears
+    HIPCHK(h, hipStreamSynchronize(st));
+    if ((rc = upload_scenes(h, t))) return rc;
+    *ntiles_out = t.ntiles;
     return GSAP_OK;
 }
 
@@ -665,16 +847,12 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
                       h->g_scene_out == (unsigned long long*)d_scene_pair_offsets &&
                       h->g_offsets.size() == n_scenes + 1 &&
                       std::equal(h->g_offsets.begin(), h->g_offsets.end(), h_scene_offsets) &&
-                      !h->timing;
+                      h->g_gen == h->buf_gen && !h->timing;
     if (!same) {
         uint32_t nt = 0;
         HIPCHK(h, hipStreamSynchronize(st));
-        rc = upload_scenes(h, st, h_scene_offsets, n_scenes, &nt);
+        rc = prepare_scenes(h, st, h_scene_offsets, n_scenes, pair_capacity, !count_only, &nt);
         if (rc) return rc;
-        if (!count_only) {
-            rc = ensure_pairs(h, pair_capacity, n, n_scenes);
-            if (rc) return rc;
-        }
         SegTable seg{};
         seg.seg_start = (const uint32_t*)h->seg_start.p;
         seg.seg_n = (const uint32_t*)h->seg_n.p;
@@ -712,6 +890,7 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
             h->g_cap = pair_capacity;
             h->g_scene_out = (unsigned long long*)d_scene_pair_offsets;
             h->g_offsets.assign(h_scene_offsets, h_scene_offsets + n_scenes + 1);
+            h->g_gen = h->buf_gen;
             HIPCHK(h, hipGraphLaunch(h->gexec, st));
         }
     } else {
@@ -756,16 +935,17 @@ extern "C" int32_t gsap_radix_sort_create(uint32_t max_elements, int32_t keys_on
     *out = nullptr;
     if (gsap_device_count() <= 0) return rs_fail(nullptr, GSAP_ERR_HIP, "no HIP device");
     gsap_radix_sort* s = new gsap_radix_sort();
+    uint64_t gen = 0;  // the sorter captures no graph
     s->max_n = max_elements;
     s->keys_only = keys_only;
     const uint32_t tiles = (max_elements + sort_tile(8) - 1) / sort_tile(8) + 1;
-    hipError_t e = grow(s->tk, (size_t)max_elements * 4);
-    if (e == hipSuccess) e = grow(s->tv, (size_t)max_elements * 4);
-    if (e == hipSuccess) e = grow(s->dv, (size_t)max_elements * 4);
-    if (e == hipSuccess) e = grow(s->hist, kMaxPasses * 256 * 4);
-    if (e == hipSuccess) e = grow(s->lb, (size_t)tiles * 256 * 8);
+    hipError_t e = grow(s->tk, (size_t)max_elements * 4, &gen);
+    if (e == hipSuccess) e = grow(s->tv, (size_t)max_elements * 4, &gen);
+    if (e == hipSuccess) e = grow(s->dv, (size_t)max_elements * 4, &gen);
+    if (e == hipSuccess) e = grow(s->hist, kMaxPasses * 256 * 4, &gen);
+    if (e == hipSuccess) e = grow(s->lb, (size_t)tiles * 256 * 8, &gen);
     if (e == hipSuccess) e = hipMemset(s->lb.p, 0, (size_t)tiles * 256 * 8);
-    if (e == hipSuccess) e = grow(s->ctrl, sizeof(Ctrl));
+    if (e == hipSuccess) e = grow(s->ctrl, sizeof(Ctrl), &gen);
     if (e == hipSuccess) e = hipMemset(s->ctrl.p, 0, sizeof(Ctrl));
     if (e != hipSuccess) {
         std::string m = std::string("gsap_radix_sort_create: ") + hipGetErrorString(e);
@@ -840,9 +1020,9 @@ int32_t attach_model(gsap_sap* h, const uint16_t* d_wb, size_t wb_bytes, const f
                      size_t master_bytes, float conf_min) {
     if (!h) return fail(nullptr, GSAP_ERR_INVALID_ARG, "null handle");
     if (!(conf_min > 0.0f && conf_min <= 1.0f)) return fail(h, GSAP_ERR_INVALID_ARG, "conf_min in (0,1]");
-    HIPCHK(h, grow(h->mlp_wb, wb_bytes));
-    HIPCHK(h, grow(h->mlp_master, master_bytes));
-    HIPCHK(h, grow(h->cand, kCand * 3 * 4));
+    HIPCHK(h, grow(h->mlp_wb, wb_bytes, &h->buf_gen));
+    HIPCHK(h, grow(h->mlp_master, master_bytes, &h->buf_gen));
+    HIPCHK(h, grow(h->cand, kCand * 3 * 4, &h->buf_gen));
     HIPCHK(h, hipDeviceSynchronize());
     HIPCHK(h, hipMemcpy(h->mlp_wb.p, d_wb, wb_bytes, hipMemcpyDeviceToDevice));
     HIPCHK(h, hipMemcpy(h->mlp_master.p, d_master, master_bytes, hipMemcpyDeviceToDevice));
@@ -851,6 +1031,7 @@ int32_t attach_model(gsap_sap* h, const uint16_t* d_wb, size_t wb_bytes, const f
     HIPCHK(h, hipMemcpy(h->cand.p, cand, sizeof cand, hipMemcpyHostToDevice));
     h->conf_min = conf_min;
     h->has_model = true;
+    ++h->buf_gen;  // conf_min is a kernel argument of a captured batched update
     return GSAP_OK;
 }
 }  // namespace gsap_internal_api
@@ -875,7 +1056,7 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(h, hipStreamSynchronize(st));
     uint32_t nt = 0;
-    if (int32_t rc = upload_scenes(h, st, h_off, ns, &nt)) return rc;
+    if (int32_t rc = prepare_scenes(h, st, h_off, ns, 0, false, &nt)) return rc;
     ObjPipeArgs a{};
     a.in = d_in;
     a.kind = h->desc.input_kind;
@@ -919,8 +1100,8 @@ extern "C" int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const 
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(h, hipStreamSynchronize(st));
     uint32_t nt = 0;
-    if (int32_t rc = upload_scenes(h, st, h_off, ns, &nt)) return rc;
-    HIPCHK(h, grow(h->ax_tests_axes, (size_t)naxes * 12));
+    if (int32_t rc = prepare_scenes(h, st, h_off, ns, 0, false, &nt)) return rc;
+    HIPCHK(h, grow(h->ax_tests_axes, (size_t)naxes * 12, &h->buf_gen));
     HIPCHK(h, hipMemcpy(h->ax_tests_axes.p, h_axes, (size_t)naxes * 12, hipMemcpyHostToDevice));
     HIPCHK(h, launch_axis_tests(d_in, h->desc.input_kind, (const uint32_t*)h->seg_start.p,
                                 (const uint32_t*)h->seg_n.p, ns, (const float*)h->ax_tests_axes.p,
@@ -1120,6 +1301,13 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
         if (rc) return;
         float ms = 0.0f;
         rc = measure(t, &ms);
+        // with a caller workspace, a configuration that needs more than it
+        // holds is not a candidate (the pair count is exact, so this is the
+        // only source of a capacity failure here)
+        if (rc == GSAP_ERR_CAPACITY && h->ws_base) {
+            rc = GSAP_OK;
+            return;
+        }
         if (!rc && ms < best_ms) {
             best_ms = ms;
             best = t;

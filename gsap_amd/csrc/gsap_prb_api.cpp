@@ -330,6 +330,8 @@ extern "C" int32_t gsap_prb_get_ptrs(gsap_prb* h, gsap_prb_ptrs* o) {
     o->num_particles = h->d.np;
     o->num_pairs = (int64_t)c[0];
     o->num_pair_entries = (int64_t)c[1];
+    o->rb_shape_idx = h->d.shape;
+    o->rb_num_particles = h->d.npart;
     if (c[3] & 1ull) return pfail(GSAP_ERR_CAPACITY, "pair buffer overflow in the last broadphase");
     return GSAP_OK;
 }

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GSAP_ABI_VERSION 1
+#define GSAP_ABI_VERSION 2
 
 /* ------------------------------------------------------------------ status */
 enum {
@@ -146,6 +146,27 @@ int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const void* d_in,
                                  void* d_pairs, uint64_t pair_capacity,
                                  uint64_t* d_scene_pair_offsets, uint64_t* h_num_pairs,
                                  gsap_stats* stats);
+
+/* Workspace.  Sap::Sap (Sap.cpp:12-114) allocates its scratch once, sized by
+ * the object count.  Here the scratch of an update depends on n, the pair
+ * capacity and the launch configuration (gsap_tune), and is allocated and
+ * grown by the handle unless the caller provides it:
+ *   gsap_sap_query_workspace  bytes an update of n objects with room for
+ *                             pair_capacity pairs (0: count only) needs under
+ *                             the handle's current configuration
+ *   gsap_sap_set_workspace    use d_ws (device, 256-byte aligned, `bytes`
+ *                             long) for every later update's scratch instead
+ *                             of allocating; d_ws == NULL returns to internal
+ *                             allocation.  Internal buffers are freed.  The
+ *                             memory must stay valid until the handle is
+ *                             destroyed or another workspace is set.  An update
+ *                             that needs more returns GSAP_ERR_CAPACITY with
+ *                             the required size in the message.  Setting it
+ *                             synchronises the device.
+ * Batched updates need the scene tables of every scene in it as well; the
+ * learned model and the judge's axis table stay internal allocations. */
+int32_t gsap_sap_query_workspace(gsap_sap* h, uint32_t n, uint64_t pair_capacity, uint64_t* bytes);
+int32_t gsap_sap_set_workspace(gsap_sap* h, void* d_ws, uint64_t bytes);
 
 /* Per-stage timing of the last gsap_sap_collide with timing enabled (ms,
  * hipEvents on the launch stream).  Stages, in order: moments+project,
@@ -288,6 +309,8 @@ typedef struct gsap_prb_ptrs {
     int32_t* pair_entries;
     int32_t num_bodies, num_particles;
     int64_t num_pairs, num_pair_entries;
+    int32_t* rb_shape_idx;     /* shape of each body (getRbShapeIdx)        */
+    int32_t* rb_num_particles; /* particles of each body (getRbNumParticles) */
 } gsap_prb_ptrs;
 
 /* ParticleRigidBody(maxNumBodies, maxNumParticles, maxPairs) */
