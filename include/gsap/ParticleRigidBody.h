@@ -3,8 +3,9 @@
 // header-only over gsap_prb_* in gsap.h.  Method names are the reference's,
 // including comupteFoceOnParticles.  RigidBodyData holds HOST arrays (the
 // reference's setRbData copies them host->device, ParticleRigidBody.cpp:101-109);
-// quaternions are float4 (x,y,z,w) and inverse inertia 3 float4 rows per body.
-// No GL buffers are created.
+// quaternions are Quaternion (a float4 x,y,z,w) and inverse inertias
+// Matrix3x3 (3 float4 rows), the reference's types (gsap/quaternion.h,
+// gsap/matrix3x3.h).  No GL buffers are created.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -13,20 +14,32 @@
 
 #include "../gsap.h"
 #include "Sap.h"
+#include "matrix3x3.h"
+#include "quaternion.h"
+
+// Template slots per shape and shape count of the reference's fixed template
+// table (ParticleRigidBody.h:39-40); here only the default layout of
+// setParticleData, the table itself is sized at run time.
+#ifndef MAX_NUM_PARTICLES
+#define MAX_NUM_PARTICLES 10
+#endif
+#ifndef MAX_PARTICLE_SHAPE_TYPE
+#define MAX_PARTICLE_SHAPE_TYPE 4
+#endif
 
 class ParticleRigidBody {
 public:
     struct RigidBodyData {
         int m_nBodies;
         float4* m_pos;
-        float4* m_quat;
+        Quaternion* m_quat;
         float4* m_linVel;
         float4* m_angVel;
         int* m_shapeIdx;
         int* m_pBufferStartOffset;
         int* m_numParticles;
         float* m_invMass;
-        float4* m_invInertia;  // 3 rows per body
+        Matrix3x3* m_invInertia;
         int m_numTotalParticles;
     };
 
@@ -46,11 +59,12 @@ public:
     }
     // setParticleData(float4* pos, float* mass, int numElements); templates are
     // laid out maxPerShape slots per shape (reference: MAX_NUM_PARTICLES = 10).
-    void setParticleData(float4* pos, float* mass, int numElements, int maxPerShape = 10) {
+    void setParticleData(float4* pos, float* mass, int numElements,
+                         int maxPerShape = MAX_NUM_PARTICLES) {
         check(gsap_prb_set_particle_data(m_h, (const float*)pos, mass, numElements, maxPerShape));
     }
-    void setParticleShapeInvInertia(float4* invInertiaRows, int numShapes) {
-        check(gsap_prb_set_shape_inv_inertia(m_h, (const float*)invInertiaRows, numShapes));
+    void setParticleShapeInvInertia(Matrix3x3* invInertia, int numShapes) {
+        check(gsap_prb_set_shape_inv_inertia(m_h, (const float*)invInertia, numShapes));
     }
     void setParams(float spMultip, float dpMultip) { check(gsap_prb_set_params(m_h, spMultip, dpMultip)); }
     void setMaxVelocity(float minRadius, float dt) { check(gsap_prb_set_max_velocity(m_h, minRadius, dt)); }
@@ -75,12 +89,20 @@ public:
         check(gsap_prb_set_num_pairs(m_h, m_pendingPairs, numPairEntries));
     }
     float4* getRbPos() { refresh(); return (float4*)m_p.rb_pos; }
-    float4* getRbQuat() { refresh(); return (float4*)m_p.rb_quat; }
+    Quaternion* getRbQuat() { refresh(); return (Quaternion*)m_p.rb_quat; }
+    int* getRbShapeIdx() { refresh(); return m_p.rb_shape_idx; }
+    int* getRbNumParticles() { refresh(); return m_p.rb_num_particles; }
     int getNumParticles() { refresh(); return m_p.num_particles; }
     float4* getParticlePos() { refresh(); return (float4*)m_p.p_pos; }
     float4* getParticleVel() { refresh(); return (float4*)m_p.p_vel; }
     float4* getParticleForce() { refresh(); return (float4*)m_p.p_force; }
     int getNumBodies() { refresh(); return m_p.num_bodies; }
+
+    // Template index of particle idx of a body of shape shapeIdx in the
+    // default layout (ParticleRigidBody.h:143-144).
+    __host__ __device__ static int getParticleRefIdx(int shapeIdx, int idx) {
+        return MAX_NUM_PARTICLES * shapeIdx + idx;
+    }
 
     // Whole DemoParticles::step sequence (demoParticles.h:856-928) with the
     // broadphase on this object's particles through `sap`.

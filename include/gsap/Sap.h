@@ -38,6 +38,18 @@ public:
     Sap& operator=(const Sap&) = delete;
 
     void setPairCapacity(unsigned long long directedPairs) { m_capacity = directedPairs; }
+    // Scratch of one collision() at the current pair capacity, and carving it
+    // from caller memory instead of allocating (the reference's constructor
+    // allocates its scratch once, Sap.cpp:12-114).
+    unsigned long long workspaceSize() {
+        uint64_t b = 0;
+        m_status = gsap_sap_query_workspace(m_h, m_n, m_capacity, &b);
+        return b;
+    }
+    void setWorkspace(void* d_ws, unsigned long long bytes) {
+        m_status = gsap_sap_set_workspace(m_h, d_ws, bytes);
+        if (m_status != GSAP_OK) std::fprintf(stderr, "Sap::setWorkspace: %s\n", gsap_last_error());
+    }
     void setStream(hipStream_t s) { m_stream = s; }
     int lastStatus() const { return m_status; }
     gsap_sap* handle() { return m_h; }

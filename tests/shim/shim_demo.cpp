@@ -5,6 +5,7 @@
 //   in.bin : int32 n, then n float4 spheres (x y z r)
 //   out.bin: int32 NPairs, int32 NPEntries, NPairs int2, NPEntries int,
 //            then n uint32 keys sorted by RadixSort and their values
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -68,22 +69,107 @@ int main(int argc, char** argv) {
     HC(hipMemcpy(keys.data(), dk, 4ull * n, hipMemcpyDeviceToHost));
     HC(hipMemcpy(vals.data(), dv, 4ull * n, hipMemcpyDeviceToHost));
 
-    // ParticleRigidBody: two single-particle bodies, reference defaults
+    // The same collision inside a caller-provided workspace: identical output.
     {
-        ParticleRigidBody prb(2, 2, 16);
-        float4 pos[2] = {{0, 5, 0, 0}, {0.9f, 5, 0, 0}}, q[2] = {{0, 0, 0, 1}, {0, 0, 0, 1}};
-        float4 lv[2] = {{1, 0, 0, 0}, {-1, 0, 0, 0}}, av[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-        int shape[2] = {0, 0}, start[2] = {0, 1}, cnt[2] = {1, 1};
-        float im[2] = {1, 1};
-        ParticleRigidBody::RigidBodyData rb = {2, pos, q, lv, av, shape, start, cnt, im, nullptr, 2};
+        Sap ws_sap(n);
+        ws_sap.setPairCapacity(maxPairs);
+        const unsigned long long wsb = ws_sap.workspaceSize();
+        void* ws = nullptr;
+        HC(hipMalloc(&ws, wsb));
+        ws_sap.setWorkspace(ws, wsb);
+        if (ws_sap.lastStatus() != GSAP_OK) return 6;
+        int2* p2 = nullptr;
+        int* e2 = nullptr;
+        HC(hipMalloc(&p2, 8ull * maxPairs));
+        HC(hipMalloc(&e2, 4ull * n));
+        int np2 = 0, ne2 = 0;
+        ws_sap.collision(1.0f, d, p2, e2, &np2, &ne2);
+        if (ws_sap.lastStatus() != GSAP_OK || np2 != np || ne2 != ne) return 7;
+        std::vector<int2> hp2(np2);
+        std::vector<int> he2(ne2);
+        HC(hipMemcpy(hp2.data(), p2, 8ull * np2, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(he2.data(), e2, 4ull * ne2, hipMemcpyDeviceToHost));
+        for (int k = 0; k < np; ++k)
+            if (hp2[k].x != hp[k].x || hp2[k].y != hp[k].y) return 8;
+        for (int k = 0; k < ne; ++k)
+            if (he2[k] != he[k]) return 8;
+        (void)hipFree(p2); (void)hipFree(e2); (void)hipFree(ws);
+    }
+
+    // ParticleRigidBody set up the way DemoParticles::init does
+    // (demoParticles.h:230-414): host RigidBodyData with Quaternion and
+    // Matrix3x3 arrays, templates in MAX_NUM_PARTICLES slots per shape, shape
+    // inverse inertias from Matrix3x3::invert, setParams / setMaxVelocity.
+    {
+        const int nb = 2;
+        ParticleRigidBody prb(nb, 2 * 3, 64);
+        float4 pos[nb] = {{0, 5, 0, 0}, {1.9f, 5, 0, 0}};
+        Quaternion q[nb] = {Quaternion(0, 0, 0, 1), Quaternion(0, 0, 0.38268343f, 0.92387953f)};
+        float4 lv[nb] = {{1, 0, 0, 0}, {-1, 0, 0, 0}}, av[nb] = {{0, 0, 0, 0}, {0, 0.5f, 0, 0}};
+        int shape[nb] = {0, 1}, start[nb] = {0, 3}, cnt[nb] = {3, 3};
+        float im[nb] = {1.0f / 3, 1.0f / 6};
+        Matrix3x3 ii[nb];
+        for (int b = 0; b < nb; ++b) ii[b].setIdentity();
+        ParticleRigidBody::RigidBodyData rb = {nb, pos, q, lv, av, shape, start, cnt, im, ii, 6};
         prb.setRbData(rb);
-        float4 ref = {0, 0, 0, 0.5f};
-        float mass = 1.0f;
-        prb.setParticleData(&ref, &mass, 1, 1);
-        Sap s2(2);
+        float4 ref[2 * MAX_NUM_PARTICLES] = {};
+        float mass[2 * MAX_NUM_PARTICLES] = {};
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 3; ++k) {
+                ref[ParticleRigidBody::getParticleRefIdx(s, k)] = make_float4(0.5f * (k - 1), 0.1f * s, 0, 0.25f);
+                mass[ParticleRigidBody::getParticleRefIdx(s, k)] = 1.0f + s;
+            }
+        prb.setParticleData(ref, mass, 2 * MAX_NUM_PARTICLES);
+        Matrix3x3 shapeInv[2] = {Matrix3x3(make_float4(2, 0, 0, 0), make_float4(0, 3, 0, 0), make_float4(0, 0, 4, 0)),
+                                 Matrix3x3(make_float4(4, 1, 0, 0), make_float4(0, 2, 0, 0), make_float4(0, 0, 1, 0))};
+        for (int s = 0; s < 2; ++s) shapeInv[s].invert();
+        prb.setParticleShapeInvInertia(shapeInv, 2);
+        prb.setParams(2.1f, 0.8f);
+        prb.setMaxVelocity(0.25f, 0.01f);
+
+        // getRbShapeIdx / getRbNumParticles as demoParticles.h:1398-1401 reads them
+        int hs[nb], hn[nb];
+        HC(hipMemcpy(hs, prb.getRbShapeIdx(), sizeof hs, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(hn, prb.getRbNumParticles(), sizeof hn, hipMemcpyDeviceToHost));
+        for (int b = 0; b < nb; ++b)
+            if (hs[b] != shape[b] || hn[b] != cnt[b]) return 9;
+
+        // particle positions = x + q r q^-1 (the header's Quaternion on the host)
+        prb.updateParticleData();
+        std::vector<float4> pp(6);
+        std::vector<Quaternion> gq(nb);
+        HC(hipMemcpy(pp.data(), prb.getParticlePos(), 6 * 16, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(gq.data(), prb.getRbQuat(), nb * 16, hipMemcpyDeviceToHost));
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < 3; ++k) {
+                float4 r;
+                gq[b].rotateVector(ref[ParticleRigidBody::getParticleRefIdx(shape[b], k)], r);
+                const float4 g = pp[start[b] + k];
+                if (std::fabs(g.x - (r.x + pos[b].x)) > 1e-5f || std::fabs(g.y - (r.y + pos[b].y)) > 1e-5f ||
+                    std::fabs(g.z - (r.z + pos[b].z)) > 1e-5f || g.w != 0.25f)
+                    return 10;
+            }
+        // prepare: I_w^-1 = R I0^-1 R^T with R from the header's quaternion
+        prb.prepare();
+        gsap_prb_ptrs ptrs;
+        if (gsap_prb_get_ptrs(prb.handle(), &ptrs) != GSAP_OK) return 11;
+        Matrix3x3 iw[nb];
+        HC(hipMemcpy(iw, ptrs.rb_inv_inertia, sizeof iw, hipMemcpyDeviceToHost));
+        for (int b = 0; b < nb; ++b) {
+            Matrix3x3 R, RT;
+            gq[b].getRotationMatrix(R);
+            RT.setTranspose(R);
+            const Matrix3x3 want = (R * shapeInv[shape[b]]) * RT;
+            for (int i = 0; i < 3; ++i)
+                if (std::fabs(iw[b][i].x - want[i].x) > 1e-5f || std::fabs(iw[b][i].y - want[i].y) > 1e-5f ||
+                    std::fabs(iw[b][i].z - want[i].z) > 1e-5f)
+                    return 11;
+        }
+
+        Sap s2(6);
         for (int k = 0; k < 3; ++k) prb.step(s2, 0.001f);
-        std::vector<float4> v(2);
-        HC(hipMemcpy(v.data(), prb.getRbPos(), 32, hipMemcpyDeviceToHost));
+        std::vector<float4> v(nb);
+        HC(hipMemcpy(v.data(), prb.getRbPos(), nb * 16, hipMemcpyDeviceToHost));
         std::fprintf(stderr, "prb pos %f %f\n", v[0].x, v[1].x);
         if (!(v[0].x < v[1].x)) return 5;
     }

@@ -65,3 +65,15 @@ def test_no_cpu_fallback_in_product():
                 txt = open(os.path.join(dirpath, f)).read()
                 for bad in ("import oracle", "from oracle", "liboracle", "gsap_oracle", "or_sap("):
                     assert bad not in txt, (f, bad)
+
+
+def test_dropin_math_headers_host_checks():
+    """The Quaternion / Matrix3x3 drop-in types (include/gsap/quaternion.h,
+    matrix3x3.h) checked on the host: quarter turns, R v = q v q^-1, R R^T = I,
+    M M^-1 = I, composition order (tests/shim/header_math.cpp)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "shim", "_build", "header_math")
+    subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "shim"), "_build/header_math"],
+                   check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
