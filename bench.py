@@ -386,8 +386,19 @@ def main():
         "wall_s": round(wall, 3),
         "cpu_baseline": cpu,
     }
+    # The timed configuration's list is checked once, after timing, against
+    # the default path on the same input (plain sweep, one pair region,
+    # whole-key pair sort); any difference fails the benchmark.
+    plain = G.Sap(max_n=n, axis_mode=mode)
+    pairs_plain = torch.empty_like(pairs)
+    P3, _, _ = plain.collide_into(bd, pairs_plain)
+    identical = P3 == P and bool(torch.equal(pairs, pairs_plain))
+    out["config"]["pairs_identical_to_default_path"] = identical
+    if not identical:
+        print(json.dumps(out))
+        raise SystemExit("timed pair list differs from the default path's")
+    del pairs_plain
     if not args.no_extras and slabs:
-        plain = G.Sap(max_n=n, axis_mode=mode)
         plain.set_timing(True)
         ms_plain = _time_ms(torch, lambda: plain.collide_into(bd, pairs), max(1, args.steps // 2), 1)
         _, _, stp = plain.collide_into(bd, pairs)

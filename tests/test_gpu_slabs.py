@@ -135,3 +135,23 @@ def test_batched_refuses_slabs(dev):
     with pytest.raises(G.GsapError) as e:
         s.collide_batched(to_dev(boxes, dev), off, None, None)
     assert e.value.code == G.ERR_UNSUPPORTED
+
+
+def test_full_size_bench_config_identical(dev):
+    """The configuration the benchmark times, at its own size: 1,048,576
+    anisotropic boxes (seed 1), automatic axis, slab sweep, 8 pair regions,
+    XCD chunk 16, the per-part second pass with 4-byte items and the local
+    pair stage.  Its list equals the default path's (plain sweep, one region,
+    whole-key pair sort), which test_full_size_aniso_properties pins."""
+    import torch
+    bd = to_dev(W.aniso(), dev)
+    p0, _, st0 = G.Sap().collide(bd)
+    s = G.Sap(max_n=len(bd), axis_mode=G.AXIS_AUTO, flags=G.FLAG_SLABS)
+    s.set_tune(slabs=1, pair_local=1, pair_regions=8, sweep_chunk=16, pair_items=16, pair_bits=8,
+               sort_items=8, sort_bits=8, pair_grid=2048)
+    p1, _, st1 = s.collide(bd)
+    assert st1.num_pairs == st0.num_pairs
+    assert torch.equal(p0, p1)
+    del p1
+    p2, _, _ = s.collide(bd)  # and again on the same handle (epochs, counters)
+    assert torch.equal(p0, p2)
