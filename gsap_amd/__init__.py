@@ -43,9 +43,10 @@ EXPORTS = [
     "gsap_mlp_get_params", "gsap_mlp_forward", "gsap_mlp_loss_grad", "gsap_mlp_train",
     "gsap_axis_candidates", "gsap_sap_set_learned_model", "gsap_sap_features_batched",
     "gsap_sap_axis_tests_batched", "gsap_sap_get_tune", "gsap_sap_set_tune", "gsap_sap_autotune",
-    "gsap_sap_query_workspace", "gsap_sap_set_workspace",
+    "gsap_sap_query_workspace", "gsap_sap_set_workspace", "gsap_sap_collide_noalloc",
 ]
 N_FEATURES, N_PARAMS, N_CANDIDATES = 64, 25348, 64
+NOALLOC_SCRATCH_BYTES = 64 << 20  # GSAP_NOALLOC_SCRATCH_BYTES
 
 
 class GsapError(RuntimeError):
@@ -157,6 +158,7 @@ def lib():
         "gsap_sap_autotune": (i32, [P, P, P, u32, P, P, u32, ctypes.c_char_p, P]),
         "gsap_sap_query_workspace": (i32, [P, u32, u64, P]),
         "gsap_sap_set_workspace": (i32, [P, P, u64]),
+        "gsap_sap_collide_noalloc": (i32, [P, P, P, u32, P, P, u64, P, P]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -265,6 +267,14 @@ class Sap:
         cap = pairs.shape[0] if pairs is not None else 0
         check(lib().gsap_sap_collide_async(self.h, _stream(stream), _ptr(boxes), n, _ptr(group),
                                            _ptr(pairs), cap, _ptr(entries), _ptr(counts)))
+
+    def collide_noalloc(self, boxes, pairs, counts, entries=None, group=None, stream=None):
+        """collide_async through the allocation-free entry point (scratch from
+        the library's static device array; nothing allocated)."""
+        n = boxes.shape[0]
+        cap = pairs.shape[0] if pairs is not None else 0
+        check(lib().gsap_sap_collide_noalloc(self.h, _stream(stream), _ptr(boxes), n, _ptr(group),
+                                             _ptr(pairs), cap, _ptr(entries), _ptr(counts)))
 
     def collide(self, boxes, group=None, stream=None):
         """Pre-count, allocate, update.  Returns (pairs, entries, stats)."""

@@ -664,6 +664,69 @@ extern "C" int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void*
                           count_only, (unsigned long long*)d_counts, nullptr);
 }
 
+extern "C" int32_t gsap_sap_collide_noalloc(gsap_sap* h, void* stream, const void* d_in,
+                                            uint32_t n, const uint32_t* d_group, void* d_pairs,
+                                            uint64_t pair_capacity, int32_t* d_entries,
+                                            uint64_t* d_counts) {
+    int32_t rc = check_common(h, d_in, n, d_group);
+    if (rc) return rc;
+    if (!d_counts) return fail(h, GSAP_ERR_INVALID_ARG, "d_counts is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    const bool count_only = (pair_capacity == 0 || d_pairs == nullptr);
+    if (!count_only && h->desc.output_kind == GSAP_OUT_SYM_CSR && !d_entries)
+        return fail(h, GSAP_ERR_INVALID_ARG, "SYM_CSR output needs d_entries");
+    if (n == 0) {
+        HIPCHK(h, launch_zero_u64((unsigned long long*)d_counts, 5, st));
+        return GSAP_OK;
+    }
+    void* base = nullptr;
+    size_t bytes = 0;
+    HIPCHK(h, scratch_region(&base, &bytes));
+    WsReq req;
+    // the sizes of this call only (not the handle's max_n)
+    const uint32_t max_n = h->desc.max_n;
+    h->desc.max_n = 0;
+    rc = req_single(h, n, pair_capacity, count_only, req);
+    h->desc.max_n = max_n;
+    if (rc) return rc;
+    const size_t need = ws_total(req.b);
+    if (need > bytes) {
+        char buf[200];
+        snprintf(buf, sizeof buf,
+                 "gsap_sap_collide_noalloc: this update needs %llu bytes of scratch, the static "
+                 "scratch holds %llu (GSAP_NOALLOC_SCRATCH_BYTES)",
+                 (unsigned long long)need, (unsigned long long)bytes);
+        return fail(h, GSAP_ERR_CAPACITY, buf);
+    }
+    // This call's buffers are carved from the scratch in place of the
+    // handle's own, which are kept as they are and restored afterwards (no
+    // buffer of the handle is freed or moved, so a captured batched update
+    // stays valid).
+    Buf saved[W_COUNT];
+    size_t off = 0;
+    for (int id = 0; id < W_COUNT; ++id) {
+        Buf* b = ws_buf(h, id);
+        saved[id] = *b;
+        b->p = req.b[id] ? (char*)base + off : nullptr;
+        b->bytes = req.b[id];
+        b->owned = false;
+        off += ws_align(req.b[id]);
+    }
+    // the scratch holds whatever the last call on this GPU left: clear the
+    // buffers that must start zeroed, every call
+    rc = ws_clear(h, st);
+    if (!rc) {
+        SegTable seg{};
+        seg.nseg = 1;
+        const uint32_t tile = sort_tile(h->items);
+        rc = enqueue_update(h, st, d_in, n, seg, (n + tile - 1) / tile, d_group, d_pairs,
+                            pair_capacity, d_entries, count_only, (unsigned long long*)d_counts,
+                            nullptr);
+    }
+    for (int id = 0; id < W_COUNT; ++id) *ws_buf(h, id) = saved[id];
+    return rc;
+}
+
 static int32_t read_counts(gsap_sap* h, hipStream_t st, uint64_t cap, bool count_only,
                            uint64_t* h_num_pairs, uint32_t* h_num_entries, gsap_stats* stats,
                            uint32_t n) {

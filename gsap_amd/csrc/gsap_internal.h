@@ -172,6 +172,9 @@ hipError_t launch_scene_offsets(const uint2* P, Ctrl* ctrl, unsigned long long c
                                 const uint32_t* seg_start, uint32_t nseg, unsigned long long* out,
                                 hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
+// k_scratch.hip: device address (this GPU's copy) and size of the static
+// scratch of the allocation-free entry point
+hipError_t scratch_region(void** ptr, size_t* bytes);
 // d_counts[0..4] = pairs, entries, sweep tests, flags, executed tests; with
 // axis_out: axis_out[0..1] = the float bits of axis[0..3]
 hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts, int sym,

@@ -134,6 +134,22 @@ int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void* d_in, uint
                                const uint32_t* d_group, void* d_pairs, uint64_t pair_capacity,
                                int32_t* d_entries, uint64_t* d_counts);
 
+/* Allocation-free variant of gsap_sap_collide_async.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and status bits; the update's scratch comes from a static
+ * device array of GSAP_NOALLOC_SCRATCH_BYTES per GPU (gsap_sap_query_workspace
+ * gives an update's need), so the call makes no hipMalloc/hipFree and leaves
+ * the handle's own buffers untouched.  An update that needs more returns
+ * GSAP_ERR_CAPACITY before any device work.  Every call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (other streams, other
+ * handles, other threads) race and must be serialised by the caller.  The
+ * handle itself is created beforehand (gsap_sap_create allocates). */
+#define GSAP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int32_t gsap_sap_collide_noalloc(gsap_sap* h, void* stream, const void* d_in, uint32_t n,
+                                 const uint32_t* d_group, void* d_pairs, uint64_t pair_capacity,
+                                 int32_t* d_entries, uint64_t* d_counts);
+
 /* Batched update over many independent scenes packed back to back.
  *   h_scene_offsets  n_scenes+1 offsets into d_in (host array)
  *   d_pairs          CANON pairs with GLOBAL indices (into d_in), sorted,
