@@ -9,10 +9,11 @@ generated once (seeded) and uploaded before timing.
 
 Prints ONE JSON line.  `roofline` is for the dominant kernel of the update,
 timed with hipEvents recorded by the library on the launch stream between
-stages, over the timed steps themselves.  `cpu_baseline` is the CPU
-restatement of the reference algorithm (oracle/) timed on this host on a
-bounded sample: full axis/projection/sort plus the sweep of every 64th sorted
-object, scaled by 64.
+stages, over the timed steps themselves.  `cpu_baseline` is the reference's
+SaP method restated on the CPU (oracle/cpu_sap.c, OpenMP, built with
+-march=native on this host): full updates of the same boxes on all threads
+and on one thread, with the host's core count and CPU model.  The timed GPU
+list is compared with the default path's and with the CPU list (SHA-256).
 """
 import argparse
 import json
@@ -53,6 +54,13 @@ def traffic_for(kernel):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
+def _roof(alg_bytes, ms):
+    """Roofline fraction of algorithmic bytes moved in `ms` against HBM peak."""
+    gbs = alg_bytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": alg_bytes}
+
+
 def _time_ms(torch, fn, steps, warmup):
     """Mean ms per call of fn over `steps` calls after `warmup`, from events on
     torch's current stream (the stream every library call here launches on)."""
@@ -66,6 +74,147 @@ def _time_ms(torch, fn, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def host_info():
+    """Cores and CPU model of the measuring host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = int(omp) if omp and omp.isdigit() else aff
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": omp,
+            "threads_used": max(1, min(threads, aff)), "cpu_model": model}
+
+
+_NATIVE = {}
+
+
+def cpu_lib():
+    """The CPU baseline library built for this host (-O3 -march=native
+    -ffp-contract=off -fopenmp), else the portable prebuilt one."""
+    if "path" not in _NATIVE:
+        import tempfile
+        from oracle import oracle as O
+        d = tempfile.mkdtemp(prefix="gsap_cpu_")
+        _NATIVE["path"] = O.build_native_cpu_sap(d)
+        _NATIVE["flags"] = ("-O3 -march=native -ffp-contract=off -fopenmp" if _NATIVE["path"] else
+                            "-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (prebuilt)")
+    return _NATIVE["path"], _NATIVE["flags"]
+
+
+def _sha(a):
+    import hashlib
+    import numpy as np
+    return hashlib.sha256(np.ascontiguousarray(a).view(np.uint8)).hexdigest()
+
+
+def cpu_baseline_c2(boxes, gpu_pairs, P, runs):
+    """The reference's SaP method restated on the CPU (oracle/cpu_sap.c:
+    coordinate max-variance axis, flipped keys, stable 8-bit LSD radix sort,
+    sweep with early exit, canonical sort), full updates of the same boxes:
+    all threads (median of `runs` after one warm-up) and one thread (one
+    update).  The CPU list's SHA-256 is compared with the GPU's."""
+    import numpy as np
+    from oracle import oracle as O
+    lib, flags = cpu_lib()
+    info = host_info()
+    nt = info["threads_used"]
+    n = len(boxes)
+    O.cpu_sap(boxes, nt, cap=P, lib_path=lib)  # warm-up (page faults, thread pool)
+    ts, phases = [], None
+    for _ in range(runs):
+        t = time.perf_counter()
+        cp, tests, ax, ph = O.cpu_sap(boxes, nt, cap=P, lib_path=lib)
+        ts.append(time.perf_counter() - t)
+        phases = ph
+    t_all = float(np.median(ts))
+    same = len(cp) == P and _sha(cp) == _sha(gpu_pairs.cpu().numpy().view(np.uint32))
+    del cp
+    t = time.perf_counter()
+    _, _, _, ph1 = O.cpu_sap(boxes, 1, cap=P, lib_path=lib)
+    t_one = time.perf_counter() - t
+    # the reference's own sweep work on this input: its 4-cell pass plus
+    # cross pass (window sizes only), for comparison with sweep_tests
+    _, ref_work = O.sap_ref_cells(boxes, count_only=True)
+    return {"value": round(n / t_all, 1), "unit": "updates/s", "cores": nt, "kind": "port",
+            "ms_per_update": round(t_all * 1e3, 1),
+            "single_thread": {"value": round(n / t_one, 1), "ms_per_update": round(t_one * 1e3, 1),
+                              "phases_s": {k: round(v, 3) for k, v in ph1.items()}},
+            "phases_s": {k: round(v, 3) for k, v in phases.items()},
+            "sweep_tests": tests, "axis": "xyz"[ax], "pairs_sha256_equal_gpu": same,
+            "host": info, "build": flags,
+            "reference_sweep_tests": ref_work["cell_tests"] + ref_work["cross_tests"],
+            "reference_sweep_work": ref_work,
+            "sample": f"full updates of all {n} boxes (no sampling): median of {runs} on {nt} "
+                      f"threads after one warm-up, and one update on 1 thread"}
+
+
+def bench_c1(G, W, torch, dev, steps, warmup, with_cpu):
+    """16,384 boxes (centres U[0,64)^3, half-extents U[0.1,0.5] per axis, seed
+    0), PCA axis: GPU updates/s, the CPU baseline on 1 and all threads (median
+    of 10 after 2 warm-ups) and the O(n^2) brute force the lists are checked
+    against."""
+    import numpy as np
+    from oracle import oracle as O
+    boxes = W.small()
+    n = len(boxes)
+    bd = torch.from_numpy(boxes).to(dev)
+    sap = G.Sap(max_n=n, axis_mode=G.AXIS_PCA)
+    P, st = sap.count(bd)
+    pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
+    ms = _time_ms(torch, lambda: sap.collide_into(bd, pairs), steps, warmup)
+    got = pairs[:P].cpu().numpy().view(np.uint32)
+    t = time.perf_counter()
+    ref = O.brute(boxes)
+    t_brute = time.perf_counter() - t
+    out = {"workload": "16384 AABBs, centres U[0,64)^3, half-extents U[0.1,0.5] per axis, seed 0, "
+                       "PCA axis", "n": n, "pairs": P, "sweep_tests": int(st.sweep_tests),
+           "ms_per_update": round(ms, 4), "updates_per_s": round(n / (ms / 1e3), 1),
+           "identical_to_brute_force": bool(np.array_equal(got, ref)),
+           "brute_force_s": round(t_brute, 3)}
+    if with_cpu:
+        lib, flags = cpu_lib()
+        info = host_info()
+        res = {}
+        for name, nt in (("cpu_1thread", 1), ("cpu_all_threads", info["threads_used"])):
+            ts = []
+            for k in range(12):
+                t = time.perf_counter()
+                cp, _, _, _ = O.cpu_sap(boxes, nt, lib_path=lib)
+                if k >= 2:
+                    ts.append(time.perf_counter() - t)
+            med = float(np.median(ts))
+            res[name] = {"threads": nt, "ms_per_update": round(med * 1e3, 3),
+                         "updates_per_s": round(n / med, 1),
+                         "identical": bool(np.array_equal(cp, ref))}
+        out.update(res)
+        out["cpu_build"] = flags
+    return out
+
+
+def sap_bytes(n, P):
+    """Algorithmic bytes of one SaP update (every stage streamed once):
+    moments 24n, projection 32n, radix sort 68n (8-bit digits), sorted-box
+    gather 52n, sweep 24n + 8P, canonical CSR 8n + 16P -> 208n + 24P."""
+    return 208.0 * n + 24.0 * P
+
+
+def particle_step_bytes(nb, npart, directed, P):
+    """Algorithmic bytes of one rigid-body step: per body 64 B read, 48 B
+    written, 48 B inverse inertia, 16 B force/torque read; per particle 36 B
+    written and 52 B read (update), 16 B written and 16 B read (forces); per
+    directed pair 8 B + 36 B partner state; plus the SaP update of the
+    particles."""
+    return nb * (64.0 + 48 + 48 + 16) + npart * (36.0 + 52 + 16 + 16) + directed * 44.0 + \
+        sap_bytes(npart, P)
 
 
 def bench_batched(G, W, torch, dev, ns, steps, warmup, cache=None):
@@ -112,6 +261,8 @@ def bench_batched(G, W, torch, dev, ns, steps, warmup, cache=None):
             "speedup_vs_loop_async": round(ms_la / ms_b, 2),
             "speedup_vs_loop_sync": round(ms_ls / ms_b, 2),
             "pairs_identical": same,
+            # the whole batched update against the SaP byte model
+            "roofline": _roof(sap_bytes(ntot, P), ms_b),
             "tune": {k: v for k, v in tune.items() if k != "from_cache"}}
 
 
@@ -188,33 +339,51 @@ def bench_particles(G, W, torch, dev, nb, steps, with_cpu, cache=None):
     tune = _tune_particles(G, torch, sap, pp, npart, cache)
     ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
     p = prb.ptrs()  # raises on pair-buffer overflow
+    directed = int(p.num_pairs)
     out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase "
                        "(slab sweep)",
            "particles": npart, "steps": steps, "ms_per_step": round(ms, 4),
            "steps_per_s": round(1e3 / ms, 1), "particle_updates_per_s": round(npart * 1e3 / ms, 1),
-           "directed_pairs_last_step": int(p.num_pairs),
+           "directed_pairs_last_step": directed,
+           "roofline": _roof(particle_step_bytes(nb, npart, directed, directed // 2), ms),
            "tune": {k: v for k, v in tune.items() if k != "from_cache"}}
     if with_cpu:
-        # Bounded sample: one full-size oracle step whose sweep visits every
-        # 64th sorted particle; the sweep and canonical-sort time is scaled
-        # back up (a full oracle step takes minutes on one core).
+        # Full-size CPU steps of the same scene: the oracle's dynamics (one
+        # thread) and the CPU-baseline broadphase on the particle AABBs (all
+        # threads, then one), in the DemoParticles::step order.
         from oracle import oracle as O
-        stride = 64
+        lib, flags = cpu_lib()
+        info = host_info()
+        nt = info["threads_used"]
         ref = O.PrbState(sc["pos"], sc["quat"] / np.linalg.norm(sc["quat"], axis=1, keepdims=True),
                          sc["linvel"], sc["angvel"], sc["shape"], sc["pstart"], sc["npart"],
                          sc["inv_mass"], sc["ref_pos"], sc["ref_mass"], sc["max_per_shape"],
                          sc["shape_inv_inertia"])
         params = O.prb_params(**kw)
-        t = time.perf_counter()
-        ref.step(dt, params, broadphase="aabb", sample_stride=stride)
-        meas = time.perf_counter() - t
-        _, sweep, canon = O.last_times()
-        est = meas + (stride - 1) * (sweep + canon)
-        out["cpu_baseline"] = {"value": round(npart / est, 1), "unit": "particle updates/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"1 full-size oracle step sweeping every {stride}th sorted "
-                                         f"particle ({meas:.1f}s), sweep+canonical sort scaled "
-                                         f"x{stride}: {est:.1f}s per step"}
+        grp = ref.pbody.astype(np.uint32)
+
+        def cpu_step(threads):
+            t = time.perf_counter()
+            ref.integrate(dt, params)
+            ref.update_particles()
+            canon, _, _, _ = O.cpu_sap(ref.particle_aabbs(), threads, group=grp, lib_path=lib)
+            sym, _ = O.sym_csr(canon)
+            ref.prepare()
+            ref.forces(sym, dt, params)
+            ref.update_velocity(dt, params)
+            return time.perf_counter() - t
+
+        cpu_step(nt)  # warm-up
+        t_all = float(np.median([cpu_step(nt) for _ in range(3)]))
+        t_one = cpu_step(1)
+        out["cpu_baseline"] = {"value": round(npart / t_all, 1), "unit": "particle updates/s",
+                               "cores": nt, "kind": "port", "ms_per_step": round(t_all * 1e3, 1),
+                               "single_thread": {"value": round(npart / t_one, 1),
+                                                 "ms_per_step": round(t_one * 1e3, 1)},
+                               "build": flags,
+                               "sample": f"full steps of all {npart} particles: median of 3 with the "
+                                         f"broadphase on {nt} threads after one warm-up, then one "
+                                         "step on 1 thread; dynamics single-threaded"}
     return out
 
 
@@ -229,7 +398,8 @@ def main():
     ap.add_argument("--no-slabs", action="store_true",
                     help="plain sweep along one axis (no slab split on a second axis)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=64)
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="timed full CPU-baseline updates at 1M boxes on all threads")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, learned-axis and particle-step sections")
     ap.add_argument("--scenes", type=int, default=512, help="batched workload size")
@@ -340,26 +510,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        stride = args.cpu_stride
-        t = time.perf_counter()
-        # the reference's own axis rule (coordinate axis of largest variance)
-        _, tests_s, _ = O.sap(boxes, axis_mode=O.AXIS_COORD_MAXVAR,
-                              sample_stride=stride, cap=max(16, 4 * P // stride + 1024))
-        setup, sweep, canon = O.last_times()
-        est = setup + stride * (sweep + canon)
-        # the reference's own sweep work on this input: its 4-cell pass plus
-        # cross pass (window sizes only), for comparison with sweep_tests
-        _, ref_work = O.sap_ref_cells(boxes, count_only=True)
-        cpu = {"value": round(n / est, 1), "unit": "updates/s", "cores": 1, "kind": "port",
-               "reference_sweep_tests": ref_work["cell_tests"] + ref_work["cross_tests"],
-               "reference_sweep_work": ref_work,
-               "sample": f"coordinate max-variance axis, projection and radix sort of all {n} "
-                         f"boxes ({setup:.2f}s), sweep and "
-                         f"canonical sort of every {stride}th sorted object ({sweep + canon:.2f}s), "
-                         f"extrapolated x{stride}: {est:.1f}s per update; "
-                         f"{time.perf_counter() - t:.1f}s measured"}
-
+        cpu = cpu_baseline_c2(boxes, pairs[:P], P, args.cpu_runs)
     out = {
         "metric": "exact-pair SaP updates/s @1M AABBs",
         "value": round(value, 1),
@@ -420,6 +571,7 @@ def main():
                                "axis_used": [round(x, 6) for x in stq.axis]}
             pca.close()
     if not args.no_extras:
+        out["c1"] = bench_c1(G, W, torch, dev, args.steps, args.warmup, not args.no_cpu_baseline)
         out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup,
                                        args.tune_cache)
         out["learned_axis"] = bench_learned(G, W, torch, dev, args.learned_scenes)

@@ -355,3 +355,58 @@ def brute_or_sap(boxes, kind, group, sample_stride=1):
     pairs, _, _ = sap(boxes, kind=kind, axis_mode=AXIS_COORD_MAXVAR, group=group, cap=cap,
                       sample_stride=sample_stride)
     return pairs
+
+
+# ---------------------------------------------------------------- CPU baseline
+_CPU_LIBS = {}
+
+
+def cpu_sap_lib(path=None):
+    """The multi-threaded CPU restatement (oracle/cpu_sap.c): the prebuilt
+    portable library, or `path` (e.g. a -march=native build of the host)."""
+    path = path or os.path.join(_HERE, "_build", "libcpusap.so")
+    if path not in _CPU_LIBS:
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        L.cpu_sap.restype = ctypes.c_int64
+        L.cpu_sap.argtypes = [P, ctypes.c_size_t, P, ctypes.c_int, P, ctypes.c_uint64, P, P, P]
+        L.cpu_sap_max_threads.restype = ctypes.c_int
+        _CPU_LIBS[path] = L
+    return _CPU_LIBS[path]
+
+
+def build_native_cpu_sap(out_dir):
+    """Compile oracle/cpu_sap.c for this host (-O3 -march=native
+    -ffp-contract=off -fopenmp) into out_dir; returns the library path, or
+    None when the compiler is unavailable."""
+    out = os.path.join(out_dir, "libcpusap_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
+           "-fopenmp", "-fPIC", "-shared", "-o", out, os.path.join(_HERE, "cpu_sap.c"), "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    return out
+
+
+def cpu_sap(boxes, threads, group=None, cap=None, lib_path=None):
+    """One CPU-baseline update of AABBs: (canonical pairs, sweep tests, axis,
+    {axis_project, sort, sweep, canon} seconds)."""
+    L = cpu_sap_lib(lib_path)
+    b = _boxes(boxes, KIND_AABB)
+    g = None if group is None else np.ascontiguousarray(group, np.uint32)
+    n = len(b)
+    cap = max(16, 64 * n) if cap is None else cap
+    tests = np.zeros(1, np.uint64)
+    ax = ctypes.c_int(0)
+    t = np.zeros(4, np.float64)
+    while True:
+        pairs = np.empty((cap, 2), np.uint32)
+        P = L.cpu_sap(_p(b), n, _p(g), int(threads), _p(pairs), cap, _p(tests), ctypes.byref(ax), _p(t))
+        if P <= cap:
+            break
+        cap = int(P)
+    keys = ("axis_project", "sort", "sweep", "canon")
+    return pairs[:P], int(tests[0]), int(ax.value), dict(zip(keys, t.tolist()))

@@ -334,3 +334,27 @@ def test_ref_cells_straddlers_across_planes():
     pairs, st = O.sap_ref_cells(boxes)
     assert pairs.tolist() == [[0, 1]]
     assert st["relevant"] >= 2
+
+
+def test_cpu_baseline_equals_oracle_and_brute():
+    """The timed CPU baseline (oracle/cpu_sap.c, OpenMP) gives the oracle's
+    canonical list and sweep-test count on 1 and several threads, with and
+    without the group filter, and the brute-force list."""
+    for seed, n in ((0, 16384), (3, 40000)):
+        b = W.small(n, seed=seed)
+        if seed:
+            b = W.aniso(n, seed=seed)
+        ref, tests, _ = O.sap(b, axis_mode=O.AXIS_COORD_MAXVAR)
+        for th in (1, 3):
+            p, t, _, _ = O.cpu_sap(b, th)
+            assert np.array_equal(p, ref) and t == tests
+        if n == 16384:
+            assert np.array_equal(p, O.brute(b))
+    g = (np.arange(20000) // 27).astype(np.uint32)
+    b = W.small(20000, seed=8)
+    b[:, 3:] = b[:, :3] + 3.0 * (b[:, 3:] - b[:, :3])
+    p, _, _, _ = O.cpu_sap(b, 4, group=g)
+    assert np.array_equal(p, O.brute(b, group=g)) and len(p) > 0
+    # capacity: a too-small buffer is retried with the exact size
+    p2, _, _, _ = O.cpu_sap(b, 2, group=g, cap=16)
+    assert np.array_equal(p2, p)
