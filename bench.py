@@ -296,6 +296,18 @@ def bench_learned(G, W, torch, dev, ns):
             "sweep_tests_auto": tests["auto"], "ms_batched_auto": round(ms["auto"], 4)}
 
 
+def _d2h(ptr, nfloat):
+    """Copy nfloat float32 values at a device address to the host."""
+    import ctypes
+    import numpy as np
+    out = np.empty(nfloat, np.float32)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ptr, out.nbytes, 2) == 0
+    return out
+
+
 def _tune_particles(G, torch, sap, pp, npart, cache=None):
     """Autotune the particle broadphase on the step's own particle AABBs and
     body ids (device buffers owned by the rigid-body handle)."""
@@ -325,28 +337,40 @@ def bench_particles(G, W, torch, dev, nb, steps, with_cpu, cache=None):
     flags = G.PRB_ABS_SPRING | G.PRB_WALLS | G.PRB_SKIP_SAME_BODY
     kw = dict(gravity=(0, -9.81, 0), k_abs=1e4, c_abs=50.0, floor_scale=1.0, wall_lo=(0, 0),
               wall_hi=(256.0, 256.0), flags=flags)
-    prb = G.ParticleRigidBody(nb, npart, npart * 8)
-    prb.set_rb_data(sc["pos"], sc["quat"], sc["linvel"], sc["angvel"], sc["shape"], sc["pstart"],
-                    sc["npart"], sc["inv_mass"], sc["inv_inertia"])
-    prb.set_particle_data(sc["ref_pos"], sc["ref_mass"], sc["max_per_shape"])
-    prb.set_particle_shape_inv_inertia(sc["shape_inv_inertia"])
-    prb.set_physics(broadphase_kind=G.INPUT_AABB, **kw)
-    sap = G.Sap(max_n=npart, input_kind=G.INPUT_AABB, output_kind=G.OUT_SYM_CSR,
-                axis_mode=G.AXIS_AUTO, flags=G.FLAG_SKIP_SAME_GROUP | G.FLAG_SLABS)
     dt = 1e-4
-    prb.step(sap, dt)  # particle AABBs of the first step, to tune the broadphase on
-    pp = prb.ptrs()
-    tune = _tune_particles(G, torch, sap, pp, npart, cache)
-    ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
-    p = prb.ptrs()  # raises on pair-buffer overflow
-    directed = int(p.num_pairs)
+    runs = {}
+    # The same scene stepped with a from-scratch broadphase handle and with a
+    # coherent one (GSAP_FLAG_COHERENT: key function kept, sort buckets from
+    # the previous step's order); both end in the same state.
+    for name, coh in (("from_scratch", 0), ("coherent", G.FLAG_COHERENT)):
+        prb = G.ParticleRigidBody(nb, npart, npart * 8)
+        prb.set_rb_data(sc["pos"], sc["quat"], sc["linvel"], sc["angvel"], sc["shape"],
+                        sc["pstart"], sc["npart"], sc["inv_mass"], sc["inv_inertia"])
+        prb.set_particle_data(sc["ref_pos"], sc["ref_mass"], sc["max_per_shape"])
+        prb.set_particle_shape_inv_inertia(sc["shape_inv_inertia"])
+        prb.set_physics(broadphase_kind=G.INPUT_AABB, **kw)
+        sap = G.Sap(max_n=npart, input_kind=G.INPUT_AABB, output_kind=G.OUT_SYM_CSR,
+                    axis_mode=G.AXIS_AUTO, flags=G.FLAG_SKIP_SAME_GROUP | G.FLAG_SLABS | coh)
+        prb.step(sap, dt)  # particle AABBs of the first step, to tune the broadphase on
+        tune = _tune_particles(G, torch, sap, prb.ptrs(), npart, cache)
+        ms = _time_ms(torch, lambda: prb.step(sap, dt), steps, 10)
+        p = prb.ptrs()  # raises on pair-buffer overflow
+        hip_pos = _d2h(p.rb_pos, nb * 4)
+        runs[name] = (ms, tune, int(p.num_pairs), hip_pos)
+        prb.close()
+        sap.close()
+    ms, tune, directed, pos_s = runs["from_scratch"]
+    ms_c, tune_c, _, pos_c = runs["coherent"]
     out = {"workload": f"{nb} cubes x 27 particles, {steps} steps, dt 1e-4, AABB broadphase "
                        "(slab sweep)",
            "particles": npart, "steps": steps, "ms_per_step": round(ms, 4),
            "steps_per_s": round(1e3 / ms, 1), "particle_updates_per_s": round(npart * 1e3 / ms, 1),
+           "ms_per_step_coherent": round(ms_c, 4),
+           "coherent_final_state_identical": bool(np.array_equal(pos_c, pos_s)),
            "directed_pairs_last_step": directed,
            "roofline": _roof(particle_step_bytes(nb, npart, directed, directed // 2), ms),
-           "tune": {k: v for k, v in tune.items() if k != "from_cache"}}
+           "tune": {k: v for k, v in tune.items() if k != "from_cache"},
+           "tune_coherent": {k: v for k, v in tune_c.items() if k != "from_cache"}}
     if with_cpu:
         # Full-size CPU steps of the same scene: the oracle's dynamics (one
         # thread) and the CPU-baseline broadphase on the particle AABBs (all

@@ -24,6 +24,7 @@ AXIS_PCA, AXIS_COORD_MAXVAR, AXIS_FIXED, AXIS_LEARNED, AXIS_AUTO = 0, 1, 2, 3, 4
 OUT_CANON, OUT_SYM_CSR = 0, 1
 FLAG_SKIP_SAME_GROUP = 1
 FLAG_SLABS = 2
+FLAG_COHERENT = 4
 PRB_ABS_SPRING, PRB_WALLS, PRB_FLOOR_CLAMP, PRB_CAP_VEL, PRB_SKIP_SAME_BODY, PRB_INERTIA = (
     1, 2, 4, 8, 16, 32)
 

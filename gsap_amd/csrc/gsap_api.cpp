@@ -87,6 +87,7 @@ struct gsap_sap {
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
     Buf axis2, vr_partials, slabp, slab_cnt;  // slab mode
     Buf wstart;                               // window starts of the pair sort's local stage
+    Buf coh;                                  // coherent object sort: splitters and buckets
     bool has_model = false;
     float conf_min = 0.95f;
     unsigned long long* h_counts = nullptr;  // pinned
@@ -96,6 +97,14 @@ struct gsap_sap {
     uint64_t region_stride = 0;  // slots between region starts
     uint64_t sort_cap = 0;       // capacity of the second pair buffer (padded total)
     bool timing = false;
+    // coherent updates (GSAP_FLAG_COHERENT): the sorted order and splitters
+    // in the buffers belong to an update of coh_n objects at buffer
+    // generation coh_gen; coh_age counts updates since the key function was
+    // refreshed
+    bool coh_valid = false;
+    uint32_t coh_n = 0, coh_age = 0;
+    int coh_slabs = 0;
+    uint64_t coh_gen = 0;
     uint32_t cus = 256;  // compute units of the device the handle was created on
     static constexpr int kEv = 20;
     hipEvent_t ev[kEv] = {};
@@ -228,7 +237,7 @@ enum WsId : int {
     W_KEYS0, W_KEYS1, W_VALS0, W_VALS1, W_REC, W_GRP, W_GRP_END, W_ROWSTART, W_HAS,
     W_BLOCK_SUMS, W_PARTIALS, W_LB_OBJ, W_VR_PARTIALS, W_TILE_SEG, W_TILE_SUMS, W_AXIS,
     W_HIST_OBJ, W_SEG_START, W_SEG_N, W_SEG_TILE0, W_SEG_MOM, W_FEATS, W_HIST_PAIR, W_AXIS2,
-    W_SLABP, W_SLAB_CNT, W_PTMP0, W_PTMP1, W_WSTART, W_LB_PAIR, W_COUNT
+    W_SLABP, W_SLAB_CNT, W_PTMP0, W_PTMP1, W_WSTART, W_LB_PAIR, W_COH, W_COUNT
 };
 static_assert(W_COUNT <= 32, "gsap_sap::ws_size holds 32 entries");
 
@@ -263,6 +272,7 @@ static Buf* ws_buf(gsap_sap* h, int id) {
         case W_PTMP0: return &h->ptmp[0];
         case W_PTMP1: return &h->ptmp[1];
         case W_WSTART: return &h->wstart;
+        case W_COH: return &h->coh;
         default: return &h->lb_pair;
     }
 }
@@ -314,6 +324,7 @@ static void req_obj(const gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
         r.b[W_SLABP] = kSlabParams * sizeof(float);
         r.b[W_SLAB_CNT] = 256 * sizeof(uint32_t);
     }
+    if ((h->desc.flags & GSAP_FLAG_COHERENT) && nseg == 1) r.b[W_COH] = kCohSmallBytes;
 }
 
 // Internal pair buffers.  The sweep writes the pairs it finds densely into
@@ -417,7 +428,8 @@ static void mark(gsap_sap* h, hipStream_t st) {
 static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint32_t n,
                               const SegTable& seg, uint32_t ntiles, const uint32_t* group,
                               void* pairs, uint64_t cap, int32_t* entries, bool count_only,
-                              unsigned long long* d_counts, unsigned long long* scene_out) {
+                              unsigned long long* d_counts, unsigned long long* scene_out,
+                              bool allow_coherent = false) {
     const int kind = h->desc.input_kind;
     const bool sym = h->desc.output_kind == GSAP_OUT_SYM_CSR;
     const uint32_t tile = sort_tile(h->items);
@@ -473,6 +485,44 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     const uint32_t n_e = slabs ? 2 * n : n;  // sorted entries
     const uint32_t ntiles_e = slabs ? (n_e + tile - 1) / tile : ntiles;
     const bool learned = h->desc.axis_mode == GSAP_AXIS_LEARNED;
+    // Coherent update (GSAP_FLAG_COHERENT): a single scene whose previous
+    // update of the same object count left its sorted order and splitters in
+    // these buffers.  Every kCohRefresh-th update refreshes the key function
+    // (and sorts the regular way); the others keep it.
+    const bool coherent = allow_coherent && (h->desc.flags & GSAP_FLAG_COHERENT) &&
+                          seg.tile_seg == nullptr && !learned &&
+                          (uint64_t)n_e <= (uint64_t)kCohB * kCohMaxB;
+    const bool keep = coherent && h->coh_valid && h->coh_n == n && h->coh_slabs == (int)slabs &&
+                      h->coh_gen == h->buf_gen && h->coh_age % kCohRefresh != 0;
+    a.keep_key = keep ? 1 : 0;
+    // the buffers the object sort ends in (the LSD passes alternate)
+    uint32_t* const keysF = (npass_obj & 1u) ? a.keys2 : a.keys;
+    uint32_t* const valsF = (npass_obj & 1u) ? a.vals2 : a.vals;
+    CohArgs ca{};
+    if (coherent) {
+        char* cb = (char*)h->coh.p;
+        ca.in = in;
+        ca.kind = kind;
+        ca.ne = n_e;
+        ca.nb = (n_e + kCohB - 1) / kCohB;
+        ca.slabs = slabs ? 1 : 0;
+        ca.axis = a.axis;
+        ca.axis2 = a.axis2;
+        ca.slab = a.slab;
+        ca.prev_vals = valsF;
+        ca.split_key = (unsigned long long*)cb;
+        ca.bcount = (uint32_t*)(cb + (size_t)kCohMaxB * 8);
+        ca.bstart = ca.bcount + kCohMaxB;
+        ca.bcursor = ca.bstart + kCohMaxB + 1;
+        ca.keys0 = a.keys;
+        ca.vals0 = a.vals;
+        ca.keys1 = a.keys2;
+        ca.vals1 = a.vals2;
+        ca.keysF = keysF;
+        ca.valsF = valsF;
+        ca.slab_cnt = a.slab_cnt;
+        ca.ctrl = ctrl;
+    }
     if (learned) {
         if (!h->has_model)
             return fail(h, GSAP_ERR_UNSUPPORTED, "GSAP_AXIS_LEARNED needs gsap_sap_set_learned_model");
@@ -493,7 +543,11 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
                                     h->conf_min, a.axis, st));
     }
     if (slabs) HIPCHK(h, launch_slab_params(a, st));
-    HIPCHK(h, launch_project(a, st));
+    if (keep) {
+        HIPCHK(h, launch_coh_project(ca, st));
+    } else {
+        HIPCHK(h, launch_project(a, st));
+    }
     mark(h, st);
 
     SortArgs s{};
@@ -509,7 +563,25 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     s.items = h->items;
     s.grid = 0;
     int which = 0;
+    if (keep) {
+        // bucket sort; if a bucket overflowed, the regular projection (entry
+        // order; entries per slab counted already) and LSD passes run instead
+        HIPCHK(h, launch_coh_sort(ca, st));
+        ObjPipeArgs af = a;
+        af.run_if = &ctrl->coh_fallback;
+        af.no_slab_count = 1;
+        HIPCHK(h, launch_project(af, st));
+        s.run_if = &ctrl->coh_fallback;
+    }
     HIPCHK(h, sort_kv32(s, a.keys, a.vals, a.keys2, a.vals2, &which, st));
+    if (coherent) {
+        HIPCHK(h, launch_coh_save(ca, st));
+        h->coh_age = keep ? h->coh_age + 1 : 1;
+        h->coh_valid = true;
+        h->coh_n = n;
+        h->coh_slabs = slabs ? 1 : 0;
+        h->coh_gen = h->buf_gen;
+    }
     mark(h, st);
     ObjPipeArgs ae = a;  // entry space
     ae.n = n_e;
@@ -661,7 +733,7 @@ extern "C" int32_t gsap_sap_collide_async(gsap_sap* h, void* stream, const void*
     SegTable seg{};
     seg.nseg = 1;
     return enqueue_update(h, st, d_in, n, seg, ntiles, d_group, d_pairs, pair_capacity, d_entries,
-                          count_only, (unsigned long long*)d_counts, nullptr);
+                          count_only, (unsigned long long*)d_counts, nullptr, true);
 }
 
 extern "C" int32_t gsap_sap_collide_noalloc(gsap_sap* h, void* stream, const void* d_in,
@@ -756,6 +828,7 @@ static int32_t read_counts(gsap_sap* h, hipStream_t st, uint64_t cap, bool count
     }
     if (c[3] & kFlagTimeout) return fail(h, GSAP_ERR_HIP, "internal: sort look-back wait expired");
     if (c[3] & kFlagSlab) return fail(h, GSAP_ERR_HIP, "internal: a box spans more than two slabs");
+    if (c[3] & kFlagCoherent) return fail(h, GSAP_ERR_HIP, "internal: coherent sort bucket overflow");
     if (c[3] & kFlagNonFinite) return fail(h, GSAP_ERR_NONFINITE, "non-finite value in input boxes");
     if (!count_only && c[0] > cap) {
         char buf[160];
@@ -817,6 +890,7 @@ extern "C" int32_t gsap_sap_set_workspace(gsap_sap* h, void* d_ws, uint64_t byte
     h->ws_base = d_ws;
     h->ws_bytes = d_ws ? bytes : 0;
     h->ws_carved = false;
+    h->coh_valid = false;
     ++h->buf_gen;
     return GSAP_OK;
 }
@@ -864,6 +938,7 @@ static int32_t upload_scenes(gsap_sap* h, const SceneTables& t) {
 // cap > 0, the pair buffers) of that batch ensured in the same step.
 static int32_t prepare_scenes(gsap_sap* h, hipStream_t st, const uint32_t* off, uint32_t ns,
                               uint64_t cap, bool with_pairs, uint32_t* ntiles_out) {
+    h->coh_valid = false;  // the object buffers will hold another object set
     SceneTables t;
     int32_t rc = scene_tables(h, off, ns, t);
     if (rc) return rc;
@@ -1213,6 +1288,7 @@ static void apply_tune(gsap_sap* h, const gsap_tune& t) {
     h->sweep_chunk = t.sweep_chunk;
     h->pair_regions = t.pair_regions;
     h->pair_local = t.pair_local;
+    h->coh_valid = false;
     if (h->gexec) {  // the captured batched update used the old configuration
         (void)hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;

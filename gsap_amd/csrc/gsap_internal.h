@@ -61,10 +61,13 @@ struct Ctrl {
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
     uint32_t local_big;              // windows the pair sort's LDS stage handed on
-    uint32_t pad[3];
+    uint32_t coh_fallback;           // coherent object sort: 1 = a bucket overflowed, the
+                                     // regular projection and LSD passes run instead
+    uint32_t pad[2];
 };
 
-enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u };
+enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u,
+                  kFlagCoherent = 16u };
 
 // ---- launch wrappers (k_sap.hip) ----
 struct ObjPipeArgs {
@@ -104,6 +107,14 @@ struct ObjPipeArgs {
     // the second-pass digit counts per part (counted by the sweep); zeroed
     // by k_moments
     uint32_t* pair_claim;
+    // k_project runs only if *run_if != 0 (null: always); no_slab_count: the
+    // entries per slab were counted already (the coherent projection)
+    const uint32_t* run_if;
+    int no_slab_count;
+    // keep the key function of the last refresh (coherent updates): k_axis is
+    // not launched and k_slab_params keeps the slab parameters while the slab
+    // width still exceeds every box's slab-axis extent
+    int keep_key;
 };
 constexpr uint32_t kPairClaimWords = 3u * kMaxRegions * 256u;
 constexpr uint32_t kPairDigitStartWords = 260u;  // digit starts of the 32-bit items (257 used)
@@ -172,6 +183,49 @@ hipError_t launch_scene_offsets(const uint2* P, Ctrl* ctrl, unsigned long long c
                                 const uint32_t* seg_start, uint32_t nseg, unsigned long long* out,
                                 hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
+// ---- coherent object sort (k_coherent.hip) ----
+// Successive updates of one object set reuse the previous update's sorted
+// order: every kCohB-th entry of it gives a splitter, each entry is placed in
+// its splitter bucket with one scatter, and each bucket is sorted in LDS by
+// (key, entry position) -- the order of the stable LSD sort, so the result is
+// identical.  A bucket of more than kCohCap entries sets ctrl->coh_fallback.
+constexpr uint32_t kCohB = 1024u;      // entries per bucket of the previous order
+constexpr uint32_t kCohCap = 2048u;    // largest bucket sorted in LDS
+constexpr uint32_t kCohMaxB = 8192u;   // buckets (entries <= kCohB * kCohMaxB)
+constexpr uint32_t kCohRefresh = 64u;  // updates between refreshes of the key function
+// small buffer: split_key[kCohMaxB] u64, then count[kCohMaxB],
+// start[kCohMaxB + 1] and cursor[kCohMaxB] u32
+constexpr size_t kCohSmallBytes = (size_t)kCohMaxB * 8 + (size_t)kCohMaxB * 4 * 3 + 64;
+struct CohArgs {
+    const void* in;
+    int kind;
+    uint32_t ne;                 // entries (slab mode 2n, else n)
+    uint32_t nb;                 // buckets, ceil(ne / kCohB)
+    int slabs;
+    const float* axis;           // sweep axis u
+    const float* axis2;          // slab axis v
+    const float* slab;           // slab parameters
+    const uint32_t* prev_vals;   // the previous update's sorted values
+    unsigned long long* split_key;  // [nb - 1] composite keys of the previous sorted
+                                    // entries at positions kCohB * (j + 1)
+    uint32_t* bcount;            // [nb]
+    uint32_t* bstart;            // [nb + 1]
+    uint32_t* bcursor;           // [nb]
+    uint32_t* keys0;             // entries in the previous order (projection output)
+    uint32_t* vals0;
+    uint32_t* keys1;             // bucketed
+    uint32_t* vals1;
+    uint32_t* keysF;             // sorted (the LSD path's final buffers)
+    uint32_t* valsF;
+    uint32_t* slab_cnt;
+    Ctrl* ctrl;
+};
+// splitters + projection in the previous order with bucket counts (the
+// projection stage), then scan, scatter and the per-bucket LDS sort
+hipError_t launch_coh_project(const CohArgs& a, hipStream_t st);
+hipError_t launch_coh_sort(const CohArgs& a, hipStream_t st);
+// the splitters of the next update from the sorted keys and values (keysF, valsF)
+hipError_t launch_coh_save(const CohArgs& a, hipStream_t st);
 // k_scratch.hip: device address (this GPU's copy) and size of the static
 // scratch of the allocation-free entry point
 hipError_t scratch_region(void** ptr, size_t* bytes);
@@ -247,6 +301,8 @@ struct SortArgs {
     // the digit starts st_out[0..256] (the exclusive scan of its counts)
     int out32;
     uint32_t* st_out;
+    // every pass returns at once unless *run_if != 0 (null: always runs)
+    const uint32_t* run_if;
 };
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.

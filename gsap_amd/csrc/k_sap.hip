@@ -445,6 +445,12 @@ __global__ __launch_bounds__(kWG) void k_slab_params(ObjPipeArgs a) {
         r[3] = fminf(r[3], s_r[ww][3]);
         r[4] = fmaxf(r[4], s_r[ww][4]);
     }
+    float* sp = a.slab;
+    // A coherent update keeps the last refresh's parameters while the slab
+    // width still exceeds (1 + 2^-10) x every box's extent: slab and key
+    // indices of values outside the old ranges clamp, a monotone map, so the
+    // sweep stays exact and no box touches more than two slabs.
+    if (a.keep_key && sp[5] > 0.0f && isfinite(r[0]) && r[0] * (1.0f + 0x1p-10f) <= sp[5]) return;
     float v0 = 0.0f, invW = 0.0f, S = 1.0f, W = 0.0f;
     const float span = r[2] - r[1];
     if (isfinite(r[0]) && isfinite(span) && span > 0.0f && r[0] > 0.0f) {
@@ -463,18 +469,9 @@ __global__ __launch_bounds__(kWG) void k_slab_params(ObjPipeArgs a) {
         idu = 16777215.0f / uspan;
         if (isfinite(idu)) u0 = r[3]; else idu = 0.0f;
     }
-    float* sp = a.slab;
     sp[0] = v0; sp[1] = invW; sp[2] = S; sp[3] = u0; sp[4] = idu; sp[5] = W; sp[6] = 0.0f; sp[7] = 0.0f;
 }
 
-__device__ __forceinline__ uint32_t slab_of(float x, const float* sp) {
-    const float q = floorf((x - sp[0]) * sp[1]);
-    return (uint32_t)fminf(fmaxf(q, 0.0f), sp[2] - 1.0f);  // NaN -> 0
-}
-__device__ __forceinline__ uint32_t qkey(float x, const float* sp) {
-    const float q = floorf((x - sp[3]) * sp[4]);
-    return (uint32_t)fminf(fmaxf(q, 0.0f), 16777215.0f);
-}
 
 // ---------------------------------------------------------------- projection
 // Digit histograms of one key for every pass of the object sort.
@@ -485,6 +482,7 @@ __device__ __forceinline__ void hist_key(uint32_t (*s_h)[256], uint32_t key, uin
 }
 
 __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
+    if (a.run_if && *a.run_if == 0u) return;  // the coherent sort needed no fallback
     __shared__ uint32_t s_h[kMaxPasses][256];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
@@ -536,7 +534,7 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
             if (s1 > s0) atomicAdd(&s_sc[s1], 1u);
         }
         __syncthreads();
-        if (s_sc[tid]) atomicAdd(&a.slab_cnt[tid], s_sc[tid]);
+        if (s_sc[tid] && !a.no_slab_count) atomicAdd(&a.slab_cnt[tid], s_sc[tid]);
     }
     __syncthreads();
     uint32_t* h = a.hist_obj + (size_t)ti.seg * npass * radix;
@@ -1167,7 +1165,7 @@ hipError_t launch_slab_params(const ObjPipeArgs& a, hipStream_t st) {
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_moments, dim3(a.ntiles), dim3(kWG), 0, st, a);
     const uint32_t nseg = a.seg.tile_seg ? a.seg.nseg : 1u;
-    hipLaunchKernelGGL(k_axis, dim3(nseg), dim3(kWG), 0, st, a);
+    if (!a.keep_key) hipLaunchKernelGGL(k_axis, dim3(nseg), dim3(kWG), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_project(const ObjPipeArgs& a, hipStream_t st) {

@@ -68,6 +68,7 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     __shared__ uint32_t s_v[KT::kHasVals ? TILE : 1];
     __shared__ uint32_t s_tile;
 
+    if (a.run_if && *a.run_if == 0u) return;  // a path the update did not take
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0 + pass * BITS;
     uint32_t n = a.n;

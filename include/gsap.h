@@ -77,7 +77,16 @@ enum {
      * touches and a pair is reported only in the slab of the larger lower bound.
      * Same pair set, far fewer sweep tests on spread-out scenes.  Single-scene
      * updates only; n <= 2^30. */
-    GSAP_FLAG_SLABS = 2u
+    GSAP_FLAG_SLABS = 2u,
+    /* Temporal coherence for successive updates of one object set (the
+     * reference carries only its cell interval between frames, Sap.cpp:226-243).
+     * The key function (axes, slab and key parameters) of a refresh is kept
+     * for the next 63 single-scene updates of the same n, and the previous
+     * update's sorted order picks balanced buckets for a one-scatter object
+     * sort with a per-bucket LDS sort.  The pair list is identical to a
+     * from-scratch update's; a bucket that would overflow falls back to the
+     * regular sort on the device.  Batched updates ignore the flag. */
+    GSAP_FLAG_COHERENT = 4u
 };
 
 typedef struct gsap_sap gsap_sap;
