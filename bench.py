@@ -495,7 +495,7 @@ def main():
     # written, P pairs (the local stage likewise: each pair read and written
     # once); sweep: 24 B per box streamed + 8 B per pair written.
     entries = 2 * n if tn["slabs"] else n
-    obj_passes = -(-32 // tn["sort_bits"])
+    obj_passes = -(-(24 if tn["slabs"] else 32) // tn["sort_bits"])  # slab keys: 24 bits
     alg_sweep = 24.0 * n + 8.0 * P
     alg_pass = 16.0 * P
     # Keys of at most 40 bits sorted in two passes with 8 pair regions: the

@@ -84,6 +84,8 @@ struct gsap_sap {
     Buf keys[2], vals[2], rec, partials, axis, hist_obj, hist_pair, lb_obj, lb_pair;
     Buf ctrl, counts, ptmp[2], rowstart, has, block_sums, learned_axis, grp, grp_end;
     Buf seg_start, seg_n, seg_tile0, tile_seg;
+    Buf seg_start_e, seg_n_e, seg_tile0_e, tile_seg_e;  // slab mode: the scene table of entries
+    uint32_t ntiles_e = 0;                               // entry tiles of the uploaded table
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
     Buf axis2, vr_partials, slabp, slab_cnt;  // slab mode
     Buf wstart;                               // window starts of the pair sort's local stage
@@ -119,7 +121,7 @@ struct gsap_sap {
     // are carved from it instead of allocated; ws_size[] is the current carve
     void* ws_base = nullptr;
     uint64_t ws_bytes = 0;
-    size_t ws_size[32] = {};
+    size_t ws_size[40] = {};
     bool ws_carved = false;
     // batched graph cache
     uint64_t g_gen = 0;
@@ -237,9 +239,10 @@ enum WsId : int {
     W_KEYS0, W_KEYS1, W_VALS0, W_VALS1, W_REC, W_GRP, W_GRP_END, W_ROWSTART, W_HAS,
     W_BLOCK_SUMS, W_PARTIALS, W_LB_OBJ, W_VR_PARTIALS, W_TILE_SEG, W_TILE_SUMS, W_AXIS,
     W_HIST_OBJ, W_SEG_START, W_SEG_N, W_SEG_TILE0, W_SEG_MOM, W_FEATS, W_HIST_PAIR, W_AXIS2,
-    W_SLABP, W_SLAB_CNT, W_PTMP0, W_PTMP1, W_WSTART, W_LB_PAIR, W_COH, W_COUNT
+    W_SLABP, W_SLAB_CNT, W_PTMP0, W_PTMP1, W_WSTART, W_LB_PAIR, W_COH, W_SEG_START_E, W_SEG_N_E,
+    W_SEG_TILE0_E, W_TILE_SEG_E, W_COUNT
 };
-static_assert(W_COUNT <= 32, "gsap_sap::ws_size holds 32 entries");
+static_assert(W_COUNT <= 40, "gsap_sap::ws_size holds 40 entries");
 
 static Buf* ws_buf(gsap_sap* h, int id) {
     switch (id) {
@@ -273,6 +276,10 @@ static Buf* ws_buf(gsap_sap* h, int id) {
         case W_PTMP1: return &h->ptmp[1];
         case W_WSTART: return &h->wstart;
         case W_COH: return &h->coh;
+        case W_SEG_START_E: return &h->seg_start_e;
+        case W_SEG_N_E: return &h->seg_n_e;
+        case W_SEG_TILE0_E: return &h->seg_tile0_e;
+        case W_TILE_SEG_E: return &h->tile_seg_e;
         default: return &h->lb_pair;
     }
 }
@@ -319,10 +326,15 @@ static void req_obj(const gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     r.b[W_FEATS] = (size_t)nseg * kFeat * 4;
     // pair digit histograms, then the first pass's per-region counts and slot counters
     r.b[W_HIST_PAIR] = (kMaxPasses * 256 + kPairClaimWords + kPairDigitStartWords) * 4;
-    if (slab_mode(h)) {
-        r.b[W_AXIS2] = 4 * sizeof(float);
-        r.b[W_SLABP] = kSlabParams * sizeof(float);
-        r.b[W_SLAB_CNT] = 256 * sizeof(uint32_t);
+    if (slab_mode(h)) {  // per scene
+        r.b[W_AXIS2] = (size_t)nseg * 4 * sizeof(float);
+        r.b[W_SLABP] = (size_t)nseg * kSlabParams * sizeof(float);
+        r.b[W_SLAB_CNT] = (size_t)nseg * 256 * sizeof(uint32_t);
+        if (nseg > 1) {  // batched: the scene table of entries (2 per object)
+            r.b[W_SEG_START_E] = r.b[W_SEG_N_E] = (size_t)nseg * 4;
+            r.b[W_SEG_TILE0_E] = (size_t)(nseg + 1) * 4;
+            r.b[W_TILE_SEG_E] = ((size_t)ntiles * 2 + nseg) * 4;
+        }
     }
     if ((h->desc.flags & GSAP_FLAG_COHERENT) && nseg == 1) r.b[W_COH] = kCohSmallBytes;
 }
@@ -443,7 +455,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     const uint32_t shift0 = local ? kbits - 8 * npass_g : 0;
     const uint32_t pair_bits = local ? 8 : (uint32_t)h->pair_bits;
     const uint32_t npass_pair = local ? npass_g : (kbits + pair_bits - 1) / pair_bits;
-    const uint32_t npass_obj = (32 + h->obj_bits - 1) / h->obj_bits;
+    // object keys: 32-bit flipped floats, or 24-bit slab keys
+    const uint32_t obj_key_bits = slab_mode(h) ? kSlabKeyBits : 32u;
+    const uint32_t npass_obj = (obj_key_bits + h->obj_bits - 1) / h->obj_bits;
     Ctrl* ctrl = (Ctrl*)h->ctrl.p;
     h->nev = 0;
     mark(h, st);
@@ -476,14 +490,28 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.learned_axis = nullptr;
     a.seg_mom = (double*)h->seg_mom.p;
     // slab mode: one scene only (the batched entry point refuses the flag)
-    const bool slabs = slab_mode(h) && seg.tile_seg == nullptr;
+    // slab mode: per scene slab axis, parameters and slab counts; the sort,
+    // gather and sweep run over entries (2 per object), with the scene table
+    // of entries when batched
+    const bool slabs = slab_mode(h);
     a.slabs = slabs ? 1 : 0;
     a.axis2 = (float*)h->axis2.p;
     a.vr_partials = (float*)h->vr_partials.p;
     a.slab = (float*)h->slabp.p;
     a.slab_cnt = (uint32_t*)h->slab_cnt.p;
     const uint32_t n_e = slabs ? 2 * n : n;  // sorted entries
-    const uint32_t ntiles_e = slabs ? (n_e + tile - 1) / tile : ntiles;
+    SegTable seg_e = seg;
+    uint32_t ntiles_e = ntiles;
+    if (slabs && seg.tile_seg) {
+        seg_e.seg_start = (const uint32_t*)h->seg_start_e.p;
+        seg_e.seg_n = (const uint32_t*)h->seg_n_e.p;
+        seg_e.seg_tile0 = (const uint32_t*)h->seg_tile0_e.p;
+        seg_e.tile_seg = (const uint32_t*)h->tile_seg_e.p;
+        seg_e.ntiles = h->ntiles_e;
+        ntiles_e = h->ntiles_e;
+    } else if (slabs) {
+        ntiles_e = (n_e + tile - 1) / tile;
+    }
     const bool learned = h->desc.axis_mode == GSAP_AXIS_LEARNED;
     // Coherent update (GSAP_FLAG_COHERENT): a single scene whose previous
     // update of the same object count left its sorted order and splitters in
@@ -552,7 +580,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
 
     SortArgs s{};
     s.n = n_e;
-    s.seg = seg;
+    s.seg = seg_e;
     s.ntiles = ntiles_e;
     s.hist = a.hist_obj;
     s.npass = npass_obj;
@@ -585,6 +613,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     mark(h, st);
     ObjPipeArgs ae = a;  // entry space
     ae.n = n_e;
+    ae.seg = seg_e;
     ae.ntiles = ntiles_e;
     HIPCHK(h, launch_gather(ae, which ? a.keys2 : a.keys, which ? a.vals2 : a.vals, st));
     mark(h, st);
@@ -594,7 +623,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     w.grp = h->grp.p;
     w.grp_end = (const uint32_t*)h->grp_end.p;
     w.n = n_e;
-    w.seg = seg;
+    w.seg = seg_e;
     w.tile = tile;
     w.ntiles = ntiles_e;
     w.slabs = slabs ? 1 : 0;
@@ -899,6 +928,10 @@ extern "C" int32_t gsap_sap_set_workspace(gsap_sap* h, void* d_ws, uint64_t byte
 struct SceneTables {
     std::vector<uint32_t> start, cnt, t0, tseg;
     uint32_t ntiles = 0;
+    // slab mode: the same table over entries (scene s owns entries
+    // [2 start, 2 (start + n)))
+    std::vector<uint32_t> start_e, cnt_e, t0_e, tseg_e;
+    uint32_t ntiles_e = 0;
 };
 
 static int32_t scene_tables(gsap_sap* h, const uint32_t* off, uint32_t ns, SceneTables& t) {
@@ -919,6 +952,23 @@ static int32_t scene_tables(gsap_sap* h, const uint32_t* off, uint32_t ns, Scene
     }
     t.t0[ns] = nt;
     t.ntiles = nt;
+    if (slab_mode(h)) {
+        t.start_e.resize(ns);
+        t.cnt_e.resize(ns);
+        t.t0_e.resize(ns + 1);
+        t.tseg_e.clear();
+        uint32_t ne = 0;
+        for (uint32_t s = 0; s < ns; ++s) {
+            t.start_e[s] = 2u * t.start[s];
+            t.cnt_e[s] = 2u * t.cnt[s];
+            t.t0_e[s] = ne;
+            const uint32_t k = (t.cnt_e[s] + tile - 1) / tile;
+            for (uint32_t i = 0; i < k; ++i) t.tseg_e.push_back(s);
+            ne += k;
+        }
+        t.t0_e[ns] = ne;
+        t.ntiles_e = ne;
+    }
     return GSAP_OK;
 }
 
@@ -930,6 +980,15 @@ static int32_t upload_scenes(gsap_sap* h, const SceneTables& t) {
     HIPCHK(h, hipMemcpy(h->seg_n.p, t.cnt.data(), ns * 4, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->seg_tile0.p, t.t0.data(), (ns + 1) * 4, hipMemcpyHostToDevice));
     if (t.ntiles) HIPCHK(h, hipMemcpy(h->tile_seg.p, t.tseg.data(), (size_t)t.ntiles * 4, hipMemcpyHostToDevice));
+    if (!t.start_e.empty() && h->seg_start_e.p) {
+        HIPCHK(h, hipMemcpy(h->seg_start_e.p, t.start_e.data(), ns * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->seg_n_e.p, t.cnt_e.data(), ns * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->seg_tile0_e.p, t.t0_e.data(), (ns + 1) * 4, hipMemcpyHostToDevice));
+        if (t.ntiles_e)
+            HIPCHK(h, hipMemcpy(h->tile_seg_e.p, t.tseg_e.data(), (size_t)t.ntiles_e * 4,
+                                hipMemcpyHostToDevice));
+    }
+    h->ntiles_e = t.ntiles_e;
     ++h->buf_gen;
     return GSAP_OK;
 }
@@ -964,8 +1023,6 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
         return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode emits canonical pairs only");
     if (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP)
         return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no group filter");
-    if (h->desc.flags & GSAP_FLAG_SLABS)
-        return fail(h, GSAP_ERR_UNSUPPORTED, "batched mode has no slab split");
     const uint32_t n = h_scene_offsets[n_scenes];
     if (h_scene_offsets[0] != 0) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets must start at 0");
     int32_t rc = check_common(h, d_in, n, nullptr);
@@ -1376,7 +1433,6 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
     const std::string key = tune_key(h, n, n_scenes, batched);
     gsap_tune best = tune_of(h);
     if (cache_path && cache_lookup(cache_path, key, &best)) {
-        if (batched) best.slabs = h->slabs;
         apply_tune(h, best);
         h->tune_ms = best.ms;
         h->tune_cached = 1;
@@ -1452,13 +1508,13 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
             best = t;
         }
     };
-    {  // stage 1: plain or slab sweep (single scene) x sweep XCD chunking x pair regions
+    {  // stage 1: plain or slab sweep x sweep XCD chunking x pair regions
         const gsap_tune base = best;
-        for (int sl = 0; sl < (batched ? 1 : 2); ++sl)
+        for (int sl = 0; sl < 2; ++sl)
             for (int ch : {0, 4, 16, 64})
                 for (int rg : {1, kMaxRegions}) {
                     gsap_tune t = base;
-                    if (!batched) t.slabs = sl;
+                    t.slabs = sl;
                     t.sweep_chunk = ch;
                     t.pair_regions = rg;
                     if (t.slabs != base.slabs || ch != base.sweep_chunk || rg != base.pair_regions)

@@ -198,15 +198,15 @@ __device__ __forceinline__ bool axis_exact(int kind, const float u[3]) {
 }
 
 // Slab-mode keys (slab parameters sp: v0, 1/W, S, u0, 1/du, W): the slab of
-// a projection on the slab axis, and the 24-bit quantisation of a lower
-// bound on the sweep axis.
+// a projection on the slab axis, and the kSlabQBits-bit quantisation of a
+// lower bound on the sweep axis.
 __device__ __forceinline__ uint32_t slab_of(float x, const float* sp) {
     const float q = floorf((x - sp[0]) * sp[1]);
     return (uint32_t)fminf(fmaxf(q, 0.0f), sp[2] - 1.0f);  // NaN -> 0
 }
 __device__ __forceinline__ uint32_t qkey(float x, const float* sp) {
     const float q = floorf((x - sp[3]) * sp[4]);
-    return (uint32_t)fminf(fmaxf(q, 0.0f), 16777215.0f);
+    return (uint32_t)fminf(fmaxf(q, 0.0f), kSlabQMax);
 }
 
 }  // namespace gsap

@@ -48,10 +48,15 @@ struct alignas(16) RecSphere {
 // first kernel of every update resets what the update accumulates.
 constexpr int kMaxRegions = 8;  // pair regions (one allocation counter each)
 struct Ctrl {
-    unsigned long long num_pairs;    // pairs (or directed pairs) emitted by the sweep
+    unsigned long long num_pairs;    // pairs (or directed pairs) found: the sum of the region
+                                     // fills (set by k_pair_regions), or of part[0] (count only)
     unsigned long long tests;        // sweep-test count
     unsigned long long num_entries;  // SYM_CSR row starts
-    unsigned long long exec_tests;   // candidate tests executed (wave granularity)
+    unsigned long long exec_unused;  // (layout)
+    // sweep workgroup totals spread over kMaxRegions slots (workgroup w adds
+    // to slot w mod kMaxRegions: one contended address per XCD, not one for
+    // the grid): [0] pairs of a count-only sweep, [1] candidate tests executed
+    unsigned long long part[2][kMaxRegions];
     unsigned long long region_fill[kMaxRegions];       // pairs written to each region
     unsigned long long region_start[kMaxRegions + 1];  // logical start of each region
     unsigned long long region_valid[kMaxRegions];      // logical end of each region's pairs
@@ -120,7 +125,14 @@ constexpr uint32_t kPairClaimWords = 3u * kMaxRegions * 256u;
 constexpr uint32_t kPairDigitStartWords = 260u;  // digit starts of the 32-bit items (257 used)
 constexpr int kVr = 5;           // max v extent, min lo_v, max hi_v, min lo_u, max hi_u
 constexpr int kSlabParams = 8;   // v0, 1/W, S, u0, 1/du, W, -, -
-constexpr uint32_t kSlabNull = 0xFFFFFFFFu;  // key of an unused second entry
+// Slab-mode key: slab << kSlabQBits | quantised lower bound on the sweep axis
+// (any non-decreasing quantisation keeps the sweep exact; 16 bits over the
+// sweep-axis range only adds ties).  24-bit keys: three 8-bit sort passes.
+constexpr uint32_t kSlabQBits = 16u;
+constexpr uint32_t kSlabKeyBits = 8u + kSlabQBits;
+constexpr float kSlabQMax = (float)((1u << kSlabQBits) - 1u);
+// key of an unused second entry: sorts after every slab (slab indices <= 254)
+constexpr uint32_t kSlabNull = (1u << kSlabKeyBits) - 1u;
 constexpr uint32_t kFirstBit = 0x80000000u;  // entry is its box's first slab
 
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st);
@@ -333,6 +345,7 @@ hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long
                              int L, const uint32_t* st, hipStream_t stream);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
+constexpr uint32_t kShortWindow = 64;      // sweep: lane-per-object path up to this window
 constexpr uint32_t kSweepHistPasses = 2;   // pair-sort digit histograms the sweep can count
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,

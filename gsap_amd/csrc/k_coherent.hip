@@ -58,11 +58,11 @@ __device__ __forceinline__ uint32_t entry_key(const CohArgs& a, uint32_t v, cons
     const uint32_t q = qkey(pl, a.slab);
     if (v & kFirstBit) {
         *slab = s0;
-        return (s0 << 24) | q;
+        return (s0 << kSlabQBits) | q;
     }
     if (s1 > s0) {
         *slab = s1;
-        return (s1 << 24) | q;
+        return (s1 << kSlabQBits) | q;
     }
     *slab = 0xFFFFFFFFu;
     return kSlabNull;

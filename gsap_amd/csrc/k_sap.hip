@@ -115,9 +115,14 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
         const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
         const uint32_t b0 = tile * chunk, b1 = min(b0 + chunk, total);
         for (uint32_t i = b0 + tid; i < b1; i += kWG) a.hist_obj[i] = 0u;
+        if (a.slabs) {  // 256 slab counters per scene
+            const uint32_t tot_s = a.seg.nseg * 256u;
+            const uint32_t ch = (tot_s + gridDim.x - 1) / gridDim.x;
+            const uint32_t c0 = tile * ch, c1 = min(c0 + ch, tot_s);
+            for (uint32_t i = c0 + tid; i < c1; i += kWG) a.slab_cnt[i] = 0u;
+        }
         if (tile == 0) {
             for (uint32_t i = tid; i < a.npass_pair * 256u; i += kWG) a.hist_pair[i] = 0u;
-            if (a.slabs) a.slab_cnt[tid] = 0u;  // kWG == 256 slabs
             if (tid < 2 * kMaxPasses) a.ctrl->tile_ctr[tid] = 0u;
             if (a.pair_claim)
                 for (uint32_t i = tid; i < kPairClaimWords; i += kWG) a.pair_claim[i] = 0u;
@@ -356,7 +361,8 @@ __global__ __launch_bounds__(kWG) void k_axis(ObjPipeArgs a) {
         if (a.seg_mom)
             for (int k = 0; k < 9; ++k) a.seg_mom[9 * (size_t)seg + k] = mm[k];
         if (a.slabs) {
-            a.axis2[0] = v[0]; a.axis2[1] = v[1]; a.axis2[2] = v[2]; a.axis2[3] = 0.0f;
+            float* v2 = a.axis2 + 4 * (size_t)seg;
+            v2[0] = v[0]; v2[1] = v[1]; v2[2] = v[2]; v2[3] = 0.0f;
         }
     }
 }
@@ -378,8 +384,10 @@ __global__ __launch_bounds__(kWG) void k_vrange(ObjPipeArgs a) {
     __shared__ float s_r[kWaves][kVr];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = lane_id();
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
-    const float u[3] = {a.axis[0], a.axis[1], a.axis[2]};
-    const float v[3] = {a.axis2[0], a.axis2[1], a.axis2[2]};
+    const float* ax = a.axis + 4 * (size_t)ti.seg;
+    const float* ax2 = a.axis2 + 4 * (size_t)ti.seg;
+    const float u[3] = {ax[0], ax[1], ax[2]};
+    const float v[3] = {ax2[0], ax2[1], ax2[2]};
     const bool exact = axis_exact(a.kind, u);
     float r[kVr] = {0.0f, __builtin_inff(), -__builtin_inff(), __builtin_inff(), -__builtin_inff()};
     for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
@@ -419,9 +427,12 @@ __global__ __launch_bounds__(kWG) void k_vrange(ObjPipeArgs a) {
 // 24 bits over its range (any monotone map keeps the sweep exact).
 __global__ __launch_bounds__(kWG) void k_slab_params(ObjPipeArgs a) {
     __shared__ float s_r[kWaves][kVr];
+    const uint32_t seg = blockIdx.x;  // one workgroup per scene, over the scene's tiles
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[seg] : 0u;
+    const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[seg + 1] : a.ntiles;
     float r[kVr] = {0.0f, __builtin_inff(), -__builtin_inff(), __builtin_inff(), -__builtin_inff()};
-    for (uint32_t t = tid; t < a.ntiles; t += kWG) {
+    for (uint32_t t = t0 + tid; t < t1; t += kWG) {
         const float* p = a.vr_partials + (size_t)t * kVr;
         r[0] = fmaxf(r[0], p[0]);
         r[1] = fminf(r[1], p[1]);
@@ -445,7 +456,7 @@ __global__ __launch_bounds__(kWG) void k_slab_params(ObjPipeArgs a) {
         r[3] = fminf(r[3], s_r[ww][3]);
         r[4] = fmaxf(r[4], s_r[ww][4]);
     }
-    float* sp = a.slab;
+    float* sp = a.slab + kSlabParams * (size_t)seg;
     // A coherent update keeps the last refresh's parameters while the slab
     // width still exceeds (1 + 2^-10) x every box's extent: slab and key
     // indices of values outside the old ranges clamp, a monotone map, so the
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(kWG) void k_slab_params(ObjPipeArgs a) {
     float u0 = 0.0f, idu = 0.0f;
     const float uspan = r[4] - r[3];
     if (isfinite(uspan) && uspan > 0.0f) {
-        idu = 16777215.0f / uspan;
+        idu = kSlabQMax / uspan;
         if (isfinite(idu)) u0 = r[3]; else idu = 0.0f;
     }
     sp[0] = v0; sp[1] = invW; sp[2] = S; sp[3] = u0; sp[4] = idu; sp[5] = W; sp[6] = 0.0f; sp[7] = 0.0f;
@@ -506,9 +517,10 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
     } else {
         // Box i -> entries 2i (its first slab, flagged) and 2i+1 (the next
         // slab when it straddles a boundary, else an unused entry that sorts
-        // last).  Key = slab << 24 | 24-bit quantised lower bound on u.
-        const float v[3] = {a.axis2[0], a.axis2[1], a.axis2[2]};
-        const float* sp = a.slab;
+        // last).  Key = slab << kSlabQBits | quantised lower bound on u.
+        const float* ax2 = a.axis2 + 4 * (size_t)ti.seg;
+        const float v[3] = {ax2[0], ax2[1], ax2[2]};
+        const float* sp = a.slab + kSlabParams * (size_t)ti.seg;
         __shared__ uint32_t s_sc[256];  // entries per slab in this tile
         s_sc[tid] = 0u;
         __syncthreads();
@@ -524,8 +536,8 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
                 s1 = s0 + 1u;
             }
             const uint32_t q = qkey(pl, sp);
-            const uint32_t k0 = (s0 << 24) | q;
-            const uint32_t k1 = s1 > s0 ? ((s1 << 24) | q) : kSlabNull;
+            const uint32_t k0 = (s0 << kSlabQBits) | q;
+            const uint32_t k1 = s1 > s0 ? ((s1 << kSlabQBits) | q) : kSlabNull;
             reinterpret_cast<uint2*>(a.keys)[i] = make_uint2(k0, k1);
             reinterpret_cast<uint2*>(a.vals)[i] = make_uint2(i | kFirstBit, i);
             hist_key(s_h, k0, npass, bits);
@@ -534,7 +546,7 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
             if (s1 > s0) atomicAdd(&s_sc[s1], 1u);
         }
         __syncthreads();
-        if (s_sc[tid] && !a.no_slab_count) atomicAdd(&a.slab_cnt[tid], s_sc[tid]);
+        if (s_sc[tid] && !a.no_slab_count) atomicAdd(&a.slab_cnt[256 * (size_t)ti.seg + tid], s_sc[tid]);
     }
     __syncthreads();
     uint32_t* h = a.hist_obj + (size_t)ti.seg * npass * radix;
@@ -561,13 +573,14 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
     const float u[3] = {ax[0], ax[1], ax[2]};
     const bool exact = axis_exact(KIND, u);
     unsigned long long tests = 0;
-    if (a.slabs) {  // entries are sorted slab-major: slab starts are the prefix of the counts
+    const float* sp = a.slab + kSlabParams * (size_t)ti.seg;
+    if (a.slabs) {  // a scene's entries are sorted slab-major: slab starts are the prefix of its counts
         __shared__ uint32_t s_tmp[kWaves];
         uint32_t tot;
-        const uint32_t c = a.slab_cnt[tid];
+        const uint32_t c = a.slab_cnt[256 * (size_t)ti.seg + tid];
         const uint32_t ex = block_excl_scan(c, s_tmp, &tot);
-        s_sb[tid] = ex;
-        if (tid == 0) s_sb[256] = tot;
+        s_sb[tid] = ti.seg_start + ex;
+        if (tid == 0) s_sb[256] = ti.seg_start + tot;
         __syncthreads();
     }
     for (uint32_t sub = ti.lo; sub < ti.hi; sub += kWG) {
@@ -579,14 +592,21 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
             const uint32_t vs = V[s];
             const uint32_t key = a.slabs ? K[s] : 0u;
             const uint32_t orig = a.slabs ? (vs & ~kFirstBit) : vs;
-            float lo[3], hi[3], c4[4];
-            load_lohi(a.in, KIND, orig, lo, hi, c4);
-            float pl, ph;
-            project_lohi(KIND, lo, hi, c4, u, exact, &pl, &ph);
+            // an unused second entry (slab mode) overlaps nothing: its box is
+            // not read
+            const bool unused = a.slabs && key == kSlabNull;
+            float lo[3], hi[3], c4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pl = 0.0f, ph = 0.0f;
+            if (!unused) {
+                load_lohi(a.in, KIND, orig, lo, hi, c4);
+                project_lohi(KIND, lo, hi, c4, u, exact, &pl, &ph);
+            } else {
+                lo[0] = lo[1] = lo[2] = hi[0] = hi[1] = hi[2] = 0.0f;
+            }
             // first t in [s+1, seg_end) with K[t] > hk
-            const uint32_t hk = a.slabs ? ((key & 0xFF000000u) | qkey(ph, a.slab)) : flip_f32(ph);
+            const uint32_t hk = a.slabs ? (((key >> kSlabQBits) << kSlabQBits) | qkey(ph, sp))
+                                        : flip_f32(ph);
             uint32_t b = s + 1, e = ti.seg_end;
-            if (a.slabs) e = key == kSlabNull ? b : s_sb[(key >> 24) + 1];  // within the slab
+            if (a.slabs) e = unused ? b : s_sb[(key >> kSlabQBits) + 1];  // within the slab
             while (b < e) {
                 const uint32_t mid = b + ((e - b) >> 1);
                 if (K[mid] <= hk) b = mid + 1; else e = mid;
@@ -603,31 +623,39 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
             r.orig = vs;  // slab mode: with kFirstBit
             r.aux = 0u;
             if (a.slabs) {
-                if (key == kSlabNull) {  // a box that overlaps nothing
+                if (unused) {  // a box that overlaps nothing
                     if constexpr (KIND == 0) {
                         r.lo[0] = r.lo[1] = r.lo[2] = __builtin_inff();
                         r.hi[0] = r.hi[1] = r.hi[2] = -__builtin_inff();
                     } else {
-                        r.c[0] = __builtin_inff();
+                        r.c[0] = r.c[1] = r.c[2] = __builtin_inff();
+                        r.c[3] = 0.0f;
                     }
                     r.orig = orig;
                     r.aux = s;
                 } else {
                     // first position of this entry's slab: candidates of another
                     // slab are told apart by it in the sweep
-                    r.aux = s_sb[key >> 24];
+                    r.aux = s_sb[key >> kSlabQBits];
                 }
             }
             reinterpret_cast<Rec*>(a.rec)[s] = r;
         }
         // place in the sub-tile ordered by (sweep end, position): the number
         // of smaller keys (keys are unique)
+        // (only in sub-tiles with a window longer than kShortWindow: the
+        // sweep takes every other sub-tile one object per lane, in any order,
+        // so they keep sorted order and skip the ranking)
         const unsigned long long mk = ((unsigned long long)end << 8) | tid;
         s_key[tid] = mk;
-        __syncthreads();
-        uint32_t rank = 0;
+        const uint32_t wlen = (valid && end > s + 1u) ? end - s - 1u : 0u;
+        const int any_long = __syncthreads_or(wlen > kShortWindow ? 1 : 0);
+        uint32_t rank = tid;
+        if (any_long) {
+            rank = 0;
 #pragma unroll 8
-        for (uint32_t j = 0; j < kWG; ++j) rank += s_key[j] < mk ? 1u : 0u;
+            for (uint32_t j = 0; j < kWG; ++j) rank += s_key[j] < mk ? 1u : 0u;
+        }
         if (valid) {
             r.aux = s;
             reinterpret_cast<Rec*>(a.grp)[sub + rank] = r;
@@ -804,6 +832,73 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         const uint32_t nobj = uniform(min(64u, hi - g0));
         const uint32_t ge = lane < nobj ? a.grp_end[g0 + lane] : 0u;  // ascending in lane order
         const uint32_t wend = uniform(wave_max_u32(ge));
+        // Short windows (every object of the wave has at most kShortWindow
+        // candidates: sparse scenes, slabs of small scenes): each lane takes
+        // its own object and walks its window, one candidate per step (the
+        // lanes' candidates are close together in sorted order), instead of
+        // testing every object against whole 128-candidate chunk pairs.  Hits
+        // are staged by ballot exactly as below; in slab mode a window never
+        // leaves its slab, so only the first-slab rule applies.
+        Rec me{};
+        uint32_t my_s = 0u, len = 0u;
+        if (lane < nobj) {
+            me = reinterpret_cast<const Rec*>(a.grp)[g0 + lane];
+            my_s = me.aux;
+            len = ge > my_s + 1u ? ge - my_s - 1u : 0u;
+        }
+        const uint32_t maxlen = uniform(wave_max_u32(len));
+        if (maxlen <= kShortWindow) {
+            const uint32_t myorig = SLAB ? (me.orig & ~kFirstBit) : me.orig;
+            const uint32_t mygrp = (GROUP && lane < nobj) ? a.group[myorig] : 0u;
+            for (uint32_t k = 1u; k <= maxlen; ++k) {
+                const bool act = k <= len;
+                bool hit = false;
+                uint32_t corig = 0u;
+                if (act) {
+                    const Rec c = R[my_s + k];
+                    if constexpr (KIND == 0) {
+                        const float d0 = c.lo[0] - me.hi[0], d1 = me.lo[0] - c.hi[0];
+                        const float d2 = c.lo[1] - me.hi[1], d3 = me.lo[1] - c.hi[1];
+                        const float d4 = c.lo[2] - me.hi[2], d5 = me.lo[2] - c.hi[2];
+                        const float mx = fmaxf(fmaxf(fmaxf(d0, d1), d2), fmaxf(fmaxf(d3, d4), d5));
+                        hit = mx <= 0.0f;
+                    } else {
+#pragma clang fp contract(off)
+                        const float x0 = me.c[0] - c.c[0];
+                        const float y0 = me.c[1] - c.c[1];
+                        const float z0 = me.c[2] - c.c[2];
+                        const float r0 = me.c[3] + c.c[3];
+                        const float t1 = (x0 * x0 + y0 * y0) + z0 * z0;
+                        const float e1 = r0 * r0;
+                        hit = t1 < e1;
+                    }
+                    corig = SLAB ? (c.orig & ~kFirstBit) : c.orig;
+                    if constexpr (SLAB) hit = hit && ((me.orig | c.orig) & kFirstBit) != 0u;
+                    if constexpr (GROUP) hit = hit && a.group[corig] != mygrp;
+                }
+                const unsigned long long m = ballot(hit);
+                if (m == 0ull) continue;
+                const uint32_t cnt = (uint32_t)__popcll(m) * mult;
+                if constexpr (MODE == 2) {
+                    npairs += cnt;
+                } else {
+                    if (fill + cnt > kCap) {  // stage full: write it out
+                        write_out(stage, fill);
+                        npairs += fill;
+                        fill = 0;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    if (hit) {
+                        const uint32_t r = fill + mult * mbcnt(m);
+                        stage[r] = make_uint2(myorig, corig);
+                        if (MODE == 1) stage[r + 1] = make_uint2(corig, myorig);
+                    }
+                    fill += cnt;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            ex = wave_sum((unsigned long long)(lane < nobj ? maxlen : 0u));
+        } else {
         CRec* __restrict__ O = (CRec*)(a.grp) + g0;
         // Slab mode: a candidate past an object's sweep end may belong to the
         // next slab (and overlap it).  That only happens in a wave whose
@@ -971,6 +1066,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         // 64-lane chunks (an upper bound: the wave's last pair may be single)
         if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 127u) / 128u) * 128u;
         ex = wave_sum(ex);
+        }  // long windows
     }
 
     // Hand this wave's remainder and counters to the workgroup; the last wave
@@ -996,8 +1092,9 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         tot += s_fill[v];
     }
     if (lane == 0) {
-        if (np) atomicAdd(&a.ctrl->num_pairs, np);
-        if (ext) atomicAdd(&a.ctrl->exec_tests, ext);
+        // pairs of a writing sweep are the sum of its slot claims (k_pair_regions)
+        if (MODE == 2 && np) atomicAdd(&a.ctrl->part[0][blockIdx.x % kMaxRegions], np);
+        if (ext) atomicAdd(&a.ctrl->part[1][blockIdx.x % kMaxRegions], ext);
     }
     if constexpr (MODE != 2) {
         if (tot) {
@@ -1085,14 +1182,20 @@ __global__ __launch_bounds__(kWG) void k_entries_write(uint32_t* has, const uint
 __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long* d_counts,
                          const float* axis, unsigned long long* axis_out) {
     if (threadIdx.x == 0) {
-        const unsigned long long np = ctrl->num_pairs;
+        unsigned long long np = ctrl->num_pairs, ex = 0;
+        for (int r = 0; r < kMaxRegions; ++r) {
+            np += ctrl->part[0][r];
+            ex += ctrl->part[1][r];
+            ctrl->part[0][r] = 0;
+            ctrl->part[1][r] = 0;
+        }
         uint32_t fl = ctrl->flags;
         if (np > cap) fl |= kFlagCapacity;
         d_counts[0] = np;
         d_counts[1] = ctrl->num_entries;
         d_counts[2] = ctrl->tests;
         d_counts[3] = fl;
-        d_counts[4] = ctrl->exec_tests;
+        d_counts[4] = ex;
         if (axis_out) {  // the (first) scene's axis and its source, as float bits
             axis_out[0] = (unsigned long long)__float_as_uint(axis[0]) |
                           ((unsigned long long)__float_as_uint(axis[1]) << 32);
@@ -1102,7 +1205,6 @@ __global__ void k_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
         ctrl->num_pairs = 0;
         ctrl->num_entries = 0;
         ctrl->tests = 0;
-        ctrl->exec_tests = 0;
         for (int r = 0; r < kMaxRegions; ++r) ctrl->region_fill[r] = 0;
         ctrl->flags = 0;
     }
@@ -1122,6 +1224,7 @@ __global__ void k_pair_regions(Ctrl* ctrl, unsigned long long region_cap, uint32
     // sorted buckets.
     unsigned long long found = 0;
     for (uint32_t r = 0; r < nregions; ++r) found += ctrl->region_fill[r];
+    ctrl->num_pairs = found;  // every pair the sweep found was claimed a slot
     const bool skip = found > region_cap;
     unsigned long long run = 0, real = 0;
     for (uint32_t r = 0; r < nregions; ++r) {
@@ -1159,7 +1262,7 @@ __global__ void k_scene_offsets(const uint2* __restrict__ P, const Ctrl* ctrl, u
 // ---------------------------------------------------------------- launchers
 hipError_t launch_slab_params(const ObjPipeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_vrange, dim3(a.ntiles), dim3(kWG), 0, st, a);
-    hipLaunchKernelGGL(k_slab_params, dim3(1), dim3(kWG), 0, st, a);
+    hipLaunchKernelGGL(k_slab_params, dim3(a.seg.tile_seg ? a.seg.nseg : 1u), dim3(kWG), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st) {

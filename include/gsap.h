@@ -75,8 +75,8 @@ enum {
      * SapKernel.cu:150-213 and 2634-2678, generalised to up to 255 slabs with no
      * cross pass): each box is keyed into the one or two slabs its projection
      * touches and a pair is reported only in the slab of the larger lower bound.
-     * Same pair set, far fewer sweep tests on spread-out scenes.  Single-scene
-     * updates only; n <= 2^30. */
+     * Same pair set, far fewer sweep tests on spread-out scenes.  Batched
+     * updates split each scene with its own slab axis and parameters; n <= 2^30. */
     GSAP_FLAG_SLABS = 2u,
     /* Temporal coherence for successive updates of one object set (the
      * reference carries only its cell interval between frames, Sap.cpp:226-243).
@@ -212,7 +212,7 @@ typedef struct gsap_tune {
     int32_t pair_items;  /* pair sort: keys per thread, 8/16                              */
     int32_t pair_bits;   /* pair sort: digit width, 6/7/8                                 */
     int32_t pair_grid;   /* pair sort: persistent workgroups per pass, 256..16384         */
-    int32_t slabs;       /* 1: slab sweep (GSAP_FLAG_SLABS) for single-scene updates      */
+    int32_t slabs;       /* 1: slab sweep (GSAP_FLAG_SLABS), per scene when batched        */
     int32_t sweep_chunk; /* sweep: sub-tiles per XCD chunk, 0 (round-robin)/4/16/64       */
     int32_t pair_regions;/* sweep output regions with one allocation counter each, 1 or 8;
                           * 8 holds 8x the pair capacity in device memory              */

@@ -129,12 +129,37 @@ def test_particle_step_identical(dev):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
-def test_batched_refuses_slabs(dev):
-    boxes, off = W.scenes(4, seed=1, lo=100, hi=200)
-    s = G.Sap(flags=G.FLAG_SLABS)
-    with pytest.raises(G.GsapError) as e:
-        s.collide_batched(to_dev(boxes, dev), off, None, None)
-    assert e.value.code == G.ERR_UNSUPPORTED
+@pytest.mark.parametrize("mode", [G.AXIS_AUTO, G.AXIS_PCA])
+def test_batched_slabs_equal_brute_per_scene(dev, mode):
+    """Batched slab sweep (each scene with its own slab axis and parameters):
+    every scene's pairs equal brute force, on the captured update and its
+    replays, with fewer sweep tests than the batched plain sweep; scenes of
+    1 and 2 objects and a scene that fills one slab included."""
+    import torch
+    boxes, off = W.scenes(24, seed=7, lo=200, hi=6000)
+    extra = [W.small(1, seed=1), W.small(2, seed=2)]
+    one = W.small(300, seed=3)
+    one[:, :3] *= 0.01
+    one[:, 3:] = one[:, :3] + 0.5  # all overlapping: one slab
+    parts = [boxes] + extra + [one]
+    boxes = np.concatenate(parts).astype(np.float32)
+    off = np.concatenate([off, off[-1] + np.cumsum([len(p) for p in parts[1:]])]).astype(np.uint32)
+    bd = to_dev(boxes, dev)
+    refs = [O.brute(boxes[off[k]:off[k + 1]]) + off[k] for k in range(len(off) - 1)]
+    total = sum(len(r) for r in refs)
+    plain = G.Sap(axis_mode=mode)
+    _, st_plain = plain.collide_batched(bd, off, None, None)
+    s = G.Sap(axis_mode=mode, flags=G.FLAG_SLABS)
+    pairs = torch.empty((total + 16, 2), dtype=torch.int32, device=dev)
+    spo = torch.empty(len(off), dtype=torch.int64, device=dev)
+    for rep in range(3):
+        got_n, st = s.collide_batched(bd, off, pairs, spo)
+        assert got_n == total
+        allp = pairs_np(pairs[:got_n])
+        so = spo.cpu().numpy()
+        for k in range(len(off) - 1):
+            assert np.array_equal(allp[so[k]:so[k + 1]], refs[k]), (rep, k)
+    assert st.sweep_tests < st_plain.sweep_tests
 
 
 def test_full_size_bench_config_identical(dev):

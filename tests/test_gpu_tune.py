@@ -89,8 +89,8 @@ def test_autotune_batched_and_grouped(dev):
     o0 = torch.empty(len(off), dtype=torch.int64, device=dev)
     base.collide_batched(bd, off, p0, o0)
     s = G.Sap()
-    t = s.autotune(bd, scene_offsets=off)
-    assert t["slabs"] == 0 and t["from_cache"] == 0
+    t = s.autotune(bd, scene_offsets=off)  # may pick the per-scene slab sweep
+    assert t["from_cache"] == 0
     p1 = torch.empty_like(p0)
     o1 = torch.empty_like(o0)
     s.collide_batched(bd, off, p1, o1)
