@@ -345,6 +345,13 @@ hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long
                              int L, const uint32_t* st, hipStream_t stream);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
+// One-word stage entries (inputs of fewer than 2^26 entries): the object's
+// slot in the wave (6 bits) above the candidate's index (26 bits).  The stage
+// fills the LDS of six workgroups per CU, the residency the sweep's registers
+// allow.
+constexpr uint32_t kStage4Shift = 26u;
+constexpr uint32_t kStage4Mask = (1u << kStage4Shift) - 1u;
+constexpr uint32_t kSweepStage4 = 1280;
 constexpr uint32_t kShortWindow = 64;      // sweep: lane-per-object path up to this window
 constexpr uint32_t kSweepHistPasses = 2;   // pair-sort digit histograms the sweep can count
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.

@@ -695,7 +695,7 @@ constexpr uint32_t kStage = kSweepStage;
 // object's slab (its slab starts at or before the object) and one of the two
 // entries to be its box's first slab (the pair's slab: that of the larger
 // lower bound), so each pair is found exactly once.
-template <int KIND, bool GROUP, int MODE, bool SLAB>  // MODE 0 canonical, 1 symmetric, 2 count
+template <int KIND, bool GROUP, int MODE, bool SLAB, bool S4>  // MODE 0 canonical, 1 symmetric, 2 count
 __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
     // The object records are read-only here; the constant address space lets
@@ -705,12 +705,21 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 #else
     typedef const Rec CRec;
 #endif
-    // the slab sweep emits denser hits, so it gets the larger stage (seven
-    // workgroups of 4 waves, as many as the registers allow, still fit a
-    // CU's LDS either way)
-    constexpr uint32_t kCap = SLAB ? kSweepStageSlab : kStage;
+    // Staged hits.  An 8-byte entry holds (object, candidate); symmetric mode
+    // stages both orientations, two entries per hit.  With S4 (fewer than
+    // 2^26 entries) an entry is one word -- the object's slot among the
+    // wave's 64 objects above the candidate's index -- and the flush expands
+    // it (symmetric mode: into both orientations) through the wave's table of
+    // object indices.  Each flush makes one contended slot claim, and the
+    // flushes are most of what emitting pairs costs, so the same LDS holding
+    // two to three times the hits per flush makes the sweep faster.
+    using SE = typename std::conditional<S4, uint32_t, uint2>::type;
+    constexpr uint32_t kCap = S4 ? kSweepStage4 : (SLAB ? kSweepStageSlab : kStage);
     constexpr uint32_t kSt = MODE == 2 ? 1u : kCap;
-    __shared__ uint2 s_stage[kWaves][kSt];
+    constexpr uint32_t kPer = (MODE == 1 && !S4) ? 2u : 1u;  // entries per hit
+    constexpr uint32_t kOut = (MODE == 1 && S4) ? 2u : 1u;   // pairs written per entry
+    __shared__ SE s_stage[kWaves][kSt];
+    __shared__ uint32_t s_wobj[kWaves][S4 ? 64 : 1];  // S4: the wave's object indices
     // digit histograms of the pairs written, for the pair sort's first
     // passes; with a.joint: the first digit's histogram, then the per-part
     // second-digit counts as 16-bit halves of kJointWords words
@@ -742,10 +751,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     // directed pairs when symmetric (also in the pre-count)
     const uint32_t mult = (MODE == 1 || (MODE == 2 && a.sym)) ? 2u : 1u;
     const unsigned long long cap = a.slot_cap;
-    uint2* stage = s_stage[w];
+    SE* stage = s_stage[w];
 
     uint32_t npairs = 0;  // pairs (or directed pairs) found by this wave (< 64 * 2n)
-    uint32_t fill = 0;    // staged pairs of this wave
+    uint32_t fill = 0;    // staged entries of this wave
     unsigned long long ex = 0;
 
     // Pair region of this workgroup: with several regions, the workgroups of
@@ -758,12 +767,12 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         if (lane == 0) braw = atomicAdd(&a.ctrl->region_fill[region], (unsigned long long)cnt);
         return braw;
     };
-    // Per-part counts: each wave adds at most kJointWave + kCap to a 16-bit
-    // half before it flushes them all (atomic exchanges, so the other waves'
-    // concurrent adds are kept), so no half exceeds kWaves * (kJointWave +
-    // kCap) < 2^16 between flushes.
-    constexpr uint32_t kJointWave = 15000u;
-    static_assert(kWaves * (kJointWave + kSweepStageSlab) < 65536u, "16-bit joint counts");
+    // Per-part counts: each wave adds at most kJointWave + kCap * kOut to a
+    // 16-bit half before it flushes them all (atomic exchanges, so the other
+    // waves' concurrent adds are kept), so no half exceeds kWaves *
+    // (kJointWave + kCap * kOut) < 2^16 between flushes.
+    constexpr uint32_t kJointWave = 65535u / kWaves - kCap * kOut - 1u;
+    static_assert(kWaves * (kJointWave + kCap * kOut) < 65536u, "16-bit joint counts");
     uint32_t jcount = 0;  // pairs this wave counted since it last flushed them
     auto flush_joint = [&]() {
         if constexpr (MODE != 2) {
@@ -774,26 +783,39 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             }
         }
     };
-    auto count_digits = [&](const uint2* st, uint32_t cnt) {
+    // entry i of a stage -> (object, candidate); tab: that wave's object table
+    auto entry = [&](const SE* st, const uint32_t* tab, uint32_t i) {
+        if constexpr (S4) {
+            const uint32_t e = st[i];
+            return make_uint2(tab[e >> kStage4Shift], e & kStage4Mask);
+        } else {
+            (void)tab;
+            return st[i];
+        }
+    };
+    auto count_key = [&](uint2 pr) {
+        const unsigned long long k = ((unsigned long long)pr.x << a.bbits) | (unsigned long long)pr.y;
+        if (a.joint) {
+            const uint32_t d1 = (uint32_t)(k >> a.hist_shift0) & 255u;
+            const uint32_t j = ((d1 >> 5) << 8) | ((uint32_t)(k >> (a.hist_shift0 + 8)) & 255u);
+            atomicAdd(&s_ph[d1], 1u);
+            atomicAdd(&s_ph[256u + (j >> 1)], 1u << ((j & 1u) << 4));
+        } else {
+            for (int q = 0; q < a.hist_npass; ++q)
+                atomicAdd(&s_ph[q * 256 + (uint32_t)((k >> (a.hist_shift0 + 8 * q)) & 255u)], 1u);
+        }
+    };
+    auto count_digits = [&](const SE* st, const uint32_t* tab, uint32_t cnt) {
         if constexpr (MODE != 2) {
             if (a.hist) {
                 for (uint32_t i = lane; i < cnt; i += 64u) {
-                    uint2 pr = st[i];
+                    uint2 pr = entry(st, tab, i);
                     if constexpr (MODE == 0) pr = make_uint2(min(pr.x, pr.y), max(pr.x, pr.y));
-                    const unsigned long long k =
-                        ((unsigned long long)pr.x << a.bbits) | (unsigned long long)pr.y;
-                    if (a.joint) {
-                        const uint32_t d1 = (uint32_t)(k >> a.hist_shift0) & 255u;
-                        const uint32_t j = ((d1 >> 5) << 8) | ((uint32_t)(k >> (a.hist_shift0 + 8)) & 255u);
-                        atomicAdd(&s_ph[d1], 1u);
-                        atomicAdd(&s_ph[256u + (j >> 1)], 1u << ((j & 1u) << 4));
-                    } else {
-                        for (int q = 0; q < a.hist_npass; ++q)
-                            atomicAdd(&s_ph[q * 256 + (uint32_t)((k >> (a.hist_shift0 + 8 * q)) & 255u)], 1u);
-                    }
+                    count_key(pr);
+                    if constexpr (kOut == 2) count_key(make_uint2(pr.y, pr.x));
                 }
                 if (a.joint) {
-                    jcount += cnt;
+                    jcount += cnt * kOut;
                     if (jcount > kJointWave) {
                         flush_joint();
                         jcount = 0;
@@ -802,12 +824,16 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             }
         }
     };
-    auto store = [&](const uint2* st, uint32_t cnt, unsigned long long base) {
+    auto store = [&](const SE* st, const uint32_t* tab, uint32_t cnt, unsigned long long base) {
         for (uint32_t i = lane; i < cnt; i += 64u) {
-            if (base + i < cap) {
-                uint2 pr = st[i];
-                if constexpr (MODE == 0) pr = make_uint2(min(pr.x, pr.y), max(pr.x, pr.y));
-                rout[base + i] = pr;
+            uint2 pr = entry(st, tab, i);
+            if constexpr (MODE == 0) pr = make_uint2(min(pr.x, pr.y), max(pr.x, pr.y));
+            if constexpr (kOut == 2) {
+                const unsigned long long s = base + 2ull * i;
+                if (s < cap) rout[s] = pr;
+                if (s + 1u < cap) rout[s + 1u] = make_uint2(pr.y, pr.x);
+            } else {
+                if (base + i < cap) rout[base + i] = pr;
             }
         }
     };
@@ -815,17 +841,35 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         return (unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)v) |
                ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32);
     };
-    // Write the staged pairs st[0, cnt) of this wave (cnt > 0, wave-uniform):
-    // claim cnt slots of the workgroup's region, count the pairs' sort digits
-    // in LDS while the claim is in flight, then store them (within the
-    // capacity).  Canonical mode stages (object, candidate) and orders the
-    // pair here, once per pair rather than once per tested chunk.  (After a
-    // capacity overflow the digit counts include unwritten pairs; the update
-    // then reports the overflow and skips the pair sort.)
-    auto write_out = [&](const uint2* st, uint32_t cnt) {
-        const unsigned long long braw = claim(cnt);
-        count_digits(st, cnt);
-        store(st, cnt, first_lane(braw));
+    // Write the staged entries st[0, cnt) of a wave (cnt > 0, wave-uniform):
+    // claim the slots of their pairs in the workgroup's region, count the
+    // pairs' sort digits in LDS while the claim is in flight, then store them
+    // (within the capacity).  Canonical mode stages (object, candidate) and
+    // orders the pair here, once per pair rather than once per tested chunk.
+    // (After a capacity overflow the digit counts include unwritten pairs;
+    // the update then reports the overflow and skips the pair sort.)
+    auto write_out = [&](const SE* st, const uint32_t* tab, uint32_t cnt) {
+        const unsigned long long braw = claim(cnt * kOut);
+        count_digits(st, tab, cnt);
+        store(st, tab, cnt, first_lane(braw));
+    };
+    const uint32_t* const wtab = s_wobj[w];
+    auto flush = [&]() {  // write this wave's stage out (fill > 0)
+        write_out(stage, wtab, fill);
+        npairs += fill * kOut;
+        fill = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
+    // stage this lane's hit (object in wave slot q, candidate) as entry r
+    auto put = [&](uint32_t r, uint32_t q, uint32_t oorig, uint32_t corig) {
+        if constexpr (S4) {
+            (void)oorig;
+            stage[r] = (q << kStage4Shift) | corig;
+        } else {
+            (void)q;
+            stage[r] = make_uint2(oorig, corig);
+            if constexpr (MODE == 1) stage[r + 1] = make_uint2(corig, oorig);
+        }
     };
 
     if (g0 < hi) {
@@ -846,6 +890,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             my_s = me.aux;
             len = ge > my_s + 1u ? ge - my_s - 1u : 0u;
         }
+        if constexpr (S4) s_wobj[w][lane] = SLAB ? (me.orig & ~kFirstBit) : me.orig;
         const uint32_t maxlen = uniform(wave_max_u32(len));
         if (maxlen <= kShortWindow) {
             const uint32_t myorig = SLAB ? (me.orig & ~kFirstBit) : me.orig;
@@ -878,21 +923,12 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 }
                 const unsigned long long m = ballot(hit);
                 if (m == 0ull) continue;
-                const uint32_t cnt = (uint32_t)__popcll(m) * mult;
                 if constexpr (MODE == 2) {
-                    npairs += cnt;
+                    npairs += (uint32_t)__popcll(m) * mult;
                 } else {
-                    if (fill + cnt > kCap) {  // stage full: write it out
-                        write_out(stage, fill);
-                        npairs += fill;
-                        fill = 0;
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                    if (hit) {
-                        const uint32_t r = fill + mult * mbcnt(m);
-                        stage[r] = make_uint2(myorig, corig);
-                        if (MODE == 1) stage[r + 1] = make_uint2(corig, myorig);
-                    }
+                    const uint32_t cnt = (uint32_t)__popcll(m) * kPer;  // entries
+                    if (fill + cnt > kCap) flush();  // stage full: write it out
+                    if (hit) put(fill + kPer * mbcnt(m), lane, myorig, corig);
                     fill += cnt;
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -967,7 +1003,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 
             // The exact test is the only per-lane arithmetic on the common path;
             // the other conditions are lane masks (scalar ANDs) or uniform branches.
-            auto one = [&](const Rec& o, const Chunk& ck, auto masked_tag, auto mixed_tag) {
+            // oorig, fsel (all lanes when the object's entry is its box's first
+            // slab, else none) and q (the object's slot in the wave) are per object.
+            auto one = [&](const Rec& o, uint32_t q, uint32_t oorig, unsigned long long fsel,
+                           const Chunk& ck, auto masked_tag, auto mixed_tag) {
                 constexpr bool MASKED = decltype(masked_tag)::value;
                 constexpr bool MIXED = decltype(mixed_tag)::value;
                 const Rec& c = ck.c;
@@ -1000,56 +1039,42 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 // (scalar ANDs; a ballot of a combined bool would be rebuilt
                 // through VGPRs), and the lanes that stage a pair are selected
                 // by the mask itself (inverse ballot: the mask becomes exec).
-                const uint32_t oorig = SLAB ? (o.orig & ~kFirstBit) : o.orig;
                 unsigned long long m = ballot(hit);
                 if (MASKED) m &= ballot(t > o.aux);  // candidates after the object
                 if constexpr (SLAB) {
                     if (MIXED) m &= ballot(c.aux <= o.aux);  // candidate in the object's slab
-                    // one of them is a first slab
-                    m &= (o.orig & kFirstBit) ? ~0ull : cfm;
+                    m &= fsel | cfm;  // one of them is a first slab
                 }
                 if (GROUP) m &= ballot(a.group[oorig] != gc);
                 if (m == 0ull) return;
-                const uint32_t cnt = (uint32_t)__popcll(m) * mult;
                 if constexpr (MODE == 2) {
-                    npairs += cnt;
+                    npairs += (uint32_t)__popcll(m) * mult;
                 } else {
-                    if (fill + cnt > kCap) {  // stage full: write it out
-                        write_out(stage, fill);
-                        npairs += fill;
-                        fill = 0;
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                    if (__builtin_amdgcn_inverse_ballot_w64(m)) {
-                        const uint32_t r = fill + mult * mbcnt(m);
-                        if (MODE == 0) {
-                            stage[r] = make_uint2(oorig, corig);
-                        } else {
-                            stage[r] = make_uint2(oorig, corig);
-                            stage[r + 1] = make_uint2(corig, oorig);
-                        }
-                    }
+                    const uint32_t cnt = (uint32_t)__popcll(m) * kPer;  // entries
+                    if (fill + cnt > kCap) flush();  // stage full: write it out
+                    if (__builtin_amdgcn_inverse_ballot_w64(m)) put(fill + kPer * mbcnt(m), q, oorig, corig);
                     fill += cnt;
                     __builtin_amdgcn_wave_barrier();
                 }
             };
-            auto process = [&](const Rec& o, auto masked_tag, auto mixed_tag) {
-                one(o, k1, masked_tag, mixed_tag);
-                if (two) one(o, k2, masked_tag, mixed_tag);
+            auto process = [&](const Rec& o, uint32_t q, auto masked_tag, auto mixed_tag) {
+                const uint32_t oorig = SLAB ? (o.orig & ~kFirstBit) : o.orig;
+                const unsigned long long fsel = (SLAB && !(o.orig & kFirstBit)) ? 0ull : ~0ull;
+                one(o, q, oorig, fsel, k1, masked_tag, mixed_tag);
+                if (two) one(o, q, oorig, fsel, k2, masked_tag, mixed_tag);
             };
             auto objects = [&](auto masked_tag, auto mixed_tag) {
-                CRec* p = O + kfirst;
-                CRec* const pe = O + nobj;
-                for (; p + 4 <= pe; p += 4) {  // four scalar loads in flight
-                    const Rec o0 = p[0], o1 = p[1], o2 = p[2], o3 = p[3];
-                    process(o0, masked_tag, mixed_tag);
-                    process(o1, masked_tag, mixed_tag);
-                    process(o2, masked_tag, mixed_tag);
-                    process(o3, masked_tag, mixed_tag);
+                uint32_t q = kfirst;
+                for (; q + 4u <= nobj; q += 4u) {  // four scalar loads in flight
+                    const Rec o0 = O[q], o1 = O[q + 1u], o2 = O[q + 2u], o3 = O[q + 3u];
+                    process(o0, q, masked_tag, mixed_tag);
+                    process(o1, q + 1u, masked_tag, mixed_tag);
+                    process(o2, q + 2u, masked_tag, mixed_tag);
+                    process(o3, q + 3u, masked_tag, mixed_tag);
                 }
-                for (; p < pe; ++p) {
-                    const Rec o = *p;
-                    process(o, masked_tag, mixed_tag);
+                for (; q < nobj; ++q) {
+                    const Rec o = O[q];
+                    process(o, q, masked_tag, mixed_tag);
                 }
             };
             // an object may sit at or after the chunk start only in the first
@@ -1071,7 +1096,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 
     // Hand this wave's remainder and counters to the workgroup; the last wave
     // to arrive writes them all out with one atomic per counter.
-    if constexpr (MODE != 2) npairs += fill;  // the staged remainder
+    if constexpr (MODE != 2) npairs += fill * kOut;  // the staged remainder
     if (lane == 0) {
         s_fill[w] = fill;
         s_np[w] = npairs;
@@ -1098,15 +1123,15 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     }
     if constexpr (MODE != 2) {
         if (tot) {
-            const unsigned long long braw = claim(tot);
+            const unsigned long long braw = claim(tot * kOut);
 #pragma unroll
-            for (int v = 0; v < kWaves; ++v) count_digits(s_stage[v], s_fill[v]);
+            for (int v = 0; v < kWaves; ++v) count_digits(s_stage[v], s_wobj[v], s_fill[v]);
             unsigned long long base = first_lane(braw);
 #pragma unroll
             for (int v = 0; v < kWaves; ++v) {
                 const uint32_t f = s_fill[v];
-                store(s_stage[v], f, base);
-                base += f;
+                store(s_stage[v], s_wobj[v], f, base);
+                base += (unsigned long long)f * kOut;
             }
         }
         if (a.hist) {  // every wave's digits are counted (they arrived after counting)
@@ -1286,10 +1311,20 @@ hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* K, const uint32_t
 template <int KIND, bool GROUP, int MODE>
 static void sweep_launch(const SweepArgs& a, hipStream_t st) {
     const uint32_t blocks = a.ntiles * (a.tile / kSweepTile);
-    if (a.slabs)
-        hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, true>), dim3(blocks), dim3(kWG), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, false>), dim3(blocks), dim3(kWG), 0, st, a);
+    // one-word stage entries while every candidate index fits below the
+    // object slot (entry indices bound the box indices)
+    const bool s4 = MODE != 2 && a.n <= kStage4Mask;
+    if (a.slabs) {
+        if (s4)
+            hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, true, true>), dim3(blocks), dim3(kWG), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, true, false>), dim3(blocks), dim3(kWG), 0, st, a);
+    } else {
+        if (s4)
+            hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, false, true>), dim3(blocks), dim3(kWG), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_sweep<KIND, GROUP, MODE, false, false>), dim3(blocks), dim3(kWG), 0, st, a);
+    }
 }
 
 template <int KIND, bool GROUP>
