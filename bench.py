@@ -267,33 +267,43 @@ def bench_batched(G, W, torch, dev, ns, steps, warmup, cache=None):
 
 
 def bench_learned(G, W, torch, dev, ns):
-    """2048 packed scenes (seed 5): sweep tests and update time with the PCA
-    axis against the trained selector; the pair lists must be identical."""
+    """2048 packed scenes (seed 5), plain sweep and slab sweep: sweep tests and
+    update time with the PCA axis (pair), the trained selector (plain: the
+    sweep axis; slabs: the sweep plane) and the closed-form automatic rule; the
+    pair lists must be identical."""
     from gsap_amd import learn as L
-    if not os.path.exists(L.DEFAULT_MODEL):
-        return {"skipped": "no trained model at " + os.path.relpath(L.DEFAULT_MODEL, ROOT)}
-    model = G.AxisModel.load(L.DEFAULT_MODEL)
     boxes, off = W.scenes(ns, seed=5)
     bd = torch.from_numpy(boxes).to(dev)
-    ev = L.evaluate(model, bd, off, conf_min=0.95)
-    ms, tests = {}, {}
-    for name, mode in (("pca", G.AXIS_PCA), ("learned", G.AXIS_LEARNED), ("auto", G.AXIS_AUTO)):
-        s = G.Sap(max_n=int(off[-1]), axis_mode=mode)
-        if mode == G.AXIS_LEARNED:
-            s.set_learned_model(model, 0.95)
-        P, st = s.collide_batched(bd, off, None, None)
-        tests[name] = int(st.sweep_tests)
-        pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
-        po = torch.empty(ns + 1, dtype=torch.int64, device=dev)
-        ms[name] = _time_ms(torch, lambda: s.collide_batched(bd, off, pairs, po), 5, 2)
-        s.close()
-    return {"workload": f"{ns} scenes, n log-U[1024,65536] seed 5", "boxes": int(off[-1]),
-            "pairs": ev["pairs"], "sweep_tests_pca": ev["tests_pca"],
-            "sweep_tests_learned": ev["tests_learned"], "reduction": round(ev["reduction"], 4),
-            "pairs_identical": ev["pairs_identical"], "ms_batched_pca": round(ms["pca"], 4),
-            "ms_batched_learned": round(ms["learned"], 4),
-            # the cost-model choice of PCA or a coordinate axis, for comparison
-            "sweep_tests_auto": tests["auto"], "ms_batched_auto": round(ms["auto"], 4)}
+    out = {"workload": f"{ns} scenes, n log-U[1024,65536] seed 5", "boxes": int(off[-1])}
+    for slabs in (False, True):
+        path = L.default_model(slabs)
+        if not os.path.exists(path):
+            rec = {"skipped": "no trained model at " + os.path.relpath(path, ROOT)}
+        else:
+            model = G.AxisModel.load(path)
+            ev = L.evaluate(model, bd, off, conf_min=0.95, slabs=slabs)
+            fl = G.FLAG_SLABS if slabs else 0
+            ms = {}
+            for name, mode in (("pca", G.AXIS_PCA), ("learned", G.AXIS_LEARNED), ("auto", G.AXIS_AUTO)):
+                s = G.Sap(max_n=int(off[-1]), axis_mode=mode, flags=fl)
+                if mode == G.AXIS_LEARNED:
+                    s.set_learned_model(model, 0.95)
+                P, _ = s.collide_batched(bd, off, None, None)
+                pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
+                po = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+                ms[name] = _time_ms(torch, lambda: s.collide_batched(bd, off, pairs, po), 5, 2)
+                s.close()
+            rec = {"pairs": ev["pairs"], "sweep_tests_pca": ev["tests_pca"],
+                   "sweep_tests_learned": ev["tests_learned"], "sweep_tests_auto": ev["tests_auto"],
+                   "reduction": round(ev["reduction"], 4),
+                   "reduction_vs_auto": round(ev["reduction_vs_auto"], 4),
+                   "pairs_identical": ev["pairs_identical"], "ms_batched_pca": round(ms["pca"], 4),
+                   "ms_batched_learned": round(ms["learned"], 4), "ms_batched_auto": round(ms["auto"], 4)}
+        if slabs:
+            out["slabs"] = rec
+        else:
+            out.update(rec)
+    return out
 
 
 def _d2h(ptr, nfloat):

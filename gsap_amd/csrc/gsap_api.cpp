@@ -87,6 +87,16 @@ struct gsap_sap {
     Buf seg_start_e, seg_n_e, seg_tile0_e, tile_seg_e;  // slab mode: the scene table of entries
     uint32_t ntiles_e = 0;                               // entry tiles of the uploaded table
     Buf feats, tile_sums, seg_mom, cand, mlp_wb, mlp_master, ax_tests_axes;
+    Buf plane_hist;  // per-scene centre histograms on the coordinate planes (features)
+    Buf normals;     // slab mode: per-scene sweep-plane normals (learned or forced)
+    // Labels of the learned selector in slab mode (gsap_sap_axis_tests_batched):
+    // every scene sweeps the plane normal to normals[s] (force_normal), and
+    // the update adds each scene's tests to lab_tests[s * lab_stride] and, when
+    // counting, its pairs to lab_pairs[s].
+    bool force_normal = false;
+    unsigned long long* lab_tests = nullptr;
+    uint32_t lab_stride = 0;
+    unsigned long long* lab_pairs = nullptr;
     Buf axis2, vr_partials, slabp, slab_cnt;  // slab mode
     Buf wstart;                               // window starts of the pair sort's local stage
     Buf coh;                                  // coherent object sort: splitters and buckets
@@ -219,7 +229,8 @@ extern "C" int32_t gsap_sap_destroy(gsap_sap* h) {
                    &h->block_sums, &h->learned_axis, &h->seg_start, &h->seg_n, &h->seg_tile0,
                    &h->tile_seg, &h->grp, &h->grp_end, &h->feats, &h->tile_sums, &h->seg_mom,
                    &h->cand, &h->mlp_wb, &h->mlp_master, &h->ax_tests_axes, &h->axis2,
-                   &h->vr_partials, &h->slabp, &h->slab_cnt, &h->wstart};
+                   &h->vr_partials, &h->slabp, &h->slab_cnt, &h->wstart, &h->plane_hist,
+                   &h->normals};
     for (Buf* b : bufs) release(*b);
     if (h->h_counts) (void)hipHostFree(h->h_counts);
     for (int i = 0; i < gsap_sap::kEv; ++i)
@@ -240,7 +251,7 @@ enum WsId : int {
     W_BLOCK_SUMS, W_PARTIALS, W_LB_OBJ, W_VR_PARTIALS, W_TILE_SEG, W_TILE_SUMS, W_AXIS,
     W_HIST_OBJ, W_SEG_START, W_SEG_N, W_SEG_TILE0, W_SEG_MOM, W_FEATS, W_HIST_PAIR, W_AXIS2,
     W_SLABP, W_SLAB_CNT, W_PTMP0, W_PTMP1, W_WSTART, W_LB_PAIR, W_COH, W_SEG_START_E, W_SEG_N_E,
-    W_SEG_TILE0_E, W_TILE_SEG_E, W_COUNT
+    W_SEG_TILE0_E, W_TILE_SEG_E, W_PLANE_HIST, W_NORMALS, W_COUNT
 };
 static_assert(W_COUNT <= 40, "gsap_sap::ws_size holds 40 entries");
 
@@ -280,6 +291,8 @@ static Buf* ws_buf(gsap_sap* h, int id) {
         case W_SEG_N_E: return &h->seg_n_e;
         case W_SEG_TILE0_E: return &h->seg_tile0_e;
         case W_TILE_SEG_E: return &h->tile_seg_e;
+        case W_PLANE_HIST: return &h->plane_hist;
+        case W_NORMALS: return &h->normals;
         default: return &h->lb_pair;
     }
 }
@@ -324,12 +337,14 @@ static void req_obj(const gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     r.b[W_SEG_TILE0] = (size_t)(nseg + 1) * 4;
     r.b[W_SEG_MOM] = (size_t)nseg * 9 * 8;
     r.b[W_FEATS] = (size_t)nseg * kFeat * 4;
+    r.b[W_PLANE_HIST] = (size_t)nseg * kDensWords * 4;
     // pair digit histograms, then the first pass's per-region counts and slot counters
     r.b[W_HIST_PAIR] = (kMaxPasses * 256 + kPairClaimWords + kPairDigitStartWords) * 4;
     if (slab_mode(h)) {  // per scene
         r.b[W_AXIS2] = (size_t)nseg * 4 * sizeof(float);
         r.b[W_SLABP] = (size_t)nseg * kSlabParams * sizeof(float);
         r.b[W_SLAB_CNT] = (size_t)nseg * 256 * sizeof(uint32_t);
+        r.b[W_NORMALS] = (size_t)nseg * 4 * sizeof(float);
         if (nseg > 1) {  // batched: the scene table of entries (2 per object)
             r.b[W_SEG_START_E] = r.b[W_SEG_N_E] = (size_t)nseg * 4;
             r.b[W_SEG_TILE0_E] = (size_t)(nseg + 1) * 4;
@@ -489,6 +504,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     a.grp_end = (uint32_t*)h->grp_end.p;
     a.learned_axis = nullptr;
     a.seg_mom = (double*)h->seg_mom.p;
+    a.plane_hist = (uint32_t*)h->plane_hist.p;
+    a.seg_tests = h->lab_tests;
+    a.seg_tests_stride = h->lab_stride;
     // slab mode: one scene only (the batched entry point refuses the flag)
     // slab mode: per scene slab axis, parameters and slab counts; the sort,
     // gather and sweep run over entries (2 per object), with the scene table
@@ -567,8 +585,16 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         mw.b1 = mb + kFeat * kHid;
         mw.b2 = mb + kFeat * kHid + kHid + kHid * kHid;
         mw.b3 = mb + kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut;
+        // slab mode: the network names the sweep plane's normal; its axis pair
+        // replaces the PCA pair of the confident scenes
+        float* const nrm = slabs ? (float*)h->normals.p : nullptr;
         HIPCHK(h, launch_axis_infer((const float*)h->feats.p, seg.nseg, mw, (const float*)h->cand.p,
-                                    h->conf_min, a.axis, st));
+                                    h->conf_min, a.axis, nrm, st));
+        if (slabs)
+            HIPCHK(h, launch_plane_pairs(a.seg_mom, seg.seg_n, n, seg.nseg, nrm, a.axis, a.axis2, st));
+    } else if (slabs && h->force_normal) {
+        HIPCHK(h, launch_plane_pairs(a.seg_mom, seg.seg_n, n, seg.tile_seg ? seg.nseg : 1u,
+                                     (const float*)h->normals.p, a.axis, a.axis2, st));
     }
     if (slabs) HIPCHK(h, launch_slab_params(a, st));
     if (keep) {
@@ -653,6 +679,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     uint32_t* const joint = msplit && npass_pair == 2 ? a.pair_claim + 2 * kMaxRegions * 256 : nullptr;
     w.joint = joint;
     w.hist_npass = (int)npass_pair;
+    w.seg_pairs = count_only ? h->lab_pairs : nullptr;
     w.hist_shift0 = (int)shift0;
     w.bbits = (int)bbits;
     HIPCHK(h, launch_sweep(w, st));
@@ -1269,6 +1296,7 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     a.hist_pair = (uint32_t*)h->hist_pair.p;
     a.npass_pair = 0;
     a.seg_mom = (double*)h->seg_mom.p;
+    a.plane_hist = (uint32_t*)h->plane_hist.p;
     if (nt == 0) return fail(h, GSAP_ERR_INVALID_ARG, "all scenes empty");
     for (uint32_t s = 0; s < ns; ++s)
         if (h_off[s + 1] == h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "empty scene");
@@ -1282,11 +1310,73 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     return GSAP_OK;
 }
 
+// Slab mode: the labels are the slab sweep's own counts, for scenes of any
+// size.  For each candidate normal every scene sweeps the plane normal to it
+// (one count-only batched update), the gather adds each scene's tests to its
+// column and the first update's sweep each scene's pairs.
+static int32_t slab_axis_tests(gsap_sap* h, hipStream_t st, const void* d_in, const uint32_t* h_off,
+                               uint32_t ns, const float* h_axes, uint32_t naxes, uint64_t* d_tests,
+                               uint64_t* d_pairs) {
+    if (h_off[0] != 0) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets must start at 0");
+    const uint32_t n = h_off[ns];
+    if (int32_t rc = check_common(h, d_in, n, nullptr)) return rc;
+    for (uint32_t k = 0; k < naxes; ++k) {
+        const float* w = h_axes + 3 * (size_t)k;
+        if (!(std::isfinite(w[0]) && std::isfinite(w[1]) && std::isfinite(w[2])) ||
+            !(w[0] * w[0] + w[1] * w[1] + w[2] * w[2] > 0.5f))
+            return fail(h, GSAP_ERR_INVALID_ARG, "axis tests: candidate normals must be unit vectors");
+    }
+    uint32_t nt = 0;
+    if (int32_t rc = prepare_scenes(h, st, h_off, ns, 0, false, &nt)) return rc;
+    SegTable seg{};
+    seg.seg_start = (const uint32_t*)h->seg_start.p;
+    seg.seg_n = (const uint32_t*)h->seg_n.p;
+    seg.seg_tile0 = (const uint32_t*)h->seg_tile0.p;
+    seg.tile_seg = (const uint32_t*)h->tile_seg.p;
+    seg.nseg = ns;
+    seg.ntiles = nt;
+    HIPCHK(h, launch_zero_u64((unsigned long long*)d_tests, ns * naxes, st));
+    if (d_pairs) HIPCHK(h, launch_zero_u64((unsigned long long*)d_pairs, ns, st));
+    std::vector<float> nr((size_t)ns * 4);
+    int32_t rc = GSAP_OK;
+    for (uint32_t k = 0; k < naxes && rc == GSAP_OK; ++k) {
+        for (uint32_t s = 0; s < ns; ++s) {
+            nr[4 * (size_t)s] = h_axes[3 * (size_t)k];
+            nr[4 * (size_t)s + 1] = h_axes[3 * (size_t)k + 1];
+            nr[4 * (size_t)s + 2] = h_axes[3 * (size_t)k + 2];
+            nr[4 * (size_t)s + 3] = 1.0f;
+        }
+        HIPCHK(h, hipStreamSynchronize(st));
+        HIPCHK(h, hipMemcpy(h->normals.p, nr.data(), nr.size() * sizeof(float), hipMemcpyHostToDevice));
+        h->force_normal = true;
+        h->lab_tests = (unsigned long long*)d_tests + k;
+        h->lab_stride = naxes;
+        h->lab_pairs = k == 0 ? (unsigned long long*)d_pairs : nullptr;
+        rc = enqueue_update(h, st, d_in, n, seg, nt, nullptr, nullptr, 0, nullptr, true,
+                            (unsigned long long*)h->counts.p, nullptr);
+        // the update's error flags, as after any count-only update
+        if (rc == GSAP_OK) rc = read_counts(h, st, 0, true, nullptr, nullptr, nullptr, n);
+    }
+    h->force_normal = false;
+    h->lab_tests = nullptr;
+    h->lab_stride = 0;
+    h->lab_pairs = nullptr;
+    return rc;
+}
+
 extern "C" int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const void* d_in,
                                                const uint32_t* h_off, uint32_t ns, const float* h_axes,
                                                uint32_t naxes, uint64_t* d_tests, uint64_t* d_pairs) {
     if (!h || !d_in || !h_off || ns == 0 || !h_axes || naxes == 0 || !d_tests)
         return fail(h, GSAP_ERR_INVALID_ARG, "gsap_sap_axis_tests_batched: bad argument");
+    if (slab_mode(h)) {
+        if (h->desc.input_kind != GSAP_INPUT_AABB || h->desc.output_kind != GSAP_OUT_CANON ||
+            (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP))
+            return fail(h, GSAP_ERR_UNSUPPORTED, "slab axis tests: AABB scenes, canonical output, no group filter");
+        for (uint32_t s = 0; s < ns; ++s)
+            if (h_off[s + 1] < h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
+        return slab_axis_tests(h, (hipStream_t)stream, d_in, h_off, ns, h_axes, naxes, d_tests, d_pairs);
+    }
     for (uint32_t s = 0; s < ns; ++s) {
         if (h_off[s + 1] < h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
         if (h_off[s + 1] - h_off[s] > 2048)

@@ -101,6 +101,13 @@ struct ObjPipeArgs {
     uint32_t* grp_end;      // [n] sweep end of each grouped record
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
     double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by k_axis)
+    // if set: the sweep tests of scene s are added to seg_tests[s * seg_tests_stride]
+    // (k_gather; the labels of the learned selector in slab mode)
+    unsigned long long* seg_tests;
+    uint32_t seg_tests_stride;
+    // per-scene histograms of the box centres on the three coordinate planes
+    // and axes, [nseg][kDensWords] (k_features; density features)
+    uint32_t* plane_hist;
     // slab mode (GSAP_FLAG_SLABS, one scene): objects i -> entries 2i, 2i+1
     int slabs;
     float* axis2;           // [4] slab axis v (largest spread orthogonal to the sweep axis)
@@ -171,8 +178,17 @@ struct SweepArgs {
     // if set (with hist, two digits): the second digit's counts per part
     // (part = first digit >> 5), [kMaxRegions][256], in place of its histogram
     uint32_t* joint;
+    // if set (count-only sweep): the pairs of scene s are added to seg_pairs[s]
+    unsigned long long* seg_pairs;
 };
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
+// Slab mode: scenes whose normals[4 s + 3] > 0 sweep the plane normal to
+// normals[4 s .. 4 s + 2]: a coordinate normal gives the two other coordinate
+// axes (the sweep axis the one of larger variance), any other normal the two
+// directions of largest spread within the plane.  Writes axis (source 3) and
+// axis2 of those scenes; the others keep theirs.
+hipError_t launch_plane_pairs(const double* seg_mom, const uint32_t* seg_n, uint32_t n, uint32_t nseg,
+                              const float* normals, float* axis, float* axis2, hipStream_t st);
 
 // Digit histograms of the pair keys for npass digits starting at bit shift0;
 // padding slots past region_valid[r] are not counted.
@@ -250,13 +266,22 @@ hipError_t launch_finish(Ctrl* ctrl, unsigned long long cap, unsigned long long*
 constexpr int kFeat = 64, kHid = 128, kOut = 4, kCand = 64;
 constexpr int kTgt = kCand + 1;  // training target row: 64 candidate regrets, log(1 + pairs)
 constexpr int kFeatSums = 39;  // per-tile partial sums feeding the features
+// density features: histograms of the box centres over mean +- 2 standard
+// deviations per axis -- a kPlaneGrid^2 grid on each coordinate plane, then
+// kAxisBins bins on each coordinate axis; kDensWords counters per scene
+constexpr int kPlaneGrid = 8, kPlaneBins = kPlaneGrid * kPlaneGrid;
+constexpr int kAxisBins = 32;
+constexpr int kDensWords = 3 * kPlaneBins + 3 * kAxisBins;
 struct MlpWeights {          // bf16 copies used by the kernels (row-major [in][out])
     const uint16_t *w1, *w2, *w3;
     const float *b1, *b2, *b3;
 };
 hipError_t launch_features(const ObjPipeArgs& a, float* tile_sums, float* feats, hipStream_t st);
+// normals == null: confident scenes get the chosen candidate as their sweep
+// axis in axis; else every scene's normals[4 s ..] is written (w = 1 when
+// confident, else 0) for launch_plane_pairs
 hipError_t launch_axis_infer(const float* feats, uint32_t nseg, MlpWeights w, const float* cand,
-                             float conf_min, float* axis, hipStream_t st);
+                             float conf_min, float* axis, float* normals, hipStream_t st);
 hipError_t launch_axis_tests(const void* in, int kind, const uint32_t* seg_start,
                              const uint32_t* seg_n, uint32_t nseg, const float* axes, uint32_t naxes,
                              unsigned long long* tests, unsigned long long* pairs, hipStream_t st);

@@ -10,8 +10,9 @@
 //                  the scene's pair count -- training labels
 //   k_mlp_fwd/bwd  a 64-128-128-4 MLP, bf16 weights and activations, fp32
 //                  accumulation, trained with Adam by these kernels
-//   k_axis_infer   inference per scene, snapped to the nearest candidate axis;
-//                  low-confidence scenes keep their PCA axis
+//   k_axis_infer   inference per scene, snapped to the nearest candidate axis
+//                  (slab mode: the nearest coordinate axis, the normal of the
+//                  sweep plane); low-confidence scenes keep their PCA axis (pair)
 // The axis never changes the pair set, only the sweep work.
 #include "gsap_device.h"
 
@@ -88,18 +89,29 @@ __device__ static SceneStats scene_stats(const double* m, uint32_t n) {
 
 __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sums, float*) {
     __shared__ float s_red[kWaves][kFeatSums];
+    __shared__ uint32_t s_pl[kDensWords];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x, l = lane_id(), w = tid >> 6;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const uint32_t n = ti.seg_end - ti.seg_start;
     const SceneStats S = scene_stats(a.seg_mom + 9 * (size_t)ti.seg, n);
+    for (uint32_t i = tid; i < (uint32_t)kDensWords; i += kWG) s_pl[i] = 0u;
     float acc[kFeatSums];
 #pragma unroll
     for (int k = 0; k < kFeatSums; ++k) acc[k] = 0.0f;
     const float ell = (float)S.ell;
+    // density grids: bins over mean +- 2 standard deviations per axis
+    float gs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double sd = sqrt(S.C[k][k] > 1e-30 ? S.C[k][k] : 1e-30);
+        gs[k] = (float)(1.0 / (4.0 * sd));
+    }
+    __syncthreads();
     for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
         float lo[3], hi[3], c4[4];
         load_lohi(a.in, a.kind, i, lo, hi, c4);
         float h[3], d[3];
+        int g[3], g1[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             h[k] = 0.5f * (hi[k] - lo[k]);
@@ -118,6 +130,19 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
         acc[36] += h[0] * h[1];
         acc[37] += h[0] * h[2];
         acc[38] += h[1] * h[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float z = d[k] * gs[k];  // in units of 4 standard deviations
+            const float q = floorf(z * (float)kPlaneGrid) + (float)(kPlaneGrid / 2);
+            g[k] = (int)fminf(fmaxf(q, 0.0f), (float)(kPlaneGrid - 1));
+            const float q1 = floorf(z * (float)kAxisBins) + (float)(kAxisBins / 2);
+            g1[k] = (int)fminf(fmaxf(q1, 0.0f), (float)(kAxisBins - 1));
+        }
+        atomicAdd(&s_pl[g[0] * kPlaneGrid + g[1]], 1u);
+        atomicAdd(&s_pl[kPlaneBins + g[0] * kPlaneGrid + g[2]], 1u);
+        atomicAdd(&s_pl[2 * kPlaneBins + g[1] * kPlaneGrid + g[2]], 1u);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) atomicAdd(&s_pl[3 * kPlaneBins + k * kAxisBins + g1[k]], 1u);
     }
 #pragma unroll
     for (int k = 0; k < kFeatSums; ++k) {
@@ -130,6 +155,8 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
         for (int ww = 0; ww < kWaves; ++ww) v += s_red[ww][tid];
         tile_sums[(size_t)tile * kFeatSums + tid] = v;
     }
+    for (uint32_t i = tid; i < (uint32_t)kDensWords; i += kWG)
+        if (s_pl[i]) atomicAdd(&a.plane_hist[(size_t)ti.seg * kDensWords + i], s_pl[i]);
 }
 
 // One wave per scene: the tile sums in tile order (fp64), then the features.
@@ -177,12 +204,47 @@ __global__ __launch_bounds__(64) void k_features_fin(ObjPipeArgs a, const float*
         f[22 + k] = (float)(s_fin[3 + k] / nn / (e * e));
         f[25 + k] = (float)(s_fin[6 + k] / nn / sqrt(s2));
         const double ckk = S.C[k][k] > 1e-30 ? S.C[k][k] : 1e-30;
-        const double kurt = s_fin[9 + k] / nn / (ckk * ckk);
-        f[28 + k] = (float)(kurt < 16.0 ? kurt / 16.0 : 1.0);
         for (int b = 0; b < 8; ++b) f[31 + 8 * k + b] = (float)(s_fin[12 + 8 * k + b] / nn);
         const double ratio = 2.0 * hk / sqrt(ckk);
         f[58 + k] = (float)(ratio < 4.0 ? ratio / 4.0 : 1.0);
-        f[61 + k] = (float)(log2(ratio * nn + 1.0) / 24.0);
+    }
+    // Density cost estimates.  A sweep tests each box against the boxes
+    // whose projected centres fall near its own: about n^2 (extent) x (the
+    // collision rate of the projected centres, sum of squared bin fractions
+    // / bin size).  Axis k (plain sweep): mean extent on k x that rate on k;
+    // plane normal to k (slab sweep): the product of the two mean extents x
+    // that rate on the plane.  The features are each cost's log2 relative to
+    // the mean of the three (only the ranking and ratios matter), halved and
+    // clamped to [-1, 1].
+    {
+        const uint32_t* dh = a.plane_hist + (size_t)seg * kDensWords;
+        double sd[3], hm[3], lc1[3], lc2[3];
+        for (int k = 0; k < 3; ++k) {
+            sd[k] = sqrt(S.C[k][k] > 1e-30 ? S.C[k][k] : 1e-30);
+            hm[k] = s_fin[k] / nn + 1e-30;
+        }
+        auto rate = [&](const uint32_t* hb, int nb) {
+            double c2 = 0.0;
+            for (int b = 0; b < nb; ++b) {
+                const double p = (double)hb[b] / nn;
+                c2 += p * p;
+            }
+            return c2 + 1e-30;
+        };
+        const int pa[3][2] = {{1, 2}, {0, 2}, {0, 1}};  // the plane normal to axis k
+        const int pbase[3] = {2, 1, 0};                 // its histogram (xy, xz, yz order)
+        for (int k = 0; k < 3; ++k) {
+            const double bin1 = 4.0 * sd[k] / kAxisBins;
+            lc1[k] = log2(hm[k] * rate(dh + 3 * kPlaneBins + k * kAxisBins, kAxisBins) / bin1);
+            const int a0 = pa[k][0], a1 = pa[k][1];
+            const double area = (4.0 * sd[a0] / kPlaneGrid) * (4.0 * sd[a1] / kPlaneGrid);
+            lc2[k] = log2(hm[a0] * hm[a1] * rate(dh + pbase[k] * kPlaneBins, kPlaneBins) / area);
+        }
+        const double m1 = (lc1[0] + lc1[1] + lc1[2]) / 3.0, m2 = (lc2[0] + lc2[1] + lc2[2]) / 3.0;
+        for (int k = 0; k < 3; ++k) {
+            f[28 + k] = (float)fmin(fmax(0.5 * (lc1[k] - m1), -1.0), 1.0);
+            f[61 + k] = (float)fmin(fmax(0.5 * (lc2[k] - m2), -1.0), 1.0);
+        }
     }
     f[55] = (float)(s_fin[36] / nn / (e * e));
     f[56] = (float)(s_fin[37] / nn / (e * e));
@@ -356,7 +418,8 @@ __global__ void k_adam(float* master, const float* g, float* m, float* v, uint32
 // Inference per scene; lane c scores candidate c.  A confident prediction
 // replaces the scene's PCA axis by the nearest candidate.
 __global__ __launch_bounds__(kWG) void k_axis_infer(const float* feats, uint32_t nseg, MlpWeights w,
-                                                    const float* cand, float conf_min, float* axis) {
+                                                    const float* cand, float conf_min, float* axis,
+                                                    float* normals) {
     __shared__ float s_x[kWaves][kFeat];
     __shared__ float s_h[kWaves][kHid];
     const uint32_t wv = threadIdx.x >> 6, seg = blockIdx.x * kWaves + wv, lane = lane_id();
@@ -366,6 +429,9 @@ __global__ __launch_bounds__(kWG) void k_axis_infer(const float* feats, uint32_t
     const float nu = sqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
     const float c0 = cand[3 * lane], c1 = cand[3 * lane + 1], c2 = cand[3 * lane + 2];
     float sc = nu > 0.0f ? fabsf(y[0] * c0 + y[1] * c1 + y[2] * c2) / nu : 0.0f;
+    // slab mode: the sweep plane's normal is one of the coordinate axes (the
+    // first three candidates); a tilted plane lengthens every box's projection
+    if (normals && lane >= 3) sc = -1.0f;
     // wave argmax, lowest index on ties
     float best = sc;
     uint32_t bi = lane;
@@ -374,7 +440,17 @@ __global__ __launch_bounds__(kWG) void k_axis_infer(const float* feats, uint32_t
         const uint32_t oi = __shfl_xor(bi, o, 64);
         if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
     }
-    if (lane == 0 && nu > 0.1f && best >= conf_min) {
+    // plain mode: the prediction must lie within acos(conf_min) of its
+    // candidate; slab mode names one of three planes (the nearest coordinate
+    // axis is always within 55 degrees), so only a degenerate output falls back
+    const bool confident = nu > 0.1f && (normals != nullptr || best >= conf_min);
+    if (lane == 0 && normals) {  // slab mode: the candidate is the sweep plane's normal
+        float* nr = normals + 4 * (size_t)seg;
+        nr[0] = cand[3 * bi];
+        nr[1] = cand[3 * bi + 1];
+        nr[2] = cand[3 * bi + 2];
+        nr[3] = confident ? 1.0f : 0.0f;
+    } else if (lane == 0 && confident) {
         float* ax = axis + 4 * (size_t)seg;
         ax[0] = cand[3 * bi];
         ax[1] = cand[3 * bi + 1];
@@ -481,16 +557,19 @@ __global__ __launch_bounds__(kWG) void k_axis_tests(const void* in, int kind, co
 
 // ------------------------------------------------------------------ launchers
 hipError_t launch_features(const ObjPipeArgs& a, float* tile_sums, float* feats, hipStream_t st) {
-    hipLaunchKernelGGL(k_features, dim3(a.ntiles), dim3(kWG), 0, st, a, tile_sums, feats);
     const uint32_t nseg = a.seg.tile_seg ? a.seg.nseg : 1u;
+    static_assert(kDensWords % 2 == 0, "density histograms cleared as 64-bit words");
+    hipError_t e = launch_zero_u64((unsigned long long*)a.plane_hist, nseg * (kDensWords / 2), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_features, dim3(a.ntiles), dim3(kWG), 0, st, a, tile_sums, feats);
     hipLaunchKernelGGL(k_features_fin, dim3(nseg), dim3(64), 0, st, a, tile_sums, feats);
     return hipGetLastError();
 }
 hipError_t launch_axis_infer(const float* feats, uint32_t nseg, MlpWeights w, const float* cand,
-                             float conf_min, float* axis, hipStream_t st) {
+                             float conf_min, float* axis, float* normals, hipStream_t st) {
     if (nseg == 0) return hipSuccess;
     hipLaunchKernelGGL(k_axis_infer, dim3((nseg + kWaves - 1) / kWaves), dim3(kWG), 0, st, feats,
-                       nseg, w, cand, conf_min, axis);
+                       nseg, w, cand, conf_min, axis, normals);
     return hipGetLastError();
 }
 hipError_t launch_axis_tests(const void* in, int kind, const uint32_t* seg_start, const uint32_t* seg_n,

@@ -367,6 +367,90 @@ __global__ __launch_bounds__(kWG) void k_axis(ObjPipeArgs a) {
     }
 }
 
+// The axis pair of a sweep plane with normal w: a coordinate normal gives the
+// two other coordinate axes, the sweep axis the one of larger variance (ties:
+// the lower index); any other normal the direction of largest spread within
+// the plane (top eigenvector of P C P, P = I - w w^T) and w x that.
+__device__ static void plane_pair(const double m[9], uint32_t n, const float wf[3], float u[3],
+                                  float v[3]) {
+    const double inv = n ? 1.0 / (double)n : 0.0;
+    const double mu[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+    double C[3][3];
+    C[0][0] = m[3] * inv - mu[0] * mu[0];
+    C[0][1] = C[1][0] = m[4] * inv - mu[0] * mu[1];
+    C[0][2] = C[2][0] = m[5] * inv - mu[0] * mu[2];
+    C[1][1] = m[6] * inv - mu[1] * mu[1];
+    C[1][2] = C[2][1] = m[7] * inv - mu[1] * mu[2];
+    C[2][2] = m[8] * inv - mu[2] * mu[2];
+    int k = 0;
+    for (int j = 1; j < 3; ++j)
+        if (fabsf(wf[j]) > fabsf(wf[k])) k = j;
+    for (int j = 0; j < 3; ++j) u[j] = v[j] = 0.0f;
+    if (fabsf(wf[k]) >= 1.0f - 1e-6f) {  // coordinate normal
+        const int a0 = k == 0 ? 1 : 0, a1 = k == 2 ? 1 : 2;
+        const bool first = C[a0][a0] >= C[a1][a1];
+        u[first ? a0 : a1] = 1.0f;
+        v[first ? a1 : a0] = 1.0f;
+        return;
+    }
+    double w[3] = {wf[0], wf[1], wf[2]};
+    const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int j = 0; j < 3; ++j) w[j] /= wn;
+    double P[3][3], PC[3][3], A[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = (i == j ? 1.0 : 0.0) - w[i] * w[j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) PC[i][j] = P[i][0] * C[0][j] + P[i][1] * C[1][j] + P[i][2] * C[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = PC[i][0] * P[0][j] + PC[i][1] * P[1][j] + PC[i][2] * P[2][j];
+    double V[3][3], ev[3];
+    jacobi3(A, V, ev);
+    // the eigenvector along w has eigenvalue 0; take the largest of the others
+    int best = -1;
+    for (int j = 0; j < 3; ++j) {
+        const double d = fabs(V[0][j] * w[0] + V[1][j] * w[1] + V[2][j] * w[2]);
+        if (d > 0.5) continue;
+        if (best < 0 || ev[j] > ev[best]) best = j;
+    }
+    double x[3];
+    if (best >= 0) {
+        for (int j = 0; j < 3; ++j) x[j] = V[j][best];
+    } else {  // no eigenvector clear of w (cannot happen for a symmetric A): any orthogonal one
+        const int e = fabs(w[0]) < 0.9 ? 0 : 1;
+        double t[3] = {0.0, 0.0, 0.0};
+        t[e] = 1.0;
+        x[0] = w[1] * t[2] - w[2] * t[1];
+        x[1] = w[2] * t[0] - w[0] * t[2];
+        x[2] = w[0] * t[1] - w[1] * t[0];
+    }
+    const double d = x[0] * w[0] + x[1] * w[1] + x[2] * w[2];
+    for (int j = 0; j < 3; ++j) x[j] -= d * w[j];
+    const double xn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (int j = 0; j < 3; ++j) x[j] /= xn;
+    double y[3] = {w[1] * x[2] - w[2] * x[1], w[2] * x[0] - w[0] * x[2], w[0] * x[1] - w[1] * x[0]};
+    const double yn = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    for (int j = 0; j < 3; ++j) {
+        u[j] = (float)x[j];
+        v[j] = (float)(y[j] / yn);
+    }
+}
+
+__global__ void k_plane_pairs(const double* seg_mom, const uint32_t* seg_n, uint32_t n, uint32_t nseg,
+                              const float* normals, float* axis, float* axis2) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const float* w = normals + 4 * (size_t)s;
+    if (!(w[3] > 0.0f)) return;
+    double m[9];
+    for (int k = 0; k < 9; ++k) m[k] = seg_mom[9 * (size_t)s + k];
+    float u[3], v[3];
+    plane_pair(m, seg_n ? seg_n[s] : n, w, u, v);
+    float* ax = axis + 4 * (size_t)s;
+    float* a2 = axis2 + 4 * (size_t)s;
+    ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = 3.0f;
+    a2[0] = v[0]; a2[1] = v[1]; a2[2] = v[2]; a2[3] = 0.0f;
+}
+
 // ---------------------------------------------------------------- slabs
 __device__ __forceinline__ float wave_minf(float v) {
 #pragma unroll
@@ -671,7 +755,10 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
     if (tid == 0) {
         unsigned long long t = 0;
         for (int v = 0; v < kWaves; ++v) t += s_t[v];
-        if (t) atomicAdd(&a.ctrl->tests, t);
+        if (t) {
+            atomicAdd(&a.ctrl->tests, t);
+            if (a.seg_tests) atomicAdd(&a.seg_tests[(size_t)ti.seg * a.seg_tests_stride], t);
+        }
     }
 }
 
@@ -1118,7 +1205,10 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
     }
     if (lane == 0) {
         // pairs of a writing sweep are the sum of its slot claims (k_pair_regions)
-        if (MODE == 2 && np) atomicAdd(&a.ctrl->part[0][blockIdx.x % kMaxRegions], np);
+        if (MODE == 2 && np) {
+            atomicAdd(&a.ctrl->part[0][blockIdx.x % kMaxRegions], np);
+            if (a.seg_pairs) atomicAdd(&a.seg_pairs[ti.seg], np);
+        }
         if (ext) atomicAdd(&a.ctrl->part[1][blockIdx.x % kMaxRegions], ext);
     }
     if constexpr (MODE != 2) {
@@ -1294,6 +1384,13 @@ hipError_t launch_moments(const ObjPipeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_moments, dim3(a.ntiles), dim3(kWG), 0, st, a);
     const uint32_t nseg = a.seg.tile_seg ? a.seg.nseg : 1u;
     if (!a.keep_key) hipLaunchKernelGGL(k_axis, dim3(nseg), dim3(kWG), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_plane_pairs(const double* seg_mom, const uint32_t* seg_n, uint32_t n, uint32_t nseg,
+                              const float* normals, float* axis, float* axis2, hipStream_t st) {
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plane_pairs, dim3((nseg + 255) / 256), dim3(256), 0, st, seg_mom, seg_n, n,
+                       nseg, normals, axis, axis2);
     return hipGetLastError();
 }
 hipError_t launch_project(const ObjPipeArgs& a, hipStream_t st) {

@@ -267,8 +267,13 @@ int32_t gsap_mlp_train(gsap_mlp* m, void* stream, const float* d_x, const float*
                        float* h_epoch_loss);
 /* The 64 fixed candidate axes (64 x 3 floats): coordinate axes first. */
 int32_t gsap_axis_candidates(float* h_axes);
-/* Attach a trained model to a handle (weights are copied); conf_min in (0,1]
- * is the |cos| between prediction and snapped candidate required to use it. */
+/* Attach a trained model to a handle (weights are copied).  Plain sweep: the
+ * model names the sweep axis, and conf_min in (0,1] is the |cos| between
+ * prediction and snapped candidate required to use it (else PCA).  Slab sweep
+ * (GSAP_FLAG_SLABS): the model names the normal of the sweep plane, snapped to
+ * the nearest coordinate axis; the sweep and slab axes are the two other
+ * coordinate axes (sweep along the one of larger variance); conf_min is not
+ * used, only a degenerate prediction keeps the PCA pair. */
 int32_t gsap_sap_set_learned_model(gsap_sap* h, const gsap_mlp* m, float conf_min);
 /* Per-scene features (n_scenes x 64 fp32, device) of packed AABB scenes. */
 int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const void* d_in,
@@ -276,7 +281,12 @@ int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const void* d_in,
                                   float* d_features);
 /* The judge: exact sweep-test count of every scene on every axis
  * (d_tests: n_scenes x n_axes) and each scene's pair count (d_pairs, may be
- * NULL).  Scenes of at most 2048 objects. */
+ * NULL).  Plain sweep: in-LDS sort per (scene, axis), scenes of at most 2048
+ * objects.  Slab handle (GSAP_FLAG_SLABS; AABB, canonical output): the axes
+ * are sweep-plane normals and the counts are the slab sweep's own, from one
+ * count-only batched update per axis, scenes of any size.  A coordinate
+ * normal sweeps the two other coordinate axes (the larger variance first); any
+ * other normal the two directions of largest spread within its plane. */
 int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const void* d_in,
                                     const uint32_t* h_scene_offsets, uint32_t n_scenes,
                                     const float* h_axes, uint32_t n_axes, uint64_t* d_tests,

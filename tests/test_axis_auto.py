@@ -30,7 +30,13 @@ def plain_tests(b, u):
 
 def slab_tests(b, u, v):
     """Window sizes of a sweep along u inside slabs of a second axis v of width
-    max v-extent (each box in the one or two slabs it touches)."""
+    max v-extent (each box in the one or two slabs it touches); a window
+    counts its own box."""
+    return slab_windows(b, u, v)[0]
+
+
+def slab_windows(b, u, v):
+    """(slab_tests(b, u, v), the number of slab entries)."""
     l, h = _proj(b, np.asarray(u, np.float64))
     vl, vh = _proj(b, np.asarray(v, np.float64))
     w = (vh - vl).max() * (1 + 2 ** -10)
@@ -38,12 +44,13 @@ def slab_tests(b, u, v):
     w = max(w, span / 254 * (1 + 2 ** -10))
     s0 = np.floor((vl - v0) / w).astype(np.int64)
     s1 = np.floor((vh - v0) / w).astype(np.int64)
-    t = 0
+    t = e = 0
     for s in range(int(s1.max()) + 1):
         m = (s0 <= s) & (s1 >= s)
         ss = np.sort(l[m])
         t += int((np.searchsorted(ss, h[m], "right") - np.searchsorted(ss, l[m], "left")).sum())
-    return t
+        e += int(m.sum())
+    return t, e
 
 
 def scenes():
