@@ -28,8 +28,8 @@ struct SegTable {
 };
 
 // Sorted-object record (32 bytes).  Two copies are written after the sort:
-// the candidate stream in sorted order (aux unused) and, per 256-object
-// sub-tile, the objects re-ordered by sweep end (aux = sorted position), read
+// the candidate stream in sorted order (aux unused) and, per 64 positions
+// (one sweep wave), the objects re-ordered by sweep end (aux = sorted position), read
 // by the sweep through the scalar cache.
 struct alignas(16) RecAabb {
     float lo[3];
@@ -97,7 +97,7 @@ struct ObjPipeArgs {
     uint32_t* keys2;
     uint32_t* vals2;
     void* rec;              // [n] sorted records (candidate stream)
-    void* grp;              // [n] records grouped by sweep end per 256-object sub-tile
+    void* grp;              // [n] records grouped by sweep end per 64 positions
     uint32_t* grp_end;      // [n] sweep end of each grouped record
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
     double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by k_axis)
@@ -151,7 +151,7 @@ hipError_t launch_gather(const ObjPipeArgs& a, const uint32_t* sorted_keys,
 
 struct SweepArgs {
     const void* rec;         // candidates, sorted order
-    const void* grp;         // objects, end-grouped per 256-object sub-tile
+    const void* grp;         // objects, end-grouped per 64 positions (one wave)
     const uint32_t* grp_end;
     uint32_t n;
     SegTable seg;

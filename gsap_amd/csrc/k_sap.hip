@@ -643,13 +643,12 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
 // ---------------------------------------------------------------- gather
 // Sorted position s -> candidate record rec[s]; sweep end (first position
 // whose key exceeds s's upper key) by binary search; the sweep-test count.
-// Each 256-object sub-tile is also written re-ordered by sweep end into
+// Each wave's 64 positions are also written re-ordered by sweep end into
 // grp/grp_end (aux = s), the object side of the sweep.
 template <int KIND>
 __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* __restrict__ K,
                                                 const uint32_t* __restrict__ V) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
-    __shared__ unsigned long long s_key[kWG];  // (sweep end, lane): the grouping keys
     __shared__ uint32_t s_sb[257];  // slab mode: first sorted position of each slab (+ end)
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
@@ -725,27 +724,31 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
             }
             reinterpret_cast<Rec*>(a.rec)[s] = r;
         }
-        // place in the sub-tile ordered by (sweep end, position): the number
-        // of smaller keys (keys are unique)
-        // (only in sub-tiles with a window longer than kShortWindow: the
-        // sweep takes every other sub-tile one object per lane, in any order,
-        // so they keep sorted order and skip the ranking)
-        const unsigned long long mk = ((unsigned long long)end << 8) | tid;
-        s_key[tid] = mk;
+        // place among the wave's 64 positions ordered by (sweep end,
+        // position): the number of smaller keys (keys are unique), one
+        // scalar-broadcast compare per lane of the wave.  Only in waves with
+        // a window longer than kShortWindow: the sweep takes the other waves
+        // one object per lane, in any order, so they keep sorted order.
+        const uint32_t lane = lane_id();
+        const unsigned long long mk = ((unsigned long long)end << 6) | lane;
         const uint32_t wlen = (valid && end > s + 1u) ? end - s - 1u : 0u;
-        const int any_long = __syncthreads_or(wlen > kShortWindow ? 1 : 0);
-        uint32_t rank = tid;
-        if (any_long) {
+        uint32_t rank = lane;
+        if (ballot(wlen > kShortWindow)) {
             rank = 0;
+            const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
 #pragma unroll 8
-            for (uint32_t j = 0; j < kWG; ++j) rank += s_key[j] < mk ? 1u : 0u;
+            for (int j = 0; j < 64; ++j) {
+                const uint32_t jlo = (uint32_t)__builtin_amdgcn_readlane((int)mlo, j);
+                const uint32_t jhi = (uint32_t)__builtin_amdgcn_readlane((int)mhi, j);
+                rank += (((unsigned long long)jhi << 32) | jlo) < mk ? 1u : 0u;
+            }
         }
         if (valid) {
             r.aux = s;
-            reinterpret_cast<Rec*>(a.grp)[sub + rank] = r;
-            a.grp_end[sub + rank] = end;
+            const uint32_t g = s - lane + rank;  // the wave's first position + rank
+            reinterpret_cast<Rec*>(a.grp)[g] = r;
+            a.grp_end[g] = end;
         }
-        __syncthreads();
     }
     // one atomic per workgroup (the counter is shared by the whole grid)
     __shared__ unsigned long long s_t[kWaves];
@@ -763,15 +766,17 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
 }
 
 // ---------------------------------------------------------------- sweep
-// One wave per 64 objects of a 256-object sub-tile, taken in sweep-end order.
-// Lanes are candidates: the wave walks its objects' candidate range 64 sorted
-// positions at a time (coalesced vector loads of the candidate stream), and
-// for each still-active object (scalar loads; the wave's 64 objects, 2 KB,
-// stay in the scalar cache) every lane runs the exact predicate against its
-// candidate.  Objects whose sweep end precedes the chunk are retired, so the
-// work is the sweep-test count plus under one chunk per object.  Candidates
-// past an object's sweep end can never overlap it (the projected intervals
-// are conservative); only t > s needs a mask, and only near the sub-tile.
+// One wave per 64 consecutive sorted objects, taken in sweep-end order (the
+// gather writes that copy).  Lanes are candidates: the wave walks its
+// objects' candidate range from its first position 64 sorted positions at a
+// time (coalesced vector loads of the candidate stream), and for each
+// still-active object (scalar loads; the wave's 64 objects, 2 KB, stay in the
+// scalar cache) every lane runs the exact predicate against its candidate.
+// Objects whose sweep end precedes the chunk are retired, so the work is the
+// sweep-test count plus under one chunk pair and the object's offset in the
+// wave per object.  Candidates past an object's sweep end can never overlap
+// it (the projected intervals are conservative); only t > s needs a mask,
+// and only in the wave's first chunks.
 // Hits are staged per wave in LDS (ballot + mbcnt compaction) and flushed to
 // the pair buffer with one atomic claim of exactly the staged count when the
 // stage fills; the last wave of the workgroup to finish flushes what every
@@ -982,12 +987,22 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         if (maxlen <= kShortWindow) {
             const uint32_t myorig = SLAB ? (me.orig & ~kFirstBit) : me.orig;
             const uint32_t mygrp = (GROUP && lane < nobj) ? a.group[myorig] : 0u;
-            for (uint32_t k = 1u; k <= maxlen; ++k) {
+            // candidates four at a time: their loads are issued together, so
+            // the wave waits on one load latency per four candidates
+            for (uint32_t k0 = 1u; k0 <= maxlen; k0 += 4u) {
+            Rec cb[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j)
+                if (k0 + j <= len) cb[j] = R[my_s + k0 + j];
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j) {
+                const uint32_t k = k0 + j;
+                if (k > maxlen) break;  // uniform
                 const bool act = k <= len;
                 bool hit = false;
                 uint32_t corig = 0u;
                 if (act) {
-                    const Rec c = R[my_s + k];
+                    const Rec& c = cb[j];
                     if constexpr (KIND == 0) {
                         const float d0 = c.lo[0] - me.hi[0], d1 = me.lo[0] - c.hi[0];
                         const float d2 = c.lo[1] - me.hi[1], d3 = me.lo[1] - c.hi[1];
@@ -1019,6 +1034,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                     fill += cnt;
                     __builtin_amdgcn_wave_barrier();
                 }
+            }
             }
             ex = wave_sum((unsigned long long)(lane < nobj ? maxlen : 0u));
         } else {
@@ -1071,11 +1087,12 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
             return k;
         };
         Rec cn{}, cn2{};
-        if (lo + 1u < wend) {
-            cn = R[min(lo + 1u + lane, wend - 1u)];
-            cn2 = R[min(lo + 65u + lane, wend - 1u)];
+        // the wave's candidates start after its first position
+        if (g0 + 1u < wend) {
+            cn = R[min(g0 + 1u + lane, wend - 1u)];
+            cn2 = R[min(g0 + 65u + lane, wend - 1u)];
         }
-        for (uint32_t t0 = lo + 1; t0 < wend; t0 += 128u) {
+        for (uint32_t t0 = g0 + 1; t0 < wend; t0 += 128u) {
             // retire objects whose sweep end is at or before this chunk pair
             kfirst = (uint32_t)__popcll(ballot(lane < nobj && ge <= t0));
             const uint32_t t = t0 + lane;
@@ -1165,18 +1182,19 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
                 }
             };
             // an object may sit at or after the chunk start only in the first
-            // chunks of the sub-tile
+            // chunks of the wave (its objects lie in [g0, g0 + nobj))
+            const bool early = t0 < g0 + nobj;  // uniform
             if (SLAB && mixed) {
-                if (t0 < hi) objects(std::true_type{}, std::true_type{});
+                if (early) objects(std::true_type{}, std::true_type{});
                 else objects(std::false_type{}, std::true_type{});
             } else {
-                if (t0 < hi) objects(std::true_type{}, std::false_type{});
+                if (early) objects(std::true_type{}, std::false_type{});
                 else objects(std::false_type{}, std::false_type{});
             }
         }
         // candidate tests executed: every active object takes whole pairs of
         // 64-lane chunks (an upper bound: the wave's last pair may be single)
-        if (lane < nobj && ge > lo + 1) ex = (unsigned long long)((ge - lo - 1 + 127u) / 128u) * 128u;
+        if (lane < nobj && ge > g0 + 1) ex = (unsigned long long)((ge - g0 - 1 + 127u) / 128u) * 128u;
         ex = wave_sum(ex);
         }  // long windows
     }
