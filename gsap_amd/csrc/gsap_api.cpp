@@ -77,6 +77,7 @@ struct gsap_sap {
     int slabs = 0;        // slab sweep for single-scene updates
     int sweep_chunk = 0;  // sweep sub-tiles per XCD chunk
     int pair_regions = 1; // sweep output regions (1 or kMaxRegions)
+    bool tune_set = false;  // a configuration was set or measured (gsap_sap_set_tune/autotune)
     int pair_local = 0;   // pair sort: onesweep over the high key bits + per-bucket LDS sort
     float tune_ms = 0.0f;
     int tune_cached = 0;
@@ -338,13 +339,14 @@ static void req_obj(const gsap_sap* h, uint32_t n, uint32_t nseg, uint32_t ntile
     r.b[W_SEG_MOM] = (size_t)nseg * 9 * 8;
     r.b[W_FEATS] = (size_t)nseg * kFeat * 4;
     r.b[W_PLANE_HIST] = (size_t)nseg * kDensWords * 4;
+    r.b[W_NORMALS] = (size_t)nseg * 4 * sizeof(float);
     // pair digit histograms, then the first pass's per-region counts and slot counters
     r.b[W_HIST_PAIR] = (kMaxPasses * 256 + kPairClaimWords + kPairDigitStartWords) * 4;
     if (slab_mode(h)) {  // per scene
         r.b[W_AXIS2] = (size_t)nseg * 4 * sizeof(float);
         r.b[W_SLABP] = (size_t)nseg * kSlabParams * sizeof(float);
         r.b[W_SLAB_CNT] = (size_t)nseg * 256 * sizeof(uint32_t);
-        r.b[W_NORMALS] = (size_t)nseg * 4 * sizeof(float);
+
         if (nseg > 1) {  // batched: the scene table of entries (2 per object)
             r.b[W_SEG_START_E] = r.b[W_SEG_N_E] = (size_t)nseg * 4;
             r.b[W_SEG_TILE0_E] = (size_t)(nseg + 1) * 4;
@@ -592,9 +594,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
                                     h->conf_min, a.axis, nrm, st));
         if (slabs)
             HIPCHK(h, launch_plane_pairs(a.seg_mom, seg.seg_n, n, seg.nseg, nrm, a.axis, a.axis2, st));
-    } else if (slabs && h->force_normal) {
+    } else if (h->force_normal) {
         HIPCHK(h, launch_plane_pairs(a.seg_mom, seg.seg_n, n, seg.tile_seg ? seg.nseg : 1u,
-                                     (const float*)h->normals.p, a.axis, a.axis2, st));
+                                     (const float*)h->normals.p, a.axis, slabs ? a.axis2 : nullptr, st));
     }
     if (slabs) HIPCHK(h, launch_slab_params(a, st));
     if (keep) {
@@ -1057,6 +1059,11 @@ extern "C" int32_t gsap_sap_collide_batched(gsap_sap* h, void* stream, const voi
     const bool count_only = (pair_capacity == 0 || d_pairs == nullptr);
     if (!count_only && !d_scene_pair_offsets)
         return fail(h, GSAP_ERR_INVALID_ARG, "d_scene_pair_offsets is NULL");
+    // An untuned handle's batched updates claim pair slots on one counter per
+    // XCD: the sweep makes at least one claim per workgroup, and with
+    // thousands of small scenes one shared counter serialises them (2,048
+    // scenes: slab sweep 3.7 -> 2.6 ms).
+    if (!h->tune_set && !count_only) h->pair_regions = kMaxRegions;
     hipStream_t st = (hipStream_t)stream;
     if (n == 0) {
         if (h_num_pairs) *h_num_pairs = 0;
@@ -1310,11 +1317,12 @@ extern "C" int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const vo
     return GSAP_OK;
 }
 
-// Slab mode: the labels are the slab sweep's own counts, for scenes of any
-// size.  For each candidate normal every scene sweeps the plane normal to it
-// (one count-only batched update), the gather adds each scene's tests to its
-// column and the first update's sweep each scene's pairs.
-static int32_t slab_axis_tests(gsap_sap* h, hipStream_t st, const void* d_in, const uint32_t* h_off,
+// Labels from the sweep's own counts, for scenes of any size (slab handles,
+// and plain handles with a scene beyond the in-LDS judge).  For each
+// candidate every scene sweeps along it (plain) or the plane normal to it
+// (slabs) in one count-only batched update; the gather adds each scene's
+// tests to its column and the first update's sweep each scene's pairs.
+static int32_t pipeline_axis_tests(gsap_sap* h, hipStream_t st, const void* d_in, const uint32_t* h_off,
                                uint32_t ns, const float* h_axes, uint32_t naxes, uint64_t* d_tests,
                                uint64_t* d_pairs) {
     if (h_off[0] != 0) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets must start at 0");
@@ -1324,7 +1332,7 @@ static int32_t slab_axis_tests(gsap_sap* h, hipStream_t st, const void* d_in, co
         const float* w = h_axes + 3 * (size_t)k;
         if (!(std::isfinite(w[0]) && std::isfinite(w[1]) && std::isfinite(w[2])) ||
             !(w[0] * w[0] + w[1] * w[1] + w[2] * w[2] > 0.5f))
-            return fail(h, GSAP_ERR_INVALID_ARG, "axis tests: candidate normals must be unit vectors");
+            return fail(h, GSAP_ERR_INVALID_ARG, "axis tests: candidates must be unit vectors");
     }
     uint32_t nt = 0;
     if (int32_t rc = prepare_scenes(h, st, h_off, ns, 0, false, &nt)) return rc;
@@ -1369,18 +1377,17 @@ extern "C" int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const 
                                                uint32_t naxes, uint64_t* d_tests, uint64_t* d_pairs) {
     if (!h || !d_in || !h_off || ns == 0 || !h_axes || naxes == 0 || !d_tests)
         return fail(h, GSAP_ERR_INVALID_ARG, "gsap_sap_axis_tests_batched: bad argument");
-    if (slab_mode(h)) {
-        if (h->desc.input_kind != GSAP_INPUT_AABB || h->desc.output_kind != GSAP_OUT_CANON ||
-            (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP))
-            return fail(h, GSAP_ERR_UNSUPPORTED, "slab axis tests: AABB scenes, canonical output, no group filter");
-        for (uint32_t s = 0; s < ns; ++s)
-            if (h_off[s + 1] < h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
-        return slab_axis_tests(h, (hipStream_t)stream, d_in, h_off, ns, h_axes, naxes, d_tests, d_pairs);
-    }
+    bool big = false;
     for (uint32_t s = 0; s < ns; ++s) {
         if (h_off[s + 1] < h_off[s]) return fail(h, GSAP_ERR_INVALID_ARG, "scene offsets decrease");
-        if (h_off[s + 1] - h_off[s] > 2048)
-            return fail(h, GSAP_ERR_UNSUPPORTED, "axis tests: scenes of at most 2048 objects");
+        big = big || h_off[s + 1] - h_off[s] > 2048;
+    }
+    if (slab_mode(h) || big) {
+        if (h->desc.input_kind != GSAP_INPUT_AABB || h->desc.output_kind != GSAP_OUT_CANON ||
+            (h->desc.flags & GSAP_FLAG_SKIP_SAME_GROUP))
+            return fail(h, GSAP_ERR_UNSUPPORTED,
+                        "axis tests beyond 2048 objects or with slabs: AABB scenes, canonical output, no group filter");
+        return pipeline_axis_tests(h, (hipStream_t)stream, d_in, h_off, ns, h_axes, naxes, d_tests, d_pairs);
     }
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(h, hipStreamSynchronize(st));
@@ -1435,6 +1442,7 @@ static void apply_tune(gsap_sap* h, const gsap_tune& t) {
     h->sweep_chunk = t.sweep_chunk;
     h->pair_regions = t.pair_regions;
     h->pair_local = t.pair_local;
+    h->tune_set = true;
     h->coh_valid = false;
     if (h->gexec) {  // the captured batched update used the old configuration
         (void)hipGraphExecDestroy(h->gexec);

@@ -186,7 +186,8 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 // normals[4 s .. 4 s + 2]: a coordinate normal gives the two other coordinate
 // axes (the sweep axis the one of larger variance), any other normal the two
 // directions of largest spread within the plane.  Writes axis (source 3) and
-// axis2 of those scenes; the others keep theirs.
+// axis2 of those scenes; the others keep theirs.  axis2 == null (plain
+// sweep): the vector itself becomes the scene's sweep axis.
 hipError_t launch_plane_pairs(const double* seg_mom, const uint32_t* seg_n, uint32_t n, uint32_t nseg,
                               const float* normals, float* axis, float* axis2, hipStream_t st);
 

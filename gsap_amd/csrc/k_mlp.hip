@@ -89,12 +89,17 @@ __device__ static SceneStats scene_stats(const double* m, uint32_t n) {
 
 __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sums, float*) {
     __shared__ float s_red[kWaves][kFeatSums];
-    __shared__ uint32_t s_pl[kDensWords];
+    // 8 copies of the density histograms, lane l adding to copy l mod 8:
+    // most of a wave's centres fall in a few central bins, and LDS atomics on
+    // one address serialise
+    constexpr uint32_t kCopies = 8;
+    __shared__ uint32_t s_pl[kCopies][kDensWords];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x, l = lane_id(), w = tid >> 6;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
     const uint32_t n = ti.seg_end - ti.seg_start;
     const SceneStats S = scene_stats(a.seg_mom + 9 * (size_t)ti.seg, n);
-    for (uint32_t i = tid; i < (uint32_t)kDensWords; i += kWG) s_pl[i] = 0u;
+    for (uint32_t i = tid; i < kCopies * kDensWords; i += kWG) (&s_pl[0][0])[i] = 0u;
+    uint32_t* const pl = s_pl[l & (kCopies - 1u)];
     float acc[kFeatSums];
 #pragma unroll
     for (int k = 0; k < kFeatSums; ++k) acc[k] = 0.0f;
@@ -138,11 +143,11 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
             const float q1 = floorf(z * (float)kAxisBins) + (float)(kAxisBins / 2);
             g1[k] = (int)fminf(fmaxf(q1, 0.0f), (float)(kAxisBins - 1));
         }
-        atomicAdd(&s_pl[g[0] * kPlaneGrid + g[1]], 1u);
-        atomicAdd(&s_pl[kPlaneBins + g[0] * kPlaneGrid + g[2]], 1u);
-        atomicAdd(&s_pl[2 * kPlaneBins + g[1] * kPlaneGrid + g[2]], 1u);
+        atomicAdd(&pl[g[0] * kPlaneGrid + g[1]], 1u);
+        atomicAdd(&pl[kPlaneBins + g[0] * kPlaneGrid + g[2]], 1u);
+        atomicAdd(&pl[2 * kPlaneBins + g[1] * kPlaneGrid + g[2]], 1u);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) atomicAdd(&s_pl[3 * kPlaneBins + k * kAxisBins + g1[k]], 1u);
+        for (int k = 0; k < 3; ++k) atomicAdd(&pl[3 * kPlaneBins + k * kAxisBins + g1[k]], 1u);
     }
 #pragma unroll
     for (int k = 0; k < kFeatSums; ++k) {
@@ -155,8 +160,12 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
         for (int ww = 0; ww < kWaves; ++ww) v += s_red[ww][tid];
         tile_sums[(size_t)tile * kFeatSums + tid] = v;
     }
-    for (uint32_t i = tid; i < (uint32_t)kDensWords; i += kWG)
-        if (s_pl[i]) atomicAdd(&a.plane_hist[(size_t)ti.seg * kDensWords + i], s_pl[i]);
+    for (uint32_t i = tid; i < (uint32_t)kDensWords; i += kWG) {
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kCopies; ++k) c += s_pl[k][i];
+        if (c) atomicAdd(&a.plane_hist[(size_t)ti.seg * kDensWords + i], c);
+    }
 }
 
 // One wave per scene: the tile sums in tile order (fp64), then the features.
@@ -173,6 +182,29 @@ __global__ __launch_bounds__(64) void k_features_fin(ObjPipeArgs a, const float*
         double v = 0.0;
         for (uint32_t t = t0; t < t1; ++t) v += (double)tile_sums[(size_t)t * kFeatSums + tid];
         s_fin[tid] = v;
+    }
+    // collision rates of the density histograms: sum of squared counts per
+    // histogram (3 planes, then 3 axes), lane-parallel
+    __shared__ double s_rate[6];
+    {
+        const uint32_t* dh = a.plane_hist + (size_t)seg * kDensWords;
+        double q[6];
+#pragma unroll
+        for (int h = 0; h < 6; ++h) {
+            const int base = h < 3 ? h * kPlaneBins : 3 * kPlaneBins + (h - 3) * kAxisBins;
+            const int nb = h < 3 ? kPlaneBins : kAxisBins;
+            double c = 0.0;
+            for (int b = (int)tid; b < nb; b += 64) {
+                const double x = (double)dh[base + b];
+                c += x * x;
+            }
+            q[h] = c;
+        }
+#pragma unroll
+        for (int h = 0; h < 6; ++h) {
+            const double v = wave_sum(q[h]);
+            if (tid == 0) s_rate[h] = v;
+        }
     }
     __syncthreads();
     if (tid != 0) return;
@@ -217,28 +249,21 @@ __global__ __launch_bounds__(64) void k_features_fin(ObjPipeArgs a, const float*
     // the mean of the three (only the ranking and ratios matter), halved and
     // clamped to [-1, 1].
     {
-        const uint32_t* dh = a.plane_hist + (size_t)seg * kDensWords;
         double sd[3], hm[3], lc1[3], lc2[3];
         for (int k = 0; k < 3; ++k) {
             sd[k] = sqrt(S.C[k][k] > 1e-30 ? S.C[k][k] : 1e-30);
             hm[k] = s_fin[k] / nn + 1e-30;
         }
-        auto rate = [&](const uint32_t* hb, int nb) {
-            double c2 = 0.0;
-            for (int b = 0; b < nb; ++b) {
-                const double p = (double)hb[b] / nn;
-                c2 += p * p;
-            }
-            return c2 + 1e-30;
-        };
+        // sum over bins of (count / n)^2
+        auto rate = [&](int h) { return s_rate[h] / (nn * nn) + 1e-30; };
         const int pa[3][2] = {{1, 2}, {0, 2}, {0, 1}};  // the plane normal to axis k
         const int pbase[3] = {2, 1, 0};                 // its histogram (xy, xz, yz order)
         for (int k = 0; k < 3; ++k) {
             const double bin1 = 4.0 * sd[k] / kAxisBins;
-            lc1[k] = log2(hm[k] * rate(dh + 3 * kPlaneBins + k * kAxisBins, kAxisBins) / bin1);
+            lc1[k] = log2(hm[k] * rate(3 + k) / bin1);
             const int a0 = pa[k][0], a1 = pa[k][1];
             const double area = (4.0 * sd[a0] / kPlaneGrid) * (4.0 * sd[a1] / kPlaneGrid);
-            lc2[k] = log2(hm[a0] * hm[a1] * rate(dh + pbase[k] * kPlaneBins, kPlaneBins) / area);
+            lc2[k] = log2(hm[a0] * hm[a1] * rate(pbase[k]) / area);
         }
         const double m1 = (lc1[0] + lc1[1] + lc1[2]) / 3.0, m2 = (lc2[0] + lc2[1] + lc2[2]) / 3.0;
         for (int k = 0; k < 3; ++k) {

@@ -215,7 +215,9 @@ typedef struct gsap_tune {
     int32_t slabs;       /* 1: slab sweep (GSAP_FLAG_SLABS), per scene when batched        */
     int32_t sweep_chunk; /* sweep: sub-tiles per XCD chunk, 0 (round-robin)/4/16/64       */
     int32_t pair_regions;/* sweep output regions with one allocation counter each, 1 or 8;
-                          * 8 holds 8x the pair capacity in device memory              */
+                          * 8 holds 8x the pair capacity in device memory.  A handle
+                          * whose configuration was never set or tuned switches to 8
+                          * at its first batched update with pairs                      */
     int32_t pair_local;  /* pair sort: 0 = onesweep passes over the whole key; 1 = onesweep
                           * passes over the high key bits only (8-bit digits, pair_bits
                           * unused), then each run of equal high bits sorted in LDS        */
@@ -281,12 +283,13 @@ int32_t gsap_sap_features_batched(gsap_sap* h, void* stream, const void* d_in,
                                   float* d_features);
 /* The judge: exact sweep-test count of every scene on every axis
  * (d_tests: n_scenes x n_axes) and each scene's pair count (d_pairs, may be
- * NULL).  Plain sweep: in-LDS sort per (scene, axis), scenes of at most 2048
- * objects.  Slab handle (GSAP_FLAG_SLABS; AABB, canonical output): the axes
- * are sweep-plane normals and the counts are the slab sweep's own, from one
- * count-only batched update per axis, scenes of any size.  A coordinate
- * normal sweeps the two other coordinate axes (the larger variance first); any
- * other normal the two directions of largest spread within its plane. */
+ * NULL).  Plain sweep with scenes of at most 2048 objects: in-LDS sort per
+ * (scene, axis).  Otherwise (AABB, canonical output, no group filter) the
+ * counts are the sweep's own, from one count-only batched update per axis,
+ * scenes of any size; on a slab handle (GSAP_FLAG_SLABS) the axes are
+ * sweep-plane normals: a coordinate normal sweeps the two other coordinate
+ * axes (the larger variance first), any other normal the two directions of
+ * largest spread within its plane. */
 int32_t gsap_sap_axis_tests_batched(gsap_sap* h, void* stream, const void* d_in,
                                     const uint32_t* h_scene_offsets, uint32_t n_scenes,
                                     const float* h_axes, uint32_t n_axes, uint64_t* d_tests,
