@@ -736,9 +736,14 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
             // odd number of passes, in t0p after an even one
             uint2* x = (npass_pair & 1u) ? t1p : t0p;
             uint2* y = (npass_pair & 1u) ? t0p : t1p;
+            // after the per-part pass (which read y) its bucket starts give
+            // the local stage's window starts without a search
+            uint32_t* const bst =
+                out32 ? pair_local_bucket_starts((uint32_t*)h->wstart.p, h->slot_cap) : nullptr;
+            if (bst) HIPCHK(h, launch_bucket_starts(p, y, bst, st));
             HIPCHK(h, launch_pair_local(x, y, outp, &ctrl->sort_real, &ctrl->num_pairs, cap,
                                         h->slot_cap, (uint32_t*)h->wstart.p, &ctrl->local_big,
-                                        (int)bbits, (int)shift0, out32 ? dstarts : nullptr, st));
+                                        (int)bbits, (int)shift0, out32 ? dstarts : nullptr, bst, st));
             mark(h, st);
         }
         if (sym)

@@ -365,10 +365,17 @@ uint64_t pair_local_wstart_words(uint64_t cap);  // u32 words of the window-star
 // bits above implied by the position: that pass's digit d occupies
 // [st[d], st[d + 1])), and x has room for 2 cap pairs (the upper half is
 // scratch for oversized windows).
+// bst (with st): bucket starts from launch_bucket_starts, which replace the
+// search for the window starts
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
-                             int L, const uint32_t* st, hipStream_t stream);
+                             int L, const uint32_t* st, const uint32_t* bst, hipStream_t stream);
+constexpr uint32_t kBucketWords = 256u * 256u + 1u;  // bucket starts of a two-pass pair sort
+uint32_t* pair_local_bucket_starts(uint32_t* wstart, uint64_t cap);
+// After a per-part second pass (k_xpass) over pass_in: the start of every
+// (second digit, first digit) bucket of its output, [256 * 256 + 1] words.
+hipError_t launch_bucket_starts(const SortArgs& a, const uint2* pass_in, uint32_t* bst, hipStream_t st);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
 // One-word stage entries (inputs of fewer than 2^26 entries): the object's

@@ -619,12 +619,37 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(const void* Xv, uint2*
 
 // Window starts (J + 2 words) followed by the list of handed-on windows.
 static uint64_t wstart_half(uint64_t cap) { return cap / kLocW + 2; }
-uint64_t pair_local_wstart_words(uint64_t cap) { return 2 * wstart_half(cap); }
+// window starts, the list of oversized windows, then the bucket starts of a
+// two-pass sort with 32-bit items (kBucketWords)
+uint64_t pair_local_wstart_words(uint64_t cap) { return 2 * wstart_half(cap) + kBucketWords; }
+uint32_t* pair_local_bucket_starts(uint32_t* wstart, uint64_t cap) { return wstart + 2 * wstart_half(cap); }
+
+// Window starts from bucket starts (k_bucket_starts, non-decreasing): window
+// j starts at the first bucket start at or after j * kLocW.  Thread k writes
+// the windows j with B[k-1] < j * kLocW <= B[k] (B clamped to the key count
+// n; k = 65536 covers the windows after the last bucket start); window J =
+// ceil(n / kLocW) is n.
+__global__ void k_window_starts(const uint32_t* __restrict__ bst, const unsigned long long* n_dev,
+                                uint32_t* __restrict__ wstart, uint32_t* __restrict__ big_ctr) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = (uint32_t)*n_dev;
+    const uint32_t J = (n + kLocW - 1) / kLocW;
+    if (k == 0) {
+        *big_ctr = 0u;  // read by the kernels after this one
+        wstart[0] = 0u;
+        wstart[J] = n;
+    }
+    if (k > kBucketWords - 1u) return;
+    const uint32_t hiB = k < kBucketWords - 1u ? min(bst[k], n) : n;
+    const uint32_t loB = k == 0 ? 0u : min(bst[k - 1u], n);
+    const uint32_t j1 = min(hiB / kLocW, J);
+    for (uint32_t j = loB / kLocW + 1u; j <= j1; ++j) wstart[j] = hiB;
+}
 
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
-                             int L, const uint32_t* stg, hipStream_t st) {
+                             int L, const uint32_t* stg, const uint32_t* bst, hipStream_t st) {
     if (L < 8 || L > 24) return hipErrorInvalidValue;
     const bool x32 = stg != nullptr;
     const int kbits = 2 * bbits;
@@ -638,8 +663,12 @@ hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long
     unsigned long long gl = J == 0 ? 1 : J;
     if (gl > 2048) gl = 2048;
     if (x32) {
-        hipLaunchKernelGGL(k_pair_wstart<true>, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits, L,
-                           wstart, big_ctr, stg);
+        if (bst)
+            hipLaunchKernelGGL(k_window_starts, dim3((kBucketWords + 255) / 256), dim3(256), 0, st, bst,
+                               n_dev, wstart, big_ctr);
+        else
+            hipLaunchKernelGGL(k_pair_wstart<true>, dim3((uint32_t)gw), dim3(kWG), 0, st, x, n_dev, bbits,
+                               L, wstart, big_ctr, stg);
         hipLaunchKernelGGL(k_pair_local<true>, dim3((uint32_t)gl), dim3(kLT), 0, st, x, out, n_dev,
                            last_n, last_cap, wstart, big, big_ctr, bbits, L, stg, hb);
         hipLaunchKernelGGL(k_pair_local_big<true>, dim3(256), dim3(kBigT), 0, st, x, z, y, out, n_dev,

@@ -573,6 +573,74 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
     }
 }
 
+// Bucket starts after k_xpass: bucket (d2, d1) -- the keys of second digit
+// d2 and first digit d1 -- starts at bst[d2 * 256 + d1] of the pass's output
+// (bst[65536] = the key count).  One workgroup per d1, one thread per d2:
+// part x = d1 >> 5's base for d2 (as in k_xpass), plus the part's d2 keys
+// that precede d1's first input position b -- the inclusive look-back prefix
+// of the tile before b's tile, and the d2 keys of b's tile before b, counted
+// from the pass's input.  Starts are non-decreasing in (d2, d1) order.
+template <int ITEMS>
+__global__ __launch_bounds__(kWG) void k_bucket_starts(const uint2* __restrict__ kin, SortArgs a,
+                                                       uint32_t* __restrict__ bst) {
+    constexpr uint32_t RADIX = 256, MASK = RADIX - 1u, TILE = kWG * ITEMS, NP = kMaxRegions;
+    static_assert(RADIX == kWG, "one digit per thread");
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_q[NP + 1];
+    __shared__ uint32_t s_b;
+    __shared__ uint32_t s_cnt[RADIX];
+    const uint32_t d1 = blockIdx.x, tid = threadIdx.x;
+    const unsigned long long nd = *a.n_dev_rest;
+    const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
+    const uint32_t tag = ((a.ctrl->epoch << 4) | ((uint32_t)a.ctr_base + 1u)) & 0x7FFFFFFFu;
+    uint32_t tot;
+    const uint32_t d1start = block_excl_scan(a.hist[tid], s_tmp, &tot);
+    if ((tid & 31u) == 0u) s_q[tid >> 5] = min(d1start, n);
+    if (tid == 0) s_q[NP] = min(tot, n);
+    if (tid == d1) s_b = min(d1start, n);
+    const uint32_t x = d1 >> 5;
+    uint32_t all = 0, pre = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < NP; ++q) {
+        const uint32_t c = a.joint[q * RADIX + tid];
+        all += c;
+        pre += q < x ? c : 0u;
+    }
+    const uint32_t dstart = block_excl_scan(all, s_tmp, &tot);
+    s_cnt[tid] = 0u;
+    __syncthreads();
+    const uint32_t plo = s_q[x], b = s_b;
+    uint32_t t0 = 0;  // the part's first look-back slot
+    for (uint32_t q = 0; q < x; ++q) t0 += (s_q[q + 1] - s_q[q] + TILE - 1) / TILE;
+    uint32_t prefix = 0;
+    if (b > plo) {  // uniform
+        const uint32_t T = (b - plo) / TILE;
+        if (T > 0) {
+            const unsigned long long wv = ld_acq(a.lookback + (size_t)(t0 + T - 1u) * RADIX + tid);
+            if ((uint32_t)(wv >> 33) != tag || !((wv >> 32) & 1ull))
+                atomicOr(&a.ctrl->flags, kFlagTimeout);  // the pass did not finish its chain
+            prefix = (uint32_t)wv;
+        }
+        const int shift = a.shift0 + 8;
+        for (uint32_t i = plo + T * TILE + tid; i < b; i += kWG)
+            atomicAdd(&s_cnt[KPair::digit(kin[i], shift, a.bbits, MASK)], 1u);
+        __syncthreads();
+        prefix += s_cnt[tid];
+    }
+    bst[tid * RADIX + d1] = dstart + pre + prefix;
+    if (d1 == 0 && tid == 0) bst[RADIX * RADIX] = n;
+}
+
+hipError_t launch_bucket_starts(const SortArgs& a, const uint2* pass_in, uint32_t* bst, hipStream_t st) {
+    if (a.items == 8)
+        hipLaunchKernelGGL(k_bucket_starts<8>, dim3(256), dim3(kWG), 0, st, pass_in, a, bst);
+    else if (a.items == 16)
+        hipLaunchKernelGGL(k_bucket_starts<16>, dim3(256), dim3(kWG), 0, st, pass_in, a, bst);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 // Histogram of every digit position (plain sorts and the pair sort).  Lanes
 // hold consecutive keys; equal digits in neighbouring lanes are counted as one
 // run (one LDS atomic per run), so the high digits of nearly sorted or narrow
