@@ -54,6 +54,47 @@ def traffic_for(kernel):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
+def issue_for(kernel):
+    """Instruction-issue fractions of `kernel` from the committed counter
+    summary (rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS, wave64
+    instructions per dispatch, over the same dispatches' average duration):
+    each count / (CUs x shader clock x duration), i.e. against one wave64
+    instruction of that kind per CU per cycle.  None when none is committed."""
+    path = os.path.join(ROOT, "profiles", "issue_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    cyc = d["cus"] * d["clock_mhz"] * 1e6 * k["avg_ns"] * 1e-9
+    return {"valu": round(k["valu"] / cyc, 4), "salu": round(k["salu"] / cyc, 4),
+            "lds": round(k["lds"] / cyc, 4), "avg_ns": k["avg_ns"],
+            "source": "profiles/issue_counters.json"}
+
+
+def copy_floor(torch, dev, nbytes, reps=50):
+    """A plain device copy of nbytes (half read, half written), timed with
+    hipEvents: the fraction of HBM peak a pass that moves nbytes could reach
+    at best at this size (launch and ramp included)."""
+    n = max(1, int(nbytes // 8))
+    src = torch.ones(n, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 8.0 * n / (ms / 1e3) / 1e9
+    return {"bytes": 8 * n, "ms": round(ms, 5), "achieved": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def _roof(alg_bytes, ms):
     """Roofline fraction of algorithmic bytes moved in `ms` against HBM peak."""
     gbs = alg_bytes / (ms / 1e3) / 1e9
@@ -515,20 +556,39 @@ def main():
         and 2 * max(1, (n - 1).bit_length()) > 32
     pass_bytes = [alg_pass, 12.0 * P if x32 else alg_pass]
     alg_local = 12.0 * P if x32 else alg_pass
-    cands = [("k_sweep", stages["sweep"], stages["sweep"], alg_sweep),
-             ("k_onesweep<pair>", float(pass_ms.sum()), float(pass_ms.mean()), alg_pass)]
+    # the dominant kernel: the longest single launch (pair-sort passes are
+    # separate kernels: k_msplit and k_xpass after the sweep's regions,
+    # k_onesweep otherwise)
+    if tn["pair_local"] and tn["pair_regions"] == 8 and len(pass_ms) == 2:
+        pass_names = ["k_msplit", "k_xpass"]
+    else:
+        pass_names = ["k_onesweep<pair>"] * len(pass_ms)
+    cands = [("k_sweep", stages["sweep"], alg_sweep)]
+    for i, (nm, t) in enumerate(zip(pass_names, pass_ms)):
+        cands.append((nm, float(t), pass_bytes[min(i, 1)] if len(pass_ms) == 2 else alg_pass))
     if tn["pair_local"]:
-        cands.append(("k_pair_local", stages["pair_local"], stages["pair_local"], alg_local))
-    roof_k, _, roof_ms, roof_b = max(cands, key=lambda c: c[1])
+        cands.append(("k_pair_local", stages["pair_local"], alg_local))
+    roof_k, roof_ms, roof_b = max(cands, key=lambda c: c[1])
     achieved = roof_b / (roof_ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(roof_k),
             "kernel": roof_k, "bytes_per_launch": roof_b, "ms_per_launch": round(roof_ms, 4)}
+    # The sweep is priced against HBM here, but what limits it is
+    # instruction issue (its exact tests, hit masks and staging): the
+    # counter-based issue fractions sit beside the HBM fraction.
+    iss = issue_for(roof_k)
+    if iss is not None:
+        roof["issue"] = iss
+        roof["limiter"] = "issue" if max(iss["valu"], iss["salu"]) > roof["frac"] else "hbm"
     obj_bytes = 16.0 * entries * obj_passes
     sort_gbs = obj_bytes / (stages["object_sort"] / 1e3) / 1e9
     sort_roof = {"achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "ms": round(stages["object_sort"], 4),
-                 "bytes": obj_bytes, "passes": obj_passes, "entries": entries}
+                 "bytes": obj_bytes, "passes": obj_passes, "entries": entries,
+                 # what a plain device copy of one pass's bytes reaches at
+                 # this size, and at the pair sort's size (1.2 GB)
+                 "copy_floor_per_pass": copy_floor(torch, dev, 16.0 * entries),
+                 "copy_large": copy_floor(torch, dev, 8.0 * P, reps=10)}
     pb = [pass_bytes[min(i, 1)] if len(pass_ms) == 2 else alg_pass for i in range(len(pass_ms))]
     pair_gbs = sum(pb) / (float(pass_ms.sum()) / 1e3) / 1e9
     pair_roof = {"achieved": round(pair_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
