@@ -373,7 +373,8 @@ static int32_t req_pairs(gsap_sap* h, uint64_t cap, WsReq& r) {
     const uint64_t tiles = (sorted + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items) + 1;
     r.b[W_PTMP0] = (size_t)(nreg * stride) * 8;
     r.b[W_PTMP1] = (size_t)sorted * 8;
-    r.b[W_WSTART] = (size_t)pair_local_wstart_words(rcap) * 4;
+    // window starts of the local stage, or the long-row list of the row sort
+    r.b[W_WSTART] = (size_t)std::max(pair_local_wstart_words(rcap), pair_rows_table_words(rcap)) * 4;
     r.b[W_LB_PAIR] = (size_t)tiles * 256 * 8;
     h->slot_cap = rcap;
     h->region_stride = stride;
@@ -467,7 +468,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     // the local stage: 8-bit onesweep passes over key bits [L, 2 bbits) with
     // 8 <= L <= 24, then every run of equal key >> L sorted by the low L bits.
     const uint32_t kbits = 2 * bbits;
-    const bool local = h->pair_local && kbits >= 16;
+    const bool local = h->pair_local == 1 && kbits >= 16;
+    // By rows: a counting sort on the first index, then each row by the
+    // second (8 <= bbits <= 24; other sizes take the whole-key passes).
+    const bool rows = !count_only && h->pair_local == 2 && bbits >= 8 && bbits <= 24;
     const uint32_t npass_g = !local ? 0 : (kbits > 24 ? (kbits - 24 + 7) / 8 : 1);
     const uint32_t shift0 = local ? kbits - 8 * npass_g : 0;
     const uint32_t pair_bits = local ? 8 : (uint32_t)h->pair_bits;
@@ -686,9 +690,21 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     w.bbits = (int)bbits;
     HIPCHK(h, launch_sweep(w, st));
     mark(h, st);
-    if (!count_only) {
-        const uint32_t nreg = (uint32_t)h->pair_regions;
+    const uint32_t nreg = (uint32_t)h->pair_regions;
+    if (!count_only)
         HIPCHK(h, launch_pair_regions(ctrl, h->slot_cap, nreg, sort_tile(h->pair_items), st));
+    if (rows) {
+        // the row counters are `has` (zero between updates), the row starts
+        // `rowstart`, the block totals `block_sums`; the long-row list lives
+        // in the window-start table
+        HIPCHK(h, launch_pair_rows(t0p, t1p, t0p, outp, ctrl, nreg, h->region_stride, h->sort_cap, cap,
+                                   n, (int)bbits, (uint32_t*)h->has.p, (uint32_t*)h->rowstart.p,
+                                   (uint32_t*)h->block_sums.p, sym ? entries : nullptr,
+                                   (uint32_t*)h->wstart.p, st));
+        mark(h, st);
+        if (scene_out)
+            HIPCHK(h, launch_scene_offsets(outp, ctrl, cap, seg.seg_start, seg.nseg, scene_out, st));
+    } else if (!count_only) {
         if (!sweep_hist)
             HIPCHK(h, launch_pair_hist(t0p, &ctrl->sort_n, h->sort_cap, a.hist_pair, npass_pair,
                                        bbits, pair_bits, shift0, ctrl->region_start,
@@ -1418,7 +1434,7 @@ static bool valid_tune(const gsap_tune* t) {
            (t->sweep_chunk == 0 || t->sweep_chunk == 4 || t->sweep_chunk == 16 ||
             t->sweep_chunk == 64) &&
            (t->pair_regions == 1 || t->pair_regions == kMaxRegions) &&
-           (t->pair_local == 0 || t->pair_local == 1);
+           (t->pair_local >= 0 && t->pair_local <= 2);
 }
 
 static gsap_tune tune_of(const gsap_sap* h) {
@@ -1650,6 +1666,11 @@ extern "C" int32_t gsap_sap_autotune(gsap_sap* h, void* stream, const void* d_in
                             bits != base.pair_bits || grid != base.pair_grid)
                             consider(t);
                     }
+        if (base.pair_local != 2) {  // by rows (digit parameters unused)
+            gsap_tune t = base;
+            t.pair_local = 2;
+            consider(t);
+        }
     }
     h->timing = timing;
     (void)hipEventDestroy(e0);

@@ -371,11 +371,33 @@ hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
                              int L, const uint32_t* st, const uint32_t* bst, hipStream_t stream);
+// The big-window kernel alone, over a given list: window k of big[0, *big_ctr)
+// is positions [wst[big[k]], wst[big[k] + 1]) of x, sorted by its low L bits
+// into out (y: scratch at the same positions).
+hipError_t launch_local_big_windows(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
+                                    const unsigned long long* last_n, unsigned long long last_cap,
+                                    const uint32_t* wst, const uint32_t* big, const uint32_t* big_ctr,
+                                    int bbits, int L, hipStream_t st);
 constexpr uint32_t kBucketWords = 256u * 256u + 1u;  // bucket starts of a two-pass pair sort
 uint32_t* pair_local_bucket_starts(uint32_t* wstart, uint64_t cap);
 // After a per-part second pass (k_xpass) over pass_in: the start of every
 // (second digit, first digit) bucket of its output, [256 * 256 + 1] words.
 hipError_t launch_bucket_starts(const SortArgs& a, const uint2* pass_in, uint32_t* bst, hipStream_t st);
+
+// ---- pair sort by rows (k_pair_rows.hip) ----
+// Counting sort of the pairs in P's regions (ctrl's region table) on the
+// first index into X (dense), then each row ordered by the second index into
+// out (positions < min(sort_real, num_pairs, cap)); Y: scratch of the sorted
+// size.  n objects, 8 <= bbits <= 24.  cnt: n zeroed words (left zeroed),
+// rs: n words, bsum: 2 ceil(n / 4096) words, table: pair_rows_table_words(cap)
+// words.  entries (non-null): the symmetric output's row starts and
+// ctrl->num_entries.
+uint64_t pair_rows_table_words(uint64_t cap);
+hipError_t launch_pair_rows(const uint2* P, uint2* X, uint2* Y, uint2* out, Ctrl* ctrl,
+                            uint32_t nreg, unsigned long long stride,
+                            unsigned long long sort_cap, unsigned long long cap, uint32_t n,
+                            int bbits, uint32_t* cnt, uint32_t* rs, uint32_t* bsum,
+                            int32_t* entries, uint32_t* table, hipStream_t st);
 constexpr uint32_t kSweepStage = 512;      // pairs staged in LDS per sweep wave
 constexpr uint32_t kSweepStageSlab = 560;  // the same, slab sweep (fewer flushes)
 // One-word stage entries (inputs of fewer than 2^26 entries): the object's

@@ -646,6 +646,16 @@ __global__ void k_window_starts(const uint32_t* __restrict__ bst, const unsigned
     for (uint32_t j = loB / kLocW + 1u; j <= j1; ++j) wstart[j] = hiB;
 }
 
+hipError_t launch_local_big_windows(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
+                                    const unsigned long long* last_n, unsigned long long last_cap,
+                                    const uint32_t* wst, const uint32_t* big, const uint32_t* big_ctr,
+                                    int bbits, int L, hipStream_t st) {
+    if (L < 8 || L > 24) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pair_local_big<false>, dim3(256), dim3(kBigT), 0, st, x, x, y, out, n_dev,
+                       last_n, last_cap, wst, big, big_ctr, bbits, L, nullptr, 0);
+    return hipGetLastError();
+}
+
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,

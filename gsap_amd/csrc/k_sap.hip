@@ -1289,16 +1289,19 @@ __global__ __launch_bounds__(kWG) void k_entries_write(uint32_t* has, const uint
                                                        uint32_t n, const uint32_t* block_sums,
                                                        int32_t* entries, Ctrl* ctrl) {
     __shared__ uint32_t s_tmp[kWaves];
-    __shared__ uint32_t s_base;
     const uint32_t b0 = blockIdx.x * 2048u, tid = threadIdx.x;
-    if (tid == 0) {
-        uint32_t s = 0;
-        for (uint32_t b = 0; b < blockIdx.x; ++b) s += block_sums[b];
-        s_base = s;
-        if (blockIdx.x == gridDim.x - 1) ctrl->num_entries = s + block_sums[blockIdx.x];
-    }
+    // this block's base: the earlier blocks' totals, summed by all threads
+    uint32_t base = 0;
+    for (uint32_t b = tid; b < blockIdx.x; b += kWG) base += block_sums[b];
+    base = wave_sum(base);
+    if (lane_id() == 0) s_tmp[tid >> 6] = base;
     __syncthreads();
-    uint32_t run = s_base;
+    base = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) base += s_tmp[w];
+    __syncthreads();  // s_tmp is the scans' scratch below
+    if (tid == 0 && blockIdx.x == gridDim.x - 1) ctrl->num_entries = base + block_sums[blockIdx.x];
+    uint32_t run = base;
     for (uint32_t c0 = b0; c0 < min(b0 + 2048u, n); c0 += kWG) {
         const uint32_t i = c0 + tid;
         const uint32_t h = (i < n) ? has[i] : 0u;

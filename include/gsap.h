@@ -196,8 +196,9 @@ int32_t gsap_sap_set_workspace(gsap_sap* h, void* d_ws, uint64_t bytes);
 /* Per-stage timing of the last gsap_sap_collide with timing enabled (ms,
  * hipEvents on the launch stream).  Stages, in order: moments+project,
  * object_sort, gather, sweep, then (unless count-only) pair_hist, one entry
- * per pair-sort onesweep pass, pair_local (only with gsap_tune.pair_local),
- * pair_tail (row starts / scene offsets); last: finish. */
+ * per pair-sort onesweep pass, pair_local (only with gsap_tune.pair_local 1),
+ * pair_tail (row starts / scene offsets); last: finish.  With pair_local 2 the
+ * row sort takes the place of pair_hist and there are no passes. */
 int32_t gsap_sap_stage_times(gsap_sap* h, float* h_ms, int32_t max_stages, int32_t* n_stages);
 int32_t gsap_sap_set_timing(gsap_sap* h, int32_t enable);
 
@@ -220,7 +221,10 @@ typedef struct gsap_tune {
                           * at its first batched update with pairs                      */
     int32_t pair_local;  /* pair sort: 0 = onesweep passes over the whole key; 1 = onesweep
                           * passes over the high key bits only (8-bit digits, pair_bits
-                          * unused), then each run of equal high bits sorted in LDS        */
+                          * unused), then each run of equal high bits sorted in LDS; 2 =
+                          * by rows: a counting sort on the first index, then each row by
+                          * the second (for lists of about one pair per object; objects
+                          * 2^8 .. 2^24, else the whole-key passes run)                    */
     float ms;            /* out: measured update time of this configuration (autotune)    */
     int32_t from_cache;  /* out: 1 if the configuration came from the cache file          */
 } gsap_tune;
