@@ -561,15 +561,12 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         ca.axis = a.axis;
         ca.axis2 = a.axis2;
         ca.slab = a.slab;
-        ca.prev_vals = valsF;
-        ca.split_key = (unsigned long long*)cb;
-        ca.bcount = (uint32_t*)(cb + (size_t)kCohMaxB * 8);
-        ca.bstart = ca.bcount + kCohMaxB;
-        ca.bcursor = ca.bstart + kCohMaxB + 1;
-        ca.keys0 = a.keys;
-        ca.vals0 = a.vals;
-        ca.keys1 = a.keys2;
-        ca.vals1 = a.vals2;
+        ca.split = (unsigned long long*)cb;
+        ca.far = ca.split + kCohMaxB;
+        ca.kept = (uint32_t*)(ca.far + kCohFar);
+        // kept entries go to the pair the LSD passes do not end in
+        ca.keysX = (npass_obj & 1u) ? a.keys : a.keys2;
+        ca.valsX = (npass_obj & 1u) ? a.vals : a.vals2;
         ca.keysF = keysF;
         ca.valsF = valsF;
         ca.slab_cnt = a.slab_cnt;
@@ -604,7 +601,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     }
     if (slabs) HIPCHK(h, launch_slab_params(a, st));
     if (keep) {
-        HIPCHK(h, launch_coh_project(ca, st));
+        HIPCHK(h, launch_coh_block(ca, st));
     } else {
         HIPCHK(h, launch_project(a, st));
     }
@@ -624,9 +621,10 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     s.grid = 0;
     int which = 0;
     if (keep) {
-        // bucket sort; if a bucket overflowed, the regular projection (entry
-        // order; entries per slab counted already) and LSD passes run instead
-        HIPCHK(h, launch_coh_sort(ca, st));
+        // side list and placement; if the side list overflowed, the regular
+        // projection (entry order; entries per slab counted already) and LSD
+        // passes run instead
+        HIPCHK(h, launch_coh_merge(ca, st));
         ObjPipeArgs af = a;
         af.run_if = &ctrl->coh_fallback;
         af.no_slab_count = 1;

@@ -66,9 +66,10 @@ struct Ctrl {
     uint32_t epoch;                  // bumped per update; lookback tags
     uint32_t tile_ctr[2 * kMaxPasses];
     uint32_t local_big;              // windows the pair sort's LDS stage handed on
-    uint32_t coh_fallback;           // coherent object sort: 1 = a bucket overflowed, the
-                                     // regular projection and LSD passes run instead
-    uint32_t pad[2];
+    uint32_t coh_fallback;           // coherent object sort: 1 = the regular projection and
+                                     // LSD passes run instead
+    uint32_t coh_far;                // coherent object sort: side-list entries claimed
+    uint32_t pad;
 };
 
 enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u,
@@ -214,46 +215,48 @@ hipError_t launch_scene_offsets(const uint2* P, Ctrl* ctrl, unsigned long long c
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
 // ---- coherent object sort (k_coherent.hip) ----
 // Successive updates of one object set reuse the previous update's sorted
-// order: every kCohB-th entry of it gives a splitter, each entry is placed in
-// its splitter bucket with one scatter, and each bucket is sorted in LDS by
-// (key, entry position) -- the order of the stable LSD sort, so the result is
-// identical.  A bucket of more than kCohCap entries sets ctrl->coh_fallback.
-constexpr uint32_t kCohB = 1024u;      // entries per bucket of the previous order
-constexpr uint32_t kCohCap = 2048u;    // largest bucket sorted in LDS
-constexpr uint32_t kCohMaxB = 8192u;   // buckets (entries <= kCohB * kCohMaxB)
+// order: each block of kCohB entries of it keeps the entries whose new
+// composite (key << 32 | entry position) stays in the block's previous range
+// and puts them in order in LDS; the others go to a side list that is sorted
+// and merged in -- the order of the stable LSD sort, so the result is
+// identical.  A side list of more than kCohFar entries, or an out-of-order
+// run within a block reaching more than kCohRun entries to either side of one
+// of its entries, sets ctrl->coh_fallback.  Kept entries out of order with
+// the one kCohDist places away go to the side list as well.
+constexpr uint32_t kCohB = 2048u;      // entries per block of the previous order
+constexpr uint32_t kCohMaxB = 8192u;   // blocks (entries <= kCohB * kCohMaxB)
+constexpr uint32_t kCohFar = 4096u;    // side list: entries that left their block's range
+constexpr uint32_t kCohDist = 64u;    // an entry out of order with the one this far away: side list
+constexpr uint32_t kCohRun = 128u;
 constexpr uint32_t kCohRefresh = 64u;  // updates between refreshes of the key function
-// small buffer: split_key[kCohMaxB] u64, then count[kCohMaxB],
-// start[kCohMaxB + 1] and cursor[kCohMaxB] u32
-constexpr size_t kCohSmallBytes = (size_t)kCohMaxB * 8 + (size_t)kCohMaxB * 4 * 3 + 64;
+// small buffer: split[kCohMaxB] u64, far[kCohFar] u64, kept[kCohMaxB] u32
+constexpr size_t kCohSmallBytes = (size_t)kCohMaxB * 8 + (size_t)kCohFar * 8 + (size_t)kCohMaxB * 4;
 struct CohArgs {
     const void* in;
     int kind;
     uint32_t ne;                 // entries (slab mode 2n, else n)
-    uint32_t nb;                 // buckets, ceil(ne / kCohB)
+    uint32_t nb;                 // blocks, ceil(ne / kCohB)
     int slabs;
     const float* axis;           // sweep axis u
     const float* axis2;          // slab axis v
     const float* slab;           // slab parameters
-    const uint32_t* prev_vals;   // the previous update's sorted values
-    unsigned long long* split_key;  // [nb - 1] composite keys of the previous sorted
-                                    // entries at positions kCohB * (j + 1)
-    uint32_t* bcount;            // [nb]
-    uint32_t* bstart;            // [nb + 1]
-    uint32_t* bcursor;           // [nb]
-    uint32_t* keys0;             // entries in the previous order (projection output)
-    uint32_t* vals0;
-    uint32_t* keys1;             // bucketed
-    uint32_t* vals1;
-    uint32_t* keysF;             // sorted (the LSD path's final buffers)
-    uint32_t* valsF;
+    unsigned long long* split;   // [nb] split[k] (k >= 1): the previous update's composite
+                                 // at position k kCohB
+    unsigned long long* far;     // [kCohFar] side list
+    uint32_t* kept;              // [nb] entries each block keeps
+    uint32_t* keysX;             // kept entries, in order within each block (at its positions)
+    uint32_t* valsX;
+    uint32_t* keysF;             // the previous sorted order in, this update's out (the LSD
+    uint32_t* valsF;             // path's final buffers)
     uint32_t* slab_cnt;
-    Ctrl* ctrl;
+    Ctrl* ctrl;                  // coh_far and coh_fallback: zeroed by k_moments
 };
-// splitters + projection in the previous order with bucket counts (the
-// projection stage), then scan, scatter and the per-bucket LDS sort
-hipError_t launch_coh_project(const CohArgs& a, hipStream_t st);
-hipError_t launch_coh_sort(const CohArgs& a, hipStream_t st);
-// the splitters of the next update from the sorted keys and values (keysF, valsF)
+// the blocks (new keys in the previous order, the kept entries in order, the
+// side list and the entries per slab)
+hipError_t launch_coh_block(const CohArgs& a, hipStream_t st);
+// the side list sorted, then every block's entries at their final positions
+hipError_t launch_coh_merge(const CohArgs& a, hipStream_t st);
+// the splits of the next update from the sorted keys and values (keysF, valsF)
 hipError_t launch_coh_save(const CohArgs& a, hipStream_t st);
 // k_scratch.hip: device address (this GPU's copy) and size of the static
 // scratch of the allocation-free entry point

@@ -6,16 +6,17 @@
 // finds about one directed pair per particle), digit passes over the whole
 // key are latency-bound: each pass is a chain of tile look-backs over a few
 // megabytes.  A counting sort on the first index needs no chain:
-//  * k_row_count: pairs per row, one atomic per pair on n row counters;
+//  * k_row_count: pairs per row, on n row counters (one atomic per row a
+//    wave holds);
 //  * k_row_sums: per block of kRowB rows, the pair total and the non-empty
 //    rows;
 //  * k_row_starts: each block's prefix (the earlier blocks' totals, summed in
 //    parallel), then every row's start.  The symmetric output's row starts
 //    (the first position of every non-empty row) come out of the same scan,
 //    and the counters are zeroed again for the next update;
-//  * k_row_scatter: each pair into its row, at a slot claimed by an atomic on
-//    the row's start (which leaves the row's end there: row r then spans
-//    [start[r - 1], start[r]));
+//  * k_row_scatter: each pair into its row, at slots a wave claims by one
+//    atomic per row on the row's start (which leaves the row's end there:
+//    row r then spans [start[r - 1], start[r]));
 //  * k_row_sort: a pair of a row of at most kRowSmall pairs goes to its row's
 //    start plus the number of the row's pairs with a smaller second index;
 //    longer rows are listed for the local stage's big-window kernel (one
@@ -31,26 +32,57 @@ constexpr uint32_t kRowPer = kRowB / kWG;    // consecutive rows per thread
 constexpr uint32_t kRowSmall = 64;           // longest row ranked by comparisons
 constexpr uint32_t kRowGrid = 2048;          // workgroups of the pair-streaming kernels
 
-// f(pair) for every pair the sweep wrote: region r holds region_valid[r] -
-// region_start[r] pairs from position r * stride of P (one region: all of
-// them from 0).  After a capacity overflow every region is empty.
+// Lanes of the wave that hold the same row x (valid lanes; all lanes of the
+// wave call it together): the sweep writes a pair's both orientations and a
+// box's pairs together, so a row recurs within a wave, and atomics on one
+// address serialise.  Up to kMatchRounds groups are found by ballots (the
+// lowest unmatched lane's row each round); lanes left after that form groups
+// of their own.
+constexpr int kMatchRounds = 16;
+__device__ __forceinline__ unsigned long long row_group(uint32_t x, bool valid) {
+    unsigned long long todo = ballot(valid);
+    unsigned long long mine = 1ull << lane_id();
+    for (int r = 0; r < kMatchRounds && todo; ++r) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t xl = (uint32_t)__builtin_amdgcn_readlane((int)x, (int)l);
+        const bool eq = valid && x == xl;
+        const unsigned long long m = ballot(eq);
+        if (eq) mine = m;
+        todo &= ~m;
+    }
+    return mine;
+}
+
+// f(pair, valid) for every pair the sweep wrote, 64 positions per wave per
+// iteration: region r holds region_valid[r] - region_start[r] pairs from
+// position r * stride of P (one region: all of them from 0).  After a
+// capacity overflow every region is empty.  Every lane of a wave runs the
+// same iterations (the group search needs the whole wave).
 template <class F>
-__device__ __forceinline__ void for_each_pair(const uint2* __restrict__ P, const Ctrl* ctrl,
-                                              uint32_t nreg, unsigned long long stride, F f) {
+__device__ __forceinline__ void for_each_pair_wave(const uint2* __restrict__ P, const Ctrl* ctrl,
+                                                   uint32_t nreg, unsigned long long stride, F f) {
     const unsigned long long g = blockIdx.x * (unsigned long long)kWG + threadIdx.x;
     const unsigned long long gs = (unsigned long long)gridDim.x * kWG;
+    const unsigned long long g0 = g - lane_id();  // the wave's first position
     for (uint32_t r = 0; r < nreg; ++r) {
         const unsigned long long m = ctrl->region_valid[r] - ctrl->region_start[r];
         const uint2* src = P + (unsigned long long)r * stride;
-        for (unsigned long long k = g; k < m; k += gs) f(src[k]);
+        for (unsigned long long k0 = g0; k0 < m; k0 += gs) {
+            const unsigned long long k = k0 + lane_id();
+            const bool valid = k < m;
+            f(valid ? src[k] : make_uint2(0xFFFFFFFFu, 0u), valid);
+        }
     }
 }
 
 __global__ __launch_bounds__(kWG) void k_row_count(const uint2* __restrict__ P, const Ctrl* ctrl,
                                                    uint32_t nreg, unsigned long long stride,
                                                    uint32_t n, uint32_t* __restrict__ cnt) {
-    for_each_pair(P, ctrl, nreg, stride, [&](uint2 p) {
-        if (p.x < n) atomicAdd(&cnt[p.x], 1u);
+    for_each_pair_wave(P, ctrl, nreg, stride, [&](uint2 p, bool valid) {
+        const bool ok = valid && p.x < n;
+        const unsigned long long grp = row_group(p.x, ok);
+        // the group's lowest lane adds the group's size
+        if (ok && (grp & lanemask_lt()) == 0ull) atomicAdd(&cnt[p.x], (uint32_t)__popcll(grp));
     });
 }
 
@@ -164,8 +196,16 @@ __global__ __launch_bounds__(kWG) void k_row_scatter(const uint2* __restrict__ P
                                                      uint32_t nreg, unsigned long long stride,
                                                      uint32_t n, uint32_t* __restrict__ rs,
                                                      uint2* __restrict__ X) {
-    for_each_pair(P, ctrl, nreg, stride, [&](uint2 p) {
-        if (p.x < n) X[atomicAdd(&rs[p.x], 1u)] = p;
+    for_each_pair_wave(P, ctrl, nreg, stride, [&](uint2 p, bool valid) {
+        const bool ok = valid && p.x < n;
+        const unsigned long long grp = row_group(p.x, ok);
+        const unsigned long long below = grp & lanemask_lt();
+        uint32_t base = 0u;
+        if (ok && below == 0ull) base = atomicAdd(&rs[p.x], (uint32_t)__popcll(grp));
+        // the group's slots follow its lowest lane's claim, in lane order
+        const uint32_t lead = (uint32_t)__builtin_ctzll(grp);
+        base = (uint32_t)__shfl((int)base, (int)lead, 64);
+        if (ok) X[base + (uint32_t)__popcll(below)] = p;
     });
 }
 

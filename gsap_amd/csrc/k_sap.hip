@@ -127,9 +127,29 @@ __global__ __launch_bounds__(kWG) void k_moments(ObjPipeArgs a) {
             if (a.pair_claim)
                 for (uint32_t i = tid; i < kPairClaimWords; i += kWG) a.pair_claim[i] = 0u;
             if (tid == 0) a.ctrl->epoch += 1u;
+            if (tid == 0 && a.keep_key) {  // the coherent sort's side list and fallback flag
+                a.ctrl->coh_far = 0u;
+                a.ctrl->coh_fallback = 0u;
+            }
         }
     }
 
+    if (a.keep_key) {  // the key function is kept (no axis from these moments): the input check only
+        bool bad = false;
+        for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
+            if (a.kind == 0) {
+                const float2* b = reinterpret_cast<const float2*>(a.in) + 3 * (size_t)i;
+                const float2 p0 = b[0], p1 = b[1], p2 = b[2];
+                bad |= !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y) &&
+                         isfinite(p2.x) && isfinite(p2.y));
+            } else {
+                const float4 q = reinterpret_cast<const float4*>(a.in)[i];
+                bad |= !(isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w));
+            }
+        }
+        if (ballot(bad) && lane_id() == 0) atomicOr(&a.ctrl->flags, kFlagNonFinite);
+        return;
+    }
     // s[0..8]: sum c, sum c c^T; s[9..11]: sum of the extents; e[0..2]: largest extents
     double s[12], e[3] = {0.0, 0.0, 0.0};
 #pragma unroll
