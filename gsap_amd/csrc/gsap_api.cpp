@@ -564,6 +564,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         ca.split = (unsigned long long*)cb;
         ca.far = ca.split + kCohMaxB;
         ca.kept = (uint32_t*)(ca.far + kCohFar);
+        ca.fstart = ca.kept + kCohMaxB;
+        ca.pstart = ca.fstart + kCohMaxB + 1;
         // kept entries go to the pair the LSD passes do not end in
         ca.keysX = (npass_obj & 1u) ? a.keys : a.keys2;
         ca.valsX = (npass_obj & 1u) ? a.vals : a.vals2;

@@ -229,8 +229,10 @@ constexpr uint32_t kCohFar = 4096u;    // side list: entries that left their blo
 constexpr uint32_t kCohDist = 64u;    // an entry out of order with the one this far away: side list
 constexpr uint32_t kCohRun = 128u;
 constexpr uint32_t kCohRefresh = 64u;  // updates between refreshes of the key function
-// small buffer: split[kCohMaxB] u64, far[kCohFar] u64, kept[kCohMaxB] u32
-constexpr size_t kCohSmallBytes = (size_t)kCohMaxB * 8 + (size_t)kCohFar * 8 + (size_t)kCohMaxB * 4;
+// small buffer: split[kCohMaxB] u64, far[kCohFar] u64, kept[kCohMaxB] u32,
+// fstart[kCohMaxB + 1] u32, pstart[kCohMaxB] u32
+constexpr size_t kCohSmallBytes =
+    (size_t)kCohMaxB * 8 + (size_t)kCohFar * 8 + (size_t)kCohMaxB * 4 * 3 + 4;
 struct CohArgs {
     const void* in;
     int kind;
@@ -244,6 +246,8 @@ struct CohArgs {
                                  // at position k kCohB
     unsigned long long* far;     // [kCohFar] side list
     uint32_t* kept;              // [nb] entries each block keeps
+    uint32_t* fstart;            // [nb + 1] each block's first side-list entry
+    uint32_t* pstart;            // [nb] each block's first final position
     uint32_t* keysX;             // kept entries, in order within each block (at its positions)
     uint32_t* valsX;
     uint32_t* keysF;             // the previous sorted order in, this update's out (the LSD

@@ -20,10 +20,12 @@
 //    out-of-order runs, each of which holds exactly its own entries, ranked
 //    by comparisons within the run.  Entries that left the range go to a
 //    side list.
-//  * k_coh_far_sort: one workgroup sorts the side list in LDS.
+//  * k_coh_far_sort: one workgroup sorts the side list in LDS, then finds
+//    each block's first side-list entry and first final position (the kept
+//    entries of the blocks before it plus the side-list entries below its
+//    range);
 //  * k_coh_place: block k's kept entries and the side-list entries of its
-//    range, merged, go to the block's final positions (the kept entries of
-//    the blocks before it plus the side-list entries below its range).
+//    range, merged, go to the block's final positions.
 // A side list longer than kCohFar, or an out-of-order run reaching more than
 // kCohRun entries to either side of one of its entries, sets
 // ctrl->coh_fallback, and the regular projection and LSD passes (gated on
@@ -367,9 +369,9 @@ __global__ __launch_bounds__(kWG) void k_coh_block(CohArgs a) {
 __global__ __launch_bounds__(kFarT) void k_coh_far_sort(CohArgs a) {
     if (a.ctrl->coh_fallback) return;
     __shared__ unsigned long long s_a[kCohFar], s_b[kCohFar];
+    __shared__ uint32_t s_w[kFarT / 64];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
     const uint32_t m = a.ctrl->coh_far;  // <= kCohFar (else the fallback is set)
-    if (m <= 1u) return;
     for (uint32_t c0 = w * 256u; c0 < m; c0 += (kFarT / 64) * 256u) {
         unsigned long long x[4];
 #pragma unroll
@@ -419,30 +421,34 @@ __global__ __launch_bounds__(kFarT) void k_coh_far_sort(CohArgs a) {
         dst = t;
     }
     for (uint32_t i = tid; i < m; i += kFarT) a.far[i] = src[i];
+    // per block: its first side-list entry (the side-list entries below its
+    // range) and its first final position (plus the kept entries before it)
+    const uint32_t nb = a.nb, per = (nb + kFarT - 1) / kFarT, k0 = tid * per;
+    uint32_t t = 0u;
+    for (uint32_t k = k0; k < min(k0 + per, nb); ++k) t += a.kept[k];
+    const uint32_t inc = wave_incl_scan(t);
+    if (lane == 63u) s_w[w] = inc;
+    __syncthreads();
+    uint32_t run = inc - t;
+    for (uint32_t j = 0; j < w; ++j) run += s_w[j];
+    for (uint32_t k = k0; k < min(k0 + per, nb); ++k) {
+        const uint32_t f = k ? lower_bound64(src, 0u, m, a.split[k]) : 0u;
+        a.fstart[k] = f;
+        a.pstart[k] = run + f;
+        run += a.kept[k];
+    }
+    if (tid == 0) a.fstart[nb] = m;
 }
 
 __global__ __launch_bounds__(kWG) void k_coh_place(CohArgs a) {
     if (a.ctrl->coh_fallback) return;
-    __shared__ uint32_t s_w[kWaves];
     const uint32_t k = blockIdx.x, tid = threadIdx.x;
     const uint32_t p0 = k * kCohB;
-    uint32_t before = 0u;  // kept entries of the blocks before this one
-    for (uint32_t j = tid; j < k; j += kWG) before += a.kept[j];
-    before = wave_sum(before);
-    if (lane_id() == 0) s_w[tid >> 6] = before;
-    __syncthreads();
-    before = 0u;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) before += s_w[w];
-    const uint32_t nf = a.ctrl->coh_far;
-    const unsigned long long lo = k ? a.split[k] : 0ull;
-    const unsigned long long hi = k + 1u < a.nb ? a.split[k + 1u] : ~0ull;
-    const uint32_t f0 = k ? lower_bound64(a.far, 0u, nf, lo) : 0u;
-    const uint32_t f1 = k + 1u < a.nb ? lower_bound64(a.far, f0, nf, hi) : nf;
-    const uint32_t M = a.kept[k], base = before + f0;
+    const uint32_t f0 = a.fstart[k], f1 = a.fstart[k + 1u];
+    const uint32_t M = a.kept[k], base = a.pstart[k];
     for (uint32_t i = tid; i < M; i += kWG) {
         const uint32_t key = a.keysX[p0 + i], v = a.valsX[p0 + i];
-        const uint32_t r = lower_bound64(a.far, f0, f1, composite(key, v, a.slabs)) - f0;
+        const uint32_t r = f0 < f1 ? lower_bound64(a.far, f0, f1, composite(key, v, a.slabs)) - f0 : 0u;
         a.keysF[base + i + r] = key;
         a.valsF[base + i + r] = v;
     }
