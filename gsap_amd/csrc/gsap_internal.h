@@ -98,8 +98,9 @@ struct ObjPipeArgs {
     uint32_t* keys2;
     uint32_t* vals2;
     void* rec;              // [n] sorted records (candidate stream)
-    void* grp;              // [n] records grouped by sweep end per 64 positions
-    uint32_t* grp_end;      // [n] sweep end of each grouped record
+    void* grp;              // [n] records grouped by sweep end per 64 positions (waves
+                            // with a window longer than kShortWindow only)
+    uint32_t* grp_end;      // [n] sweep end - 1 per grouped slot; top bit: grp written
     const float* learned_axis;  // [nseg][4] learned axis proposal (w = confidence) or null
     double* seg_mom;        // [nseg][9] per-scene fp64 moments (written by k_axis)
     // if set: the sweep tests of scene s are added to seg_tests[s * seg_tests_stride]

@@ -663,8 +663,10 @@ __global__ __launch_bounds__(kWG) void k_project(ObjPipeArgs a) {
 // ---------------------------------------------------------------- gather
 // Sorted position s -> candidate record rec[s]; sweep end (first position
 // whose key exceeds s's upper key) by binary search; the sweep-test count.
-// Each wave's 64 positions are also written re-ordered by sweep end into
-// grp/grp_end (aux = s), the object side of the sweep.
+// grp_end[g]: the sweep ends of each wave's 64 positions minus one, re-ordered
+// by sweep end where a window is longer than kShortWindow (then the records
+// too, into grp with aux = s, and the top bit of the wave's grp_end words is
+// set); the object side of the sweep.
 template <int KIND>
 __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* __restrict__ K,
                                                 const uint32_t* __restrict__ V) {
@@ -753,7 +755,8 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
         const unsigned long long mk = ((unsigned long long)end << 6) | lane;
         const uint32_t wlen = (valid && end > s + 1u) ? end - s - 1u : 0u;
         uint32_t rank = lane;
-        if (ballot(wlen > kShortWindow)) {
+        const bool perm = ballot(wlen > kShortWindow) != 0ull;  // uniform
+        if (perm) {
             rank = 0;
             const uint32_t mlo = (uint32_t)mk, mhi = (uint32_t)(mk >> 32);
 #pragma unroll 8
@@ -764,10 +767,14 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
             }
         }
         if (valid) {
-            r.aux = s;
             const uint32_t g = s - lane + rank;  // the wave's first position + rank
-            reinterpret_cast<Rec*>(a.grp)[g] = r;
-            a.grp_end[g] = end;
+            // the grouped record only where the order differs from rec's; the
+            // top bit of every grp_end word of the wave says which
+            if (perm) {
+                r.aux = s;
+                reinterpret_cast<Rec*>(a.grp)[g] = r;
+            }
+            a.grp_end[g] = (end - 1u) | (perm ? 0x80000000u : 0u);
         }
     }
     // one atomic per workgroup (the counter is shared by the whole grid)
@@ -986,7 +993,11 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
 
     if (g0 < hi) {
         const uint32_t nobj = uniform(min(64u, hi - g0));
-        const uint32_t ge = lane < nobj ? a.grp_end[g0 + lane] : 0u;  // ascending in lane order
+        // sweep ends (stored minus one, top bit: the wave was regrouped, and
+        // its records are in grp; ascending in lane order then)
+        const uint32_t gw = lane < nobj ? a.grp_end[g0 + lane] : 0u;
+        const bool perm = ballot(lane < nobj && (gw >> 31) != 0u) != 0ull;
+        const uint32_t ge = lane < nobj ? (gw & 0x7FFFFFFFu) + 1u : 0u;
         const uint32_t wend = uniform(wave_max_u32(ge));
         // Short windows (every object of the wave has at most kShortWindow
         // candidates: sparse scenes, slabs of small scenes): each lane takes
@@ -998,8 +1009,13 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs a) {
         Rec me{};
         uint32_t my_s = 0u, len = 0u;
         if (lane < nobj) {
-            me = reinterpret_cast<const Rec*>(a.grp)[g0 + lane];
-            my_s = me.aux;
+            if (perm) {
+                me = reinterpret_cast<const Rec*>(a.grp)[g0 + lane];
+                my_s = me.aux;
+            } else {  // not regrouped: every window is short, and rec has the records
+                me = R[g0 + lane];
+                my_s = g0 + lane;
+            }
             len = ge > my_s + 1u ? ge - my_s - 1u : 0u;
         }
         if constexpr (S4) s_wobj[w][lane] = SLAB ? (me.orig & ~kFirstBit) : me.orig;

@@ -49,7 +49,7 @@ def test_set_tune_rejects_invalid(dev):
     s = G.Sap()
     for bad in (dict(sort_items=5), dict(sort_bits=5), dict(pair_bits=9), dict(pair_items=12),
                 dict(pair_grid=100), dict(slabs=2), dict(sweep_chunk=5), dict(pair_regions=4),
-                dict(pair_local=2)):
+                dict(pair_local=3)):
         with pytest.raises(G.GsapError) as e:
             s.set_tune(**bad)
         assert e.value.code == G.ERR_INVALID_ARG
