@@ -87,23 +87,33 @@ __device__ static SceneStats scene_stats(const double* m, uint32_t n) {
     return S;
 }
 
+constexpr uint32_t kFeatGroup = 8;  // sort tiles per feature workgroup
+
 __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sums, float*) {
     __shared__ float s_red[kWaves][kFeatSums];
-    // 8 copies of the density histograms, lane l adding to copy l mod 8:
+    // 16 copies of the density histograms, lane l adding to copy l mod 16:
     // most of a wave's centres fall in a few central bins, and LDS atomics on
-    // one address serialise
-    constexpr uint32_t kCopies = 8;
-    __shared__ uint32_t s_pl[kCopies][kDensWords];
+    // one address serialise.  A copy is kDensWords + 1 words long, so the
+    // copies of a bin lie in different LDS banks (kDensWords is a multiple of
+    // the 32 banks).
+    constexpr uint32_t kCopies = 16;
+    __shared__ uint32_t s_pl[kCopies][kDensWords + 1];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x, l = lane_id(), w = tid >> 6;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
+    // one workgroup per kFeatGroup sort tiles of a scene (the first one's):
+    // fewer partial histograms to add up
+    const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg] : 0u;
+    if ((tile - t0) % kFeatGroup != 0u) return;
+    const uint32_t i_end = min(ti.lo + kFeatGroup * a.tile, ti.seg_end);
     const uint32_t n = ti.seg_end - ti.seg_start;
     const SceneStats S = scene_stats(a.seg_mom + 9 * (size_t)ti.seg, n);
-    for (uint32_t i = tid; i < kCopies * kDensWords; i += kWG) (&s_pl[0][0])[i] = 0u;
+    for (uint32_t i = tid; i < kCopies * (kDensWords + 1); i += kWG) (&s_pl[0][0])[i] = 0u;
     uint32_t* const pl = s_pl[l & (kCopies - 1u)];
     float acc[kFeatSums];
 #pragma unroll
     for (int k = 0; k < kFeatSums; ++k) acc[k] = 0.0f;
-    const float ell = (float)S.ell;
+    const float inv_ell = (float)(1.0 / S.ell);
+    unsigned long long ext8[3] = {0ull, 0ull, 0ull};  // extent classes, 8 x 8-bit counts per axis
     // density grids: bins over mean +- 2 standard deviations per axis
     float gs[3];
 #pragma unroll
@@ -112,7 +122,7 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
         gs[k] = (float)(1.0 / (4.0 * sd));
     }
     __syncthreads();
-    for (uint32_t i = ti.lo + tid; i < ti.hi; i += kWG) {
+    for (uint32_t i = ti.lo + tid; i < i_end; i += kWG) {
         float lo[3], hi[3], c4[4];
         load_lohi(a.in, a.kind, i, lo, hi, c4);
         float h[3], d[3];
@@ -126,11 +136,13 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
             acc[6 + k] += fabsf(d[k]);
             const float d2 = d[k] * d[k];
             acc[9 + k] += d2 * d2;
-            const float r = h[k] / ell;
-            int bin = r > 0.0f ? (int)floorf(log2f(r)) + 4 : 0;
+            // extent class: floor(log2(h / ell)) + 4 in [0, 7], from the
+            // exponent field; counted in 8-bit fields (a thread takes at most
+            // kFeatGroup * tile / kWG <= 128 boxes)
+            const float r = h[k] * inv_ell;
+            int bin = r > 0.0f ? (int)((__float_as_uint(r) >> 23) & 0xFFu) - 127 + 4 : 0;
             bin = bin < 0 ? 0 : (bin > 7 ? 7 : bin);
-#pragma unroll
-            for (int b = 0; b < 8; ++b) acc[12 + 8 * k + b] += (b == bin) ? 1.0f : 0.0f;
+            ext8[k] += 1ull << (8 * bin);
         }
         acc[36] += h[0] * h[1];
         acc[37] += h[0] * h[2];
@@ -149,6 +161,10 @@ __global__ __launch_bounds__(kWG) void k_features(ObjPipeArgs a, float* tile_sum
 #pragma unroll
         for (int k = 0; k < 3; ++k) atomicAdd(&pl[3 * kPlaneBins + k * kAxisBins + g1[k]], 1u);
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[12 + 8 * k + b] = (float)((ext8[k] >> (8 * b)) & 0xFFu);
 #pragma unroll
     for (int k = 0; k < kFeatSums; ++k) {
         const float v = wave_sum(acc[k]);
@@ -180,7 +196,7 @@ __global__ __launch_bounds__(64) void k_features_fin(ObjPipeArgs a, const float*
     const uint32_t t1 = a.seg.tile_seg ? a.seg.seg_tile0[seg + 1] : a.ntiles;
     if (tid < kFeatSums) {
         double v = 0.0;
-        for (uint32_t t = t0; t < t1; ++t) v += (double)tile_sums[(size_t)t * kFeatSums + tid];
+        for (uint32_t t = t0; t < t1; t += kFeatGroup) v += (double)tile_sums[(size_t)t * kFeatSums + tid];
         s_fin[tid] = v;
     }
     // collision rates of the density histograms: sum of squared counts per
