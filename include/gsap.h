@@ -81,11 +81,13 @@ enum {
     /* Temporal coherence for successive updates of one object set (the
      * reference carries only its cell interval between frames, Sap.cpp:226-243).
      * The key function (axes, slab and key parameters) of a refresh is kept
-     * for the next 63 single-scene updates of the same n, and the previous
-     * update's sorted order picks balanced buckets for a one-scatter object
-     * sort with a per-bucket LDS sort.  The pair list is identical to a
-     * from-scratch update's; a bucket that would overflow falls back to the
-     * regular sort on the device.  Batched updates ignore the flag. */
+     * for the next 63 single-scene updates of the same n, and the object
+     * sort starts from the previous update's order: blocks of it keep the
+     * entries that stay in their previous key range and fix their local
+     * order in LDS; the entries that moved farther are sorted on their own
+     * and merged in.  The pair list is identical to a from-scratch update's;
+     * too many far movers fall back to the regular sort on the device.
+     * Batched updates ignore the flag. */
     GSAP_FLAG_COHERENT = 4u
 };
 
