@@ -89,3 +89,48 @@ def test_coherent_particle_trajectories_bit_identical(dev):
         for a, b in zip(st[0][:4], st[1][:4]):
             assert np.isfinite(a).all() and np.array_equal(a, b), k
     assert st[0][4] > 0  # bodies were in contact
+
+
+def _drift(n, seed, steps, jumpers):
+    """n random AABBs drifting by small steps, `jumpers` of them moved far
+    each step."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 40.0, (n, 3)).astype(np.float32)
+    h = rng.uniform(0.1, 0.6, (n, 3)).astype(np.float32)
+    for _ in range(steps):
+        c = c + rng.normal(0.0, 0.05, (n, 3)).astype(np.float32)
+        if jumpers:
+            j = rng.choice(n, min(n, jumpers), replace=False)
+            c[j] = rng.uniform(0.0, 40.0, (len(j), 3)).astype(np.float32)
+        yield np.concatenate([c - h, c + h], 1).astype(np.float32)
+
+
+# sizes around one block of the previous order (2,048 entries; two per box
+# with slabs)
+@pytest.mark.parametrize("n,slabs", [(1, True), (700, True), (1024, True), (1025, True),
+                                     (2048, False), (2049, False), (5000, True)])
+def test_coherent_small_and_ragged_sizes(dev, n, slabs):
+    import torch
+    flags = G.FLAG_SLABS if slabs else 0
+    coh = G.Sap(axis_mode=G.AXIS_AUTO, flags=flags | G.FLAG_COHERENT)
+    for t, boxes in enumerate(_drift(n, n, 6, 3)):
+        x = to_dev(boxes, dev)
+        ref, _, _ = G.Sap(axis_mode=G.AXIS_AUTO, flags=flags).collide(x)
+        got, _, _ = coh.collide(x)
+        assert torch.equal(got, ref), t
+        if t == 5:
+            assert np.array_equal(pairs_np(got), O.brute(boxes)), t
+
+
+@pytest.mark.parametrize("slabs", [True, False])
+def test_coherent_side_list_overflow_falls_back(dev, slabs):
+    # 8,000 of 60,000 boxes jump each step: more far movers than the side
+    # list holds, so the device runs the regular sort; pairs stay equal
+    import torch
+    flags = G.FLAG_SLABS if slabs else 0
+    coh = G.Sap(axis_mode=G.AXIS_AUTO, flags=flags | G.FLAG_COHERENT)
+    for t, boxes in enumerate(_drift(60_000, 7, 5, 8000)):
+        x = to_dev(boxes, dev)
+        ref, _, _ = G.Sap(axis_mode=G.AXIS_AUTO, flags=flags).collide(x)
+        got, _, _ = coh.collide(x)
+        assert torch.equal(got, ref), t
