@@ -416,6 +416,7 @@ constexpr uint32_t kStage4Shift = 26u;
 constexpr uint32_t kStage4Mask = (1u << kStage4Shift) - 1u;
 constexpr uint32_t kSweepStage4 = 1280;
 constexpr uint32_t kShortWindow = 64;      // sweep: lane-per-object path up to this window
+constexpr uint32_t kGatherWin = 320;       // gather: sorted keys each wave holds in LDS
 constexpr uint32_t kSweepHistPasses = 2;   // pair-sort digit histograms the sweep can count
 // Histogram of u32 keys for a plain (non-fused) sort, all passes.
 hipError_t launch_key_hist(const uint32_t* keys, uint32_t n, uint32_t* hist, int npass, int bits,

@@ -672,8 +672,13 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
                                                 const uint32_t* __restrict__ V) {
     using Rec = typename std::conditional<KIND == 0, RecAabb, RecSphere>::type;
     __shared__ uint32_t s_sb[257];  // slab mode: first sorted position of each slab (+ end)
+    // each wave's next kGatherWin sorted keys: most sweep ends are found
+    // there, by a binary search in LDS, instead of a chain of dependent
+    // global loads
+    __shared__ uint32_t s_kw[kWaves][kGatherWin];
     const uint32_t tile = blockIdx.x, tid = threadIdx.x;
     const TileInfo ti = tile_info(a.seg, a.n, tile, a.tile);
+    uint32_t* const kw = s_kw[tid >> 6];
     const float* ax = a.axis + 4 * (size_t)ti.seg;
     const float u[3] = {ax[0], ax[1], ax[2]};
     const bool exact = axis_exact(KIND, u);
@@ -691,6 +696,13 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
     for (uint32_t sub = ti.lo; sub < ti.hi; sub += kWG) {
         const uint32_t s = sub + tid;
         const bool valid = s < ti.hi;
+        // keys at positions [wb, we): after the wave's first position
+        const uint32_t wb = sub + (tid & ~63u) + 1u;
+        const uint32_t we = min(wb + kGatherWin, ti.seg_end);
+#pragma unroll
+        for (uint32_t j = lane_id(); j < kGatherWin; j += 64u)
+            if (wb + j < we) kw[j] = K[wb + j];
+        __builtin_amdgcn_wave_barrier();
         Rec r;
         uint32_t end = 0xFFFFFFFFu;  // invalid lanes rank last
         if (valid) {
@@ -712,9 +724,17 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
                                         : flip_f32(ph);
             uint32_t b = s + 1, e = ti.seg_end;
             if (a.slabs) e = unused ? b : s_sb[(key >> kSlabQBits) + 1];  // within the slab
-            while (b < e) {
-                const uint32_t mid = b + ((e - b) >> 1);
-                if (K[mid] <= hk) b = mid + 1; else e = mid;
+            const uint32_t le = min(e, we);  // the part of [b, e) held in LDS
+            uint32_t lh = le;
+            while (b < lh) {
+                const uint32_t mid = b + ((lh - b) >> 1);
+                if (kw[mid - wb] <= hk) b = mid + 1; else lh = mid;
+            }
+            if (b == le) {  // every held key is <= hk: the end lies past them
+                while (b < e) {
+                    const uint32_t mid = b + ((e - b) >> 1);
+                    if (K[mid] <= hk) b = mid + 1; else e = mid;
+                }
             }
             end = b;
             tests += (unsigned long long)(b - s - 1);

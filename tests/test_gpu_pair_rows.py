@@ -180,3 +180,28 @@ def test_rows_particle_trajectories_bit_identical(dev):
         for a, b in zip(st[0][:4], st[1][:4]):
             assert np.isfinite(a).all() and np.array_equal(a, b), k
     assert st[0][4] > 0  # bodies were in contact
+
+
+def test_rows_sym_csr_capacity_error_then_retry(dev):
+    # the sweep counts the rows as it writes: after an overflowing update its
+    # counters are cleared and the row starts are not written; the retry
+    # matches the whole-key sort, pairs and row starts
+    import torch
+    sph = np.concatenate([np.random.default_rng(4).uniform(0, 30, (20000, 3)),
+                          np.full((20000, 1), 0.5)], 1).astype(np.float32)
+    g = to_dev((np.arange(20000) // 27).astype(np.int32), dev)
+    x = to_dev(sph, dev)
+    ref = G.Sap(input_kind=G.INPUT_SPHERE, output_kind=G.OUT_SYM_CSR,
+                flags=G.FLAG_SKIP_SAME_GROUP)
+    ref.set_tune(pair_local=0)
+    rp, re_, _ = ref.collide(x, group=g)
+    h = G.Sap(input_kind=G.INPUT_SPHERE, output_kind=G.OUT_SYM_CSR, flags=G.FLAG_SKIP_SAME_GROUP)
+    h.set_tune(pair_local=2, pair_regions=8)
+    small = torch.empty((rp.shape[0] // 3, 2), dtype=torch.int32, device=dev)
+    ent = torch.empty(20000, dtype=torch.int32, device=dev)
+    with pytest.raises(G.GsapError) as e:
+        h.collide_into(x, small, group=g, entries=ent)
+    assert e.value.code == G.ERR_CAPACITY
+    for _ in range(2):
+        got, ge, _ = h.collide(x, group=g)
+        assert torch.equal(got, rp) and torch.equal(ge, re_)
