@@ -632,6 +632,9 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         af.no_slab_count = 1;
         HIPCHK(h, launch_project(af, st));
         s.run_if = &ctrl->coh_fallback;
+        // the passes claim their tiles, so one workgroup per CU sorts them
+        // all; launched that small, they cost little when they return at once
+        s.grid = h->cus;
     }
     HIPCHK(h, sort_kv32(s, a.keys, a.vals, a.keys2, a.vals2, &which, st));
     if (coherent) {
