@@ -199,17 +199,18 @@ def cpu_baseline_c2(boxes, gpu_pairs, P, runs):
                       f"threads after one warm-up, and one update on 1 thread"}
 
 
-def bench_c1(G, W, torch, dev, steps, warmup, with_cpu):
+def bench_c1(G, W, torch, dev, steps, warmup, with_cpu, cache=None):
     """16,384 boxes (centres U[0,64)^3, half-extents U[0.1,0.5] per axis, seed
-    0), PCA axis: GPU updates/s, the CPU baseline on 1 and all threads (median
-    of 10 after 2 warm-ups) and the O(n^2) brute force the lists are checked
-    against."""
+    0), PCA axis, autotuned handle (as the headline's): GPU updates/s, the CPU
+    baseline on 1 and all threads (median of 10 after 2 warm-ups) and the
+    O(n^2) brute force the lists are checked against."""
     import numpy as np
     from oracle import oracle as O
     boxes = W.small()
     n = len(boxes)
     bd = torch.from_numpy(boxes).to(dev)
     sap = G.Sap(max_n=n, axis_mode=G.AXIS_PCA)
+    tune = sap.autotune(bd, cache=cache)
     P, st = sap.count(bd)
     pairs = torch.empty((max(P, 1), 2), dtype=torch.int32, device=dev)
     ms = _time_ms(torch, lambda: sap.collide_into(bd, pairs), steps, warmup)
@@ -221,7 +222,8 @@ def bench_c1(G, W, torch, dev, steps, warmup, with_cpu):
                        "PCA axis", "n": n, "pairs": P, "sweep_tests": int(st.sweep_tests),
            "ms_per_update": round(ms, 4), "updates_per_s": round(n / (ms / 1e3), 1),
            "identical_to_brute_force": bool(np.array_equal(got, ref)),
-           "brute_force_s": round(t_brute, 3)}
+           "brute_force_s": round(t_brute, 3),
+           "tune": {k: v for k, v in tune.items() if k != "from_cache"}}
     if with_cpu:
         lib, flags = cpu_lib()
         info = host_info()
@@ -484,7 +486,7 @@ def main():
     ap.add_argument("--prb-steps", type=int, default=1000)
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file (a hit skips the measurement of configurations)")
-    ap.add_argument("--only", choices=["batched", "learned", "particles"], default=None,
+    ap.add_argument("--only", choices=["batched", "learned", "particles", "c1"], default=None,
                     help="run one extra section alone and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -505,6 +507,9 @@ def main():
             r = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup, args.tune_cache)
         elif args.only == "learned":
             r = bench_learned(G, W, torch, dev, args.learned_scenes)
+        elif args.only == "c1":
+            r = bench_c1(G, W, torch, dev, args.steps, args.warmup, not args.no_cpu_baseline,
+                         args.tune_cache)
         else:
             r = bench_particles(G, W, torch, dev, args.prb_bodies, args.prb_steps,
                                 not args.no_cpu_baseline, args.tune_cache)
@@ -672,7 +677,8 @@ def main():
                                "axis_used": [round(x, 6) for x in stq.axis]}
             pca.close()
     if not args.no_extras:
-        out["c1"] = bench_c1(G, W, torch, dev, args.steps, args.warmup, not args.no_cpu_baseline)
+        out["c1"] = bench_c1(G, W, torch, dev, args.steps, args.warmup, not args.no_cpu_baseline,
+                             args.tune_cache)
         out["batched"] = bench_batched(G, W, torch, dev, args.scenes, args.steps, args.warmup,
                                        args.tune_cache)
         out["learned_axis"] = bench_learned(G, W, torch, dev, args.learned_scenes)
