@@ -449,8 +449,11 @@ __global__ __launch_bounds__(kWG) void k_coh_place(CohArgs a) {
     for (uint32_t i = tid; i < M; i += kWG) {
         const uint32_t key = a.keysX[p0 + i], v = a.valsX[p0 + i];
         const uint32_t r = f0 < f1 ? lower_bound64(a.far, f0, f1, composite(key, v, a.slabs)) - f0 : 0u;
-        a.keysF[base + i + r] = key;
-        a.valsF[base + i + r] = v;
+        const uint32_t o = base + i + r;  // < ne: the blocks' ranges tile [0, ne)
+        if (o < a.ne) {
+            a.keysF[o] = key;
+            a.valsF[o] = v;
+        }
     }
     for (uint32_t j = f0 + tid; j < f1; j += kWG) {
         const unsigned long long d = a.far[j];
@@ -459,8 +462,11 @@ __global__ __launch_bounds__(kWG) void k_coh_place(CohArgs a) {
             const uint32_t mid = b + ((e - b) >> 1);
             if (composite(a.keysX[p0 + mid], a.valsX[p0 + mid], a.slabs) < d) b = mid + 1u; else e = mid;
         }
-        a.keysF[base + (j - f0) + b] = (uint32_t)(d >> 32);
-        a.valsF[base + (j - f0) + b] = entry_val((uint32_t)d, a.slabs);
+        const uint32_t o = base + (j - f0) + b;
+        if (o < a.ne) {
+            a.keysF[o] = (uint32_t)(d >> 32);
+            a.valsF[o] = entry_val((uint32_t)d, a.slabs);
+        }
     }
 }
 

@@ -210,11 +210,15 @@ __global__ __launch_bounds__(kWG) void k_row_scatter(const uint2* __restrict__ P
 }
 
 __global__ __launch_bounds__(kWG) void k_row_sort(const uint2* __restrict__ X, const Ctrl* ctrl,
-                                                  unsigned long long cap,
+                                                  unsigned long long cap, uint32_t nobj,
                                                   const uint32_t* __restrict__ rs,
                                                   uint2* __restrict__ out, uint32_t* __restrict__ wst,
                                                   uint32_t* __restrict__ big, uint32_t* big_ctr) {
-    const unsigned long long n = ctrl->sort_real;
+    // the placed pairs: every row ends where the last one does (a pair with
+    // an out-of-range first index, which the sweep never writes, was not
+    // placed)
+    const unsigned long long placed = rs[nobj - 1u];
+    const unsigned long long n = min(ctrl->sort_real, placed);
     unsigned long long lim = n;
     if (ctrl->num_pairs < lim) lim = ctrl->num_pairs;
     if (cap < lim) lim = cap;
@@ -222,6 +226,7 @@ __global__ __launch_bounds__(kWG) void k_row_sort(const uint2* __restrict__ X, c
     const unsigned long long gs = (unsigned long long)gridDim.x * kWG;
     for (unsigned long long p = blockIdx.x * (unsigned long long)kWG + threadIdx.x; p < n; p += gs) {
         const uint2 q = X[p];
+        if (q.x >= nobj) continue;
         const uint32_t s = q.x ? rs[q.x - 1u] : 0u, e = rs[q.x];
         if (e - s <= kRowSmall) {
             uint32_t rank = 0;
@@ -257,7 +262,7 @@ hipError_t launch_pair_rows(const uint2* P, uint2* X, uint2* Y, uint2* out, Ctrl
     hipLaunchKernelGGL(k_row_sums, dim3(nb), dim3(kWG), 0, st, cnt, n, bsum, nb);
     hipLaunchKernelGGL(k_row_starts, dim3(nb), dim3(kWG), 0, st, cnt, n, bsum, nb, rs, entries, ctrl);
     hipLaunchKernelGGL(k_row_scatter, dim3(grid), dim3(kWG), 0, st, P, ctrl, nreg, stride, n, rs, X);
-    hipLaunchKernelGGL(k_row_sort, dim3(grid), dim3(kWG), 0, st, X, ctrl, cap, rs, out, wst, big,
+    hipLaunchKernelGGL(k_row_sort, dim3(grid), dim3(kWG), 0, st, X, ctrl, cap, n, rs, out, wst, big,
                        big_ctr);
     return launch_local_big_windows(X, Y, out, &ctrl->sort_real, &ctrl->num_pairs, cap, wst, big,
                                     big_ctr, bbits, bbits, st);
