@@ -31,9 +31,17 @@ HBM_PEAK_GBS = 8000.0
 def stage_dict(t, local=False):
     """Library stage times [moments+project, object_sort, gather, sweep,
     pair_hist, one per pair-sort onesweep pass..., pair_local (with the local
-    stage), pair_tail, finish] -> (named stages, per-pass array)."""
+    stage), pair_tail, finish] -> (named stages, per-pass array).  With the
+    sort by rows (local == 2) the stages after the sweep are [rows,
+    pair_tail, finish]; the rows stage is reported as pair_local."""
     import numpy as np
     t = np.asarray(t, np.float64)
+    if local == 2:
+        st = {"moments+project": t[0], "object_sort": t[1], "gather": t[2], "sweep": t[3],
+              "pair_hist": 0.0, "pair_passes": 0.0, "pair_local": t[4], "pair_tail": t[5],
+              "finish": t[6]}
+        st["pair_sort"] = st["pair_local"] + st["pair_tail"]
+        return {k: float(v) for k, v in st.items()}, np.zeros(0)
     passes = t[5:len(t) - (3 if local else 2)]
     st = {"moments+project": t[0], "object_sort": t[1], "gather": t[2], "sweep": t[3],
           "pair_hist": t[4], "pair_passes": float(passes.sum()),
@@ -561,12 +569,13 @@ def main():
     obj_passes = -(-(24 if tn["slabs"] else 32) // tn["sort_bits"])  # slab keys: 24 bits
     alg_sweep = 24.0 * n + 8.0 * P
     alg_pass = 16.0 * P
-    # Keys of at most 40 bits sorted in two passes with 8 pair regions: the
-    # second pass writes 4-byte items (8 B read + 4 B written per pair) and
-    # the local stage reads them (4 B read + 8 B written).
-    x32 = bool(tn["pair_local"]) and tn["pair_regions"] == 8 and 2 * max(1, (n - 1).bit_length()) <= 40 \
+    # Keys of 34 to 40 bits sorted in two passes with 8 pair regions: both
+    # passes write 4-byte items (first pass: 8 B read + 4 B written per pair;
+    # second: 4 B + 4 B) and the local stage reads them (4 B read + 8 B
+    # written).
+    x32 = tn["pair_local"] == 1 and tn["pair_regions"] == 8 and 2 * max(1, (n - 1).bit_length()) <= 40 \
         and 2 * max(1, (n - 1).bit_length()) > 32
-    pass_bytes = [alg_pass, 12.0 * P if x32 else alg_pass]
+    pass_bytes = [12.0 * P if x32 else alg_pass, 8.0 * P if x32 else alg_pass]
     alg_local = 12.0 * P if x32 else alg_pass
     # the dominant kernel: the longest single launch (pair-sort passes are
     # separate kernels: k_msplit and k_xpass after the sweep's regions,

@@ -726,12 +726,11 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.hist_reg = msplit ? a.pair_claim : nullptr;
         p.claim = msplit ? a.pair_claim + kMaxRegions * 256 : nullptr;
         p.joint = joint;
-        // keys of at most 40 bits after the per-part pass: 32-bit items for
+        // after the per-part pass (keys of at most 40 bits): 32-bit items for
         // the local stage (it reads 4 bytes per pair; the top bits are the
         // pass's digit, implied by the position)
-        const bool out32 = joint != nullptr && kbits <= 40;
+        const bool out32 = joint != nullptr;
         uint32_t* const dstarts = a.pair_claim + kPairClaimWords;
-        p.out32 = out32 ? 1 : 0;
         p.st_out = out32 ? dstarts : nullptr;
         // the per-part pass runs best at three workgroups per CU (more tiles
         // in flight per XCD evict the lines neighbouring runs share)
@@ -910,7 +909,6 @@ static int32_t read_counts(gsap_sap* h, hipStream_t st, uint64_t cap, bool count
     }
     if (c[3] & kFlagTimeout) return fail(h, GSAP_ERR_HIP, "internal: sort look-back wait expired");
     if (c[3] & kFlagSlab) return fail(h, GSAP_ERR_HIP, "internal: a box spans more than two slabs");
-    if (c[3] & kFlagCoherent) return fail(h, GSAP_ERR_HIP, "internal: coherent sort bucket overflow");
     if (c[3] & kFlagNonFinite) return fail(h, GSAP_ERR_NONFINITE, "non-finite value in input boxes");
     if (!count_only && c[0] > cap) {
         char buf[160];

@@ -72,8 +72,7 @@ struct Ctrl {
     uint32_t pad;
 };
 
-enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u,
-                  kFlagCoherent = 16u };
+enum : uint32_t { kFlagCapacity = 1u, kFlagNonFinite = 2u, kFlagTimeout = 4u, kFlagSlab = 8u };
 
 // ---- launch wrappers (k_sap.hip) ----
 struct ObjPipeArgs {
@@ -337,15 +336,15 @@ struct SortArgs {
     // counters of the same shape (zero), or null for a onesweep first pass
     const uint32_t* hist_reg;
     uint32_t* claim;
-    // pair sort of two passes after a per-region first pass: [kMaxRegions][256]
-    // counts of the second digit per part (part = first digit >> 5), taken
-    // by the sweep; the second pass then runs per part, on xgrid workgroups
+    // pair sort of two passes after a per-region first pass (keys of 34 to
+    // 40 bits): [kMaxRegions][256] counts of the second digit per part
+    // (part = first digit >> 5), taken by the sweep; the second pass then
+    // runs per part, on xgrid workgroups.  Both passes write 32-bit items:
+    // the first the key without its digit, the second the key's low 32
+    // bits (the bits above are its digit's) and the digit starts
+    // st_out[0..256] (the exclusive scan of its counts)
     uint32_t* joint;
     uint32_t xgrid;
-    // with joint and keys of at most 40 bits: the per-part pass writes each
-    // key's low 32 bits (uint32 items; the bits above are its digit's) and
-    // the digit starts st_out[0..256] (the exclusive scan of its counts)
-    int out32;
     uint32_t* st_out;
     // every pass returns at once unless *run_if != 0 (null: always runs)
     const uint32_t* run_if;

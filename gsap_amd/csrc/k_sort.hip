@@ -51,6 +51,23 @@ __device__ __forceinline__ unsigned long long lb_word(uint32_t tag, bool inc, ui
     return ((unsigned long long)tag << 33) | (inc ? (1ull << 32) : 0ull) | (unsigned long long)v;
 }
 
+// A pair key of at most 40 bits without its 8-bit digit at `shift`: the bits
+// above the digit move down to `shift`, the bits below stay (32 bits).
+__device__ __forceinline__ uint32_t pair_mid32(uint2 k, int shift, int bbits) {
+    const unsigned long long L = ((unsigned long long)k.x << bbits) | (unsigned long long)k.y;
+    return (uint32_t)(((L >> (shift + 8)) << shift) | (L & ((1ull << shift) - 1ull)));
+}
+
+// The digit d whose range [st[d], st[d + 1]) of a pass's output holds
+// position p (st: nondecreasing, st[0] = 0 <= p; the last such d).
+__device__ __forceinline__ uint32_t digit_at(const uint32_t* st, uint32_t lo, uint32_t hi, uint32_t p) {
+    while (lo < hi) {  // the last d in [lo, hi] with st[d] <= p
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (st[mid] <= p) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
 template <class KT, int BITS, int ITEMS>
 __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restrict__ kin,
                                                   typename KT::T* __restrict__ kout,
@@ -264,8 +281,10 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
 // that two neighbouring runs share before writing them back.  Lines written
 // in parts from two L2s cost HBM about twice their bytes.  Items are staged
 // in LDS in digit order (LDS-atomic ranks) for a coalesced scatter, never
-// past the pair count.
-template <int BITS, int ITEMS>
+// past the pair count.  MID32 (followed by k_xpass, keys of at most 40 bits):
+// the output position implies the digit, so each key is written as the 32
+// bits left without it (pair_mid32).
+template <int BITS, int ITEMS, bool MID32>
 __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, uint2* __restrict__ kout,
                                                 SortArgs a) {
     constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
@@ -353,7 +372,12 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
             const uint2 kk = s_k[i];
             const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
             const uint32_t gp = s_gbase[d] + (i - s_tstart[d]);
-            if (gp < nreal) kout[gp] = kk;
+            if (gp < nreal) {
+                if constexpr (MID32)
+                    reinterpret_cast<uint32_t*>(kout)[gp] = pair_mid32(kk, shift, a.bbits);
+                else
+                    kout[gp] = kk;
+            }
         }
         __syncthreads();
     }
@@ -372,24 +396,29 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
 // before writing them back.  Within a part: tiles claimed through the
 // part's counter, ranking (below), look-back to the part's first tile
 // (inclusive), LDS-staged coalesced scatter.
-template <int ITEMS, bool OUT32>
-__global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, uint2* __restrict__ kout,
+// Keys have at most 40 bits and both passes write 32-bit items: the input
+// item is the key without d1 (pair_mid32; d1 is the first pass's digit range
+// that holds the position), the output item the key's low 32 bits (the bits
+// above lie in d, implied by the position; the digit starts go to st_out).
+template <int ITEMS>
+__global__ __launch_bounds__(kWG) void k_xpass(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
                                                SortArgs a) {
     constexpr uint32_t BITS = 8, RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
-    constexpr uint32_t NP = kMaxRegions;
+    constexpr uint32_t NP = kMaxRegions, DPP = RADIX / NP;  // first digits per part
     static_assert(RADIX == kWG, "one digit per thread");
     __shared__ uint32_t s_wh[kWaves][RADIX];
     __shared__ uint32_t s_tstart[RADIX];
     __shared__ uint32_t s_gbase[RADIX];
     __shared__ uint32_t s_tmp[2 * kWaves];
     __shared__ uint32_t s_cnt[RADIX];
-    __shared__ uint2 s_k[TILE];
-    __shared__ uint32_t s_q[NP + 1];   // part starts in the input
-    __shared__ uint32_t s_t0[NP];      // each part's first look-back slot
+    __shared__ uint2 s_k[TILE];          // (output item, digit)
+    __shared__ uint32_t s_d1[RADIX + 1]; // the first pass's digit starts
+    __shared__ uint32_t s_t0[NP];        // each part's first look-back slot
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const int shift = a.shift0 + (int)BITS;
+    const int sh = a.shift0;  // an input item's digit d: item >> sh
+    const uint32_t lowm = (1u << sh) - 1u;
     unsigned long long nd = *a.n_dev_rest;
     const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
     const uint32_t x = blockIdx.x % NP;
@@ -398,11 +427,11 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
     const unsigned long long ltm = lanemask_lt();
     const unsigned long long gtm = ~ltm & ~(1ull << lane);
     const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
-    // part starts: the first pass's digit starts at d1 = 32 x
+    // the first pass's digit starts; part x spans those of d1 = DPP x
     uint32_t tot;
     const uint32_t d1start = block_excl_scan(a.hist[tid], s_tmp, &tot);
-    if ((tid & 31u) == 0u) s_q[tid >> 5] = min(d1start, n);
-    if (tid == 0) s_q[NP] = min(tot, n);
+    s_d1[tid] = min(d1start, n);
+    if (tid == 0) s_d1[RADIX] = min(tot, n);
     // digit tid's base for this part: its start over all parts plus its
     // counts in the parts before this one
     uint32_t all = 0, pre = 0;
@@ -414,7 +443,7 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
     }
     const uint32_t dstart = block_excl_scan(all, s_tmp, &tot);
     const uint32_t dbase = dstart + pre;
-    if (OUT32 && blockIdx.x == 0) {  // the digit starts, for the readers of the 32-bit items
+    if (blockIdx.x == 0) {  // the digit starts, for the readers of the output items
         a.st_out[tid] = dstart;
         if (tid == 0) a.st_out[RADIX] = tot;
     }
@@ -423,11 +452,11 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
         uint32_t t = 0;
         for (uint32_t q = 0; q < NP; ++q) {
             s_t0[q] = t;
-            t += (s_q[q + 1] - s_q[q] + TILE - 1) / TILE;
+            t += (s_d1[(q + 1) * DPP] - s_d1[q * DPP] + TILE - 1) / TILE;
         }
     }
     s_cnt[tid] = 0u;
-    const uint32_t plo = s_q[x], phi = s_q[x + 1];
+    const uint32_t plo = s_d1[x * DPP], phi = s_d1[(x + 1) * DPP];
 
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_ctr[ctr_idx], 1u);
@@ -440,20 +469,19 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
         const bool first = ti == 0u;
         const uint32_t gtile = s_t0[x] + ti;
 
-        uint2 key[ITEMS];
-        uint32_t rank[ITEMS], dig[ITEMS];
+        uint32_t item[ITEMS], rank[ITEMS], dig[ITEMS];
         const uint32_t base = lo + wave * 64u * ITEMS + lane;
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t idx = base + 64u * k;
-            if (idx < hi) key[k] = kin[idx];
+            if (idx < hi) item[k] = kin[idx];
         }
         // tile digit counts, published at once; a run of equal digits in
         // neighbouring lanes is one LDS atomic
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const bool valid = (base + 64u * k) < hi;
-            const uint32_t d = valid ? KPair::digit(key[k], shift, a.bbits, MASK) : 0u;
+            const uint32_t d = valid ? (item[k] >> sh) & MASK : 0u;
             dig[k] = d;
             const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
             const bool head = valid && (lane == 0u || prev != d);
@@ -475,8 +503,9 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
         // one d1 value: its items take their places in any order (one LDS
         // atomic each); a tile across a d1 boundary ranks them stably
         // (items in tile order (wave, item, lane), peers by BITS ballots).
-        const bool one_d1 = KPair::digit(kin[lo], a.shift0, a.bbits, MASK) ==
-                            KPair::digit(kin[hi - 1], a.shift0, a.bbits, MASK);  // uniform
+        const uint32_t d1lo = digit_at(s_d1, x * DPP, x * DPP + DPP - 1u, lo);
+        const uint32_t d1hi = digit_at(s_d1, d1lo, x * DPP + DPP - 1u, hi - 1u);
+        const bool one_d1 = d1lo == d1hi;  // uniform
         __syncthreads();  // s_cnt is zero
         if (one_d1) {
 #pragma unroll
@@ -551,23 +580,22 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
-            if ((base + 64u * k) < hi) {
+            const uint32_t idx = base + 64u * k;
+            if (idx < hi) {
                 const uint32_t d = dig[k];
-                s_k[s_tstart[d] + s_wh[wave][d] + rank[k]] = key[k];
+                const uint32_t d1 = one_d1 ? d1lo : digit_at(s_d1, d1lo, d1hi, idx);
+                // the key's low 32 bits: d above d1 above the item's low bits
+                const unsigned long long L = ((unsigned long long)d << (sh + 8)) |
+                                             ((unsigned long long)d1 << sh) | (item[k] & lowm);
+                s_k[s_tstart[d] + s_wh[wave][d] + rank[k]] = make_uint2((uint32_t)L, d);
             }
         }
         __syncthreads();
         const uint32_t tn = hi - lo;
         for (uint32_t i = tid; i < tn; i += kWG) {
             const uint2 kk = s_k[i];
-            const uint32_t d = KPair::digit(kk, shift, a.bbits, MASK);
-            const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
-            if (g < n) {
-                if constexpr (OUT32)  // the key's low 32 bits; the bits above are d's
-                    reinterpret_cast<uint32_t*>(kout)[g] = (kk.x << a.bbits) | kk.y;
-                else
-                    kout[g] = kk;
-            }
+            const uint32_t g = s_gbase[kk.y] + (i - s_tstart[kk.y]);
+            if (g < n) kout[g] = kk.x;
         }
         __syncthreads();
     }
@@ -579,9 +607,10 @@ __global__ __launch_bounds__(kWG) void k_xpass(const uint2* __restrict__ kin, ui
 // part x = d1 >> 5's base for d2 (as in k_xpass), plus the part's d2 keys
 // that precede d1's first input position b -- the inclusive look-back prefix
 // of the tile before b's tile, and the d2 keys of b's tile before b, counted
-// from the pass's input.  Starts are non-decreasing in (d2, d1) order.
+// from the pass's input (32-bit items, the digit at shift0).  Starts are
+// non-decreasing in (d2, d1) order.
 template <int ITEMS>
-__global__ __launch_bounds__(kWG) void k_bucket_starts(const uint2* __restrict__ kin, SortArgs a,
+__global__ __launch_bounds__(kWG) void k_bucket_starts(const uint32_t* __restrict__ kin, SortArgs a,
                                                        uint32_t* __restrict__ bst) {
     constexpr uint32_t RADIX = 256, MASK = RADIX - 1u, TILE = kWG * ITEMS, NP = kMaxRegions;
     static_assert(RADIX == kWG, "one digit per thread");
@@ -621,9 +650,8 @@ __global__ __launch_bounds__(kWG) void k_bucket_starts(const uint2* __restrict__
                 atomicOr(&a.ctrl->flags, kFlagTimeout);  // the pass did not finish its chain
             prefix = (uint32_t)wv;
         }
-        const int shift = a.shift0 + 8;
         for (uint32_t i = plo + T * TILE + tid; i < b; i += kWG)
-            atomicAdd(&s_cnt[KPair::digit(kin[i], shift, a.bbits, MASK)], 1u);
+            atomicAdd(&s_cnt[(kin[i] >> a.shift0) & MASK], 1u);
         __syncthreads();
         prefix += s_cnt[tid];
     }
@@ -632,10 +660,11 @@ __global__ __launch_bounds__(kWG) void k_bucket_starts(const uint2* __restrict__
 }
 
 hipError_t launch_bucket_starts(const SortArgs& a, const uint2* pass_in, uint32_t* bst, hipStream_t st) {
+    const uint32_t* in = reinterpret_cast<const uint32_t*>(pass_in);
     if (a.items == 8)
-        hipLaunchKernelGGL(k_bucket_starts<8>, dim3(256), dim3(kWG), 0, st, pass_in, a, bst);
+        hipLaunchKernelGGL(k_bucket_starts<8>, dim3(256), dim3(kWG), 0, st, in, a, bst);
     else if (a.items == 16)
-        hipLaunchKernelGGL(k_bucket_starts<16>, dim3(256), dim3(kWG), 0, st, pass_in, a, bst);
+        hipLaunchKernelGGL(k_bucket_starts<16>, dim3(256), dim3(kWG), 0, st, in, a, bst);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -750,9 +779,13 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
         uint32_t* vo = cur ? v0 : v1;
         if constexpr (!KT::kHasVals && BITS == 8) {
             if (p == 0 && a.claim && a.npass > 1) {  // never the final pass: no capacity clamp
-                // a whole number of workgroups per region
+                // a whole number of workgroups per region; 32-bit items for
+                // the per-part pass
                 const uint32_t mg = (grid + a.nregions - 1) / a.nregions * a.nregions;
-                hipLaunchKernelGGL((k_msplit<BITS, ITEMS>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
+                if (a.joint && a.npass == 2)
+                    hipLaunchKernelGGL((k_msplit<BITS, ITEMS, true>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
+                else
+                    hipLaunchKernelGGL((k_msplit<BITS, ITEMS, false>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
@@ -764,10 +797,9 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
             if (p == 1 && a.joint && a.npass == 2) {  // the last pass, per part
                 const uint32_t xg = a.xgrid ? a.xgrid : grid;
                 const uint32_t mg = (xg + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
-                if (a.out32)
-                    hipLaunchKernelGGL((k_xpass<ITEMS, true>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
-                else
-                    hipLaunchKernelGGL((k_xpass<ITEMS, false>), dim3(mg), dim3(kWG), 0, st, ki, ko, a);
+                if (!a.st_out) return hipErrorInvalidValue;
+                hipLaunchKernelGGL((k_xpass<ITEMS>), dim3(mg), dim3(kWG), 0, st,
+                                   reinterpret_cast<const uint32_t*>(ki), reinterpret_cast<uint32_t*>(ko), a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
