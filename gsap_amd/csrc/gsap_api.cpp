@@ -731,10 +731,14 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         // pass's digit, implied by the position)
         const bool out32 = joint != nullptr;
         uint32_t* const dstarts = a.pair_claim + kPairClaimWords;
+        uint32_t* const bst = out32 ? pair_local_bucket_starts((uint32_t*)h->wstart.p, h->slot_cap) : nullptr;
         p.st_out = out32 ? dstarts : nullptr;
-        // the per-part pass runs best at three workgroups per CU (more tiles
-        // in flight per XCD evict the lines neighbouring runs share)
-        p.xgrid = 3 * h->cus;
+        p.bst = bst;
+        p.xcnt = out32 ? bst + kBucketWords : nullptr;
+        p.xcur = out32 ? bst + kBucketWords + kRadixSq : nullptr;
+        // the per-part pass: tiles assigned statically, many workgroups
+        // per CU for the balance
+        p.xgrid = 12 * h->cus;
         p.npass = npass_pair;
         p.lookback = (unsigned long long*)h->lb_pair.p;
         p.ctrl = ctrl;
@@ -754,11 +758,8 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
             // odd number of passes, in t0p after an even one
             uint2* x = (npass_pair & 1u) ? t1p : t0p;
             uint2* y = (npass_pair & 1u) ? t0p : t1p;
-            // after the per-part pass (which read y) its bucket starts give
-            // the local stage's window starts without a search
-            uint32_t* const bst =
-                out32 ? pair_local_bucket_starts((uint32_t*)h->wstart.p, h->slot_cap) : nullptr;
-            if (bst) HIPCHK(h, launch_bucket_starts(p, y, bst, st));
+            // after the per-part pass its bucket starts give the local
+            // stage's window starts without a search
             HIPCHK(h, launch_pair_local(x, y, outp, &ctrl->sort_real, &ctrl->num_pairs, cap,
                                         h->slot_cap, (uint32_t*)h->wstart.p, &ctrl->local_big,
                                         (int)bbits, (int)shift0, out32 ? dstarts : nullptr, bst, st));

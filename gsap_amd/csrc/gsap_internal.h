@@ -339,13 +339,19 @@ struct SortArgs {
     // pair sort of two passes after a per-region first pass (keys of 34 to
     // 40 bits): [kMaxRegions][256] counts of the second digit per part
     // (part = first digit >> 5), taken by the sweep; the second pass then
-    // runs per part, on xgrid workgroups.  Both passes write 32-bit items:
-    // the first the key without its digit, the second the key's low 32
-    // bits (the bits above are its digit's) and the digit starts
-    // st_out[0..256] (the exclusive scan of its counts)
+    // runs per part, on xgrid workgroups, bucket by bucket of equal (second
+    // digit, first digit).  Both passes write 32-bit items: the first the
+    // key without its digit, the second the key's low 32 bits (the bits
+    // above are its digit's).  The second pass leaves the digit starts in
+    // st_out[0..256] and the bucket starts in bst[kBucketWords] ([d2][d1],
+    // then the key count); xcnt (bucket sizes) and xcur (slot counters) are
+    // kRadixSq words each
     uint32_t* joint;
     uint32_t xgrid;
     uint32_t* st_out;
+    uint32_t* bst;
+    uint32_t* xcnt;
+    uint32_t* xcur;
     // every pass returns at once unless *run_if != 0 (null: always runs)
     const uint32_t* run_if;
 };
@@ -372,8 +378,8 @@ uint64_t pair_local_wstart_words(uint64_t cap);  // u32 words of the window-star
 // bits above implied by the position: that pass's digit d occupies
 // [st[d], st[d + 1])), and x has room for 2 cap pairs (the upper half is
 // scratch for oversized windows).
-// bst (with st): bucket starts from launch_bucket_starts, which replace the
-// search for the window starts
+// bst (with st): the second pass's bucket starts (SortArgs::bst), which
+// replace the search for the window starts
 hipError_t launch_pair_local(uint2* x, uint2* y, uint2* out, const unsigned long long* n_dev,
                              const unsigned long long* last_n, unsigned long long last_cap,
                              unsigned long long cap, uint32_t* wstart, uint32_t* big_ctr, int bbits,
@@ -385,11 +391,11 @@ hipError_t launch_local_big_windows(uint2* x, uint2* y, uint2* out, const unsign
                                     const unsigned long long* last_n, unsigned long long last_cap,
                                     const uint32_t* wst, const uint32_t* big, const uint32_t* big_ctr,
                                     int bbits, int L, hipStream_t st);
-constexpr uint32_t kBucketWords = 256u * 256u + 1u;  // bucket starts of a two-pass pair sort
+constexpr uint32_t kRadixSq = 256u * 256u;
+constexpr uint32_t kBucketWords = kRadixSq + 1u;  // bucket starts of a two-pass pair sort
+// In the window-start table: the two-pass sort's bucket starts
+// (kBucketWords), then its bucket sizes and slot counters (kRadixSq each).
 uint32_t* pair_local_bucket_starts(uint32_t* wstart, uint64_t cap);
-// After a per-part second pass (k_xpass) over pass_in: the start of every
-// (second digit, first digit) bucket of its output, [256 * 256 + 1] words.
-hipError_t launch_bucket_starts(const SortArgs& a, const uint2* pass_in, uint32_t* bst, hipStream_t st);
 
 // ---- pair sort by rows (k_pair_rows.hip) ----
 // Counting sort of the pairs in P's regions (ctrl's region table) on the

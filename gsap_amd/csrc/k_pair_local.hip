@@ -619,12 +619,12 @@ __global__ __launch_bounds__(kBigT) void k_pair_local_big(const void* Xv, uint2*
 
 // Window starts (J + 2 words) followed by the list of handed-on windows.
 static uint64_t wstart_half(uint64_t cap) { return cap / kLocW + 2; }
-// window starts, the list of oversized windows, then the bucket starts of a
-// two-pass sort with 32-bit items (kBucketWords)
-uint64_t pair_local_wstart_words(uint64_t cap) { return 2 * wstart_half(cap) + kBucketWords; }
+// window starts, the list of oversized windows, then the bucket starts,
+// sizes and slot counters of a two-pass sort with 32-bit items
+uint64_t pair_local_wstart_words(uint64_t cap) { return 2 * wstart_half(cap) + kBucketWords + 2 * kRadixSq; }
 uint32_t* pair_local_bucket_starts(uint32_t* wstart, uint64_t cap) { return wstart + 2 * wstart_half(cap); }
 
-// Window starts from bucket starts (k_bucket_starts, non-decreasing): window
+// Window starts from bucket starts (k_xstarts, non-decreasing): window
 // j starts at the first bucket start at or after j * kLocW.  Thread k writes
 // the windows j with B[k-1] < j * kLocW <= B[k] (B clamped to the key count
 // n; k = 65536 covers the windows after the last bucket start); window J =

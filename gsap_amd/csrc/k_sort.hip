@@ -58,6 +58,30 @@ __device__ __forceinline__ uint32_t pair_mid32(uint2 k, int shift, int bbits) {
     return (uint32_t)(((L >> (shift + 8)) << shift) | (L & ((1ull << shift) - 1ull)));
 }
 
+// Rank of a valid lane's item among the tile's items of digit d (in claim
+// order; any order within a digit will do), counting them in s_cnt: a run of
+// equal digits in neighbouring lanes claims its ranks with one LDS atomic by
+// its first lane, the others take theirs from it, so the runs of equal keys
+// the sweep writes do not serialise on one LDS address.  Valid lanes are a
+// prefix of the wave; the whole wave calls it.
+__device__ __forceinline__ uint32_t run_rank(uint32_t d, bool valid, uint32_t* s_cnt) {
+    const uint32_t lane = lane_id();
+    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+    const bool head = valid && (lane == 0u || prev != d);
+    const unsigned long long hm = ballot(head);
+    const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+    const unsigned long long le = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= mine
+    uint32_t base = 0u;
+    if (head) {
+        const unsigned long long nxt = hm & ~le;
+        const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+        base = atomicAdd(&s_cnt[d], end - lane);
+    }
+    const unsigned long long mine = hm & le;  // my run's head: the last head at or below me
+    const uint32_t hl = mine ? 63u - (uint32_t)__builtin_clzll(mine) : 0u;
+    return (uint32_t)__shfl((int)base, (int)hl, 64) + (lane - hl);
+}
+
 // The digit d whose range [st[d], st[d + 1]) of a pass's output holds
 // position p (st: nondecreasing, st[0] = 0 <= p; the last such d).
 __device__ __forceinline__ uint32_t digit_at(const uint32_t* st, uint32_t lo, uint32_t hi, uint32_t p) {
@@ -281,9 +305,10 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
 // that two neighbouring runs share before writing them back.  Lines written
 // in parts from two L2s cost HBM about twice their bytes.  Items are staged
 // in LDS in digit order (LDS-atomic ranks) for a coalesced scatter, never
-// past the pair count.  MID32 (followed by k_xpass, keys of at most 40 bits):
-// the output position implies the digit, so each key is written as the 32
-// bits left without it (pair_mid32).
+// past the pair count.  MID32 (followed by k_xcount / k_xsplit, keys of at
+// most 40 bits): the output position implies the digit, so each key is
+// written as the 32 bits left without it (pair_mid32), and the bucket sizes
+// of the second pass are zeroed here.
 template <int BITS, int ITEMS, bool MID32>
 __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, uint2* __restrict__ kout,
                                                 SortArgs a) {
@@ -297,6 +322,8 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0;
+    if constexpr (MID32)
+        for (uint32_t i = blockIdx.x * kWG + tid; i < kRadixSq; i += gridDim.x * kWG) a.xcnt[i] = 0u;
     unsigned long long nd = *a.n_dev;
     const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
     unsigned long long nr = *a.n_dev_rest;
@@ -383,291 +410,207 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
     }
 }
 
-// Second (last) onesweep pass of a two-pass pair sort after k_msplit, in
-// kMaxRegions parts: part x holds the keys whose first digit d1 has
-// d1 >> 5 == x, a contiguous range of the first pass's output (which is
-// ordered by d1).  Digit d's output range is laid out part after part: part
-// x's keys of digit d start at start(d) + sum over x' < x of J[x'][d] (J:
-// the per-part digit counts the sweep took as it wrote the pairs), so the
-// result is ordered by (d, d1) as after one stable pass, while each part
-// runs its own look-back chain on the workgroups of one XCD (w mod
-// kMaxRegions == x, the grid a multiple of kMaxRegions).  A digit's
-// neighbouring runs then lie in one L2, which merges the lines they share
-// before writing them back.  Within a part: tiles claimed through the
-// part's counter, ranking (below), look-back to the part's first tile
-// (inclusive), LDS-staged coalesced scatter.
-// Keys have at most 40 bits and both passes write 32-bit items: the input
-// item is the key without d1 (pair_mid32; d1 is the first pass's digit range
-// that holds the position), the output item the key's low 32 bits (the bits
-// above lie in d, implied by the position; the digit starts go to st_out).
-template <int ITEMS>
-__global__ __launch_bounds__(kWG) void k_xpass(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
-                                               SortArgs a) {
-    constexpr uint32_t BITS = 8, RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
-    constexpr uint32_t NP = kMaxRegions, DPP = RADIX / NP;  // first digits per part
-    static_assert(RADIX == kWG, "one digit per thread");
-    __shared__ uint32_t s_wh[kWaves][RADIX];
-    __shared__ uint32_t s_tstart[RADIX];
-    __shared__ uint32_t s_gbase[RADIX];
-    __shared__ uint32_t s_tmp[2 * kWaves];
-    __shared__ uint32_t s_cnt[RADIX];
-    __shared__ uint2 s_k[TILE];          // (output item, digit)
-    __shared__ uint32_t s_d1[RADIX + 1]; // the first pass's digit starts
-    __shared__ uint32_t s_t0[NP];        // each part's first look-back slot
-    __shared__ uint32_t s_tile;
+// Two-pass pair sort, second pass without look-back.  The local stage
+// orders the pairs inside each bucket of equal (d2, d1) -- d2 this pass's
+// digit, d1 the first pass's -- so the pass need only put every key into its
+// bucket, in any order within it:
+//  * k_xcount: bucket sizes, from the first pass's 32-bit items (an item's
+//    d1 is the first pass's digit range that holds its position): per chunk
+//    of positions and d1, LDS digit counts, then one atomic per non-empty
+//    bucket;
+//  * k_xstarts: every bucket's start -- d2's start (the scan of the per-part
+//    digit counts the sweep took) plus the sizes of the buckets (d2, d1')
+//    with d1' < d1 -- for the local stage, and a copy as slot counters;
+//  * k_xsplit: per tile and d1, LDS digit counts, one slot claim per
+//    non-empty bucket, LDS-staged coalesced scatter of each key's low 32 bits.
+// Part x (first digits 32 x to 32 x + 31, a contiguous range of the first
+// pass's output) runs on the workgroups of XCD x (w mod kMaxRegions == x, the
+// grid a multiple of kMaxRegions), so a bucket's counters and its
+// neighbouring runs stay in one L2.  Tables are [d2][d1] (bst[65536] = the
+// key count); k_msplit zeroes the sizes.
+namespace {
+constexpr uint32_t kXRadix = 256u, kXParts = kMaxRegions, kXDpp = kXRadix / kXParts;
 
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const int sh = a.shift0;  // an input item's digit d: item >> sh
-    const uint32_t lowm = (1u << sh) - 1u;
-    unsigned long long nd = *a.n_dev_rest;
-    const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
-    const uint32_t x = blockIdx.x % NP;
-    const uint32_t ctr_idx = (uint32_t)a.ctr_base + x;
-    const uint32_t tag = ((a.ctrl->epoch << 4) | ((uint32_t)a.ctr_base + 1u)) & 0x7FFFFFFFu;
-    const unsigned long long ltm = lanemask_lt();
-    const unsigned long long gtm = ~ltm & ~(1ull << lane);
-    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
-    // the first pass's digit starts; part x spans those of d1 = DPP x
+// the first pass's digit starts (clamped to n) into s_d1[0..256]
+__device__ __forceinline__ void x_digit_starts(const SortArgs& a, uint32_t n, uint32_t* s_d1,
+                                               uint32_t* s_tmp) {
     uint32_t tot;
-    const uint32_t d1start = block_excl_scan(a.hist[tid], s_tmp, &tot);
-    s_d1[tid] = min(d1start, n);
-    if (tid == 0) s_d1[RADIX] = min(tot, n);
-    // digit tid's base for this part: its start over all parts plus its
-    // counts in the parts before this one
-    uint32_t all = 0, pre = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < NP; ++q) {
-        const uint32_t c = a.joint[q * RADIX + tid];
-        all += c;
-        pre += q < x ? c : 0u;
-    }
-    const uint32_t dstart = block_excl_scan(all, s_tmp, &tot);
-    const uint32_t dbase = dstart + pre;
-    if (blockIdx.x == 0) {  // the digit starts, for the readers of the output items
-        a.st_out[tid] = dstart;
-        if (tid == 0) a.st_out[RADIX] = tot;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t t = 0;
-        for (uint32_t q = 0; q < NP; ++q) {
-            s_t0[q] = t;
-            t += (s_d1[(q + 1) * DPP] - s_d1[q * DPP] + TILE - 1) / TILE;
-        }
-    }
-    s_cnt[tid] = 0u;
-    const uint32_t plo = s_d1[x * DPP], phi = s_d1[(x + 1) * DPP];
+    const uint32_t d1start = block_excl_scan(a.hist[threadIdx.x], s_tmp, &tot);
+    s_d1[threadIdx.x] = min(d1start, n);
+    if (threadIdx.x == 0) s_d1[kXRadix] = min(tot, n);
+}
 
-    for (;;) {
-        if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_ctr[ctr_idx], 1u);
-        for (uint32_t i = tid; i < kWaves * RADIX; i += kWG) (&s_wh[0][0])[i] = 0u;
-        __syncthreads();
-        const uint32_t ti = s_tile;
-        const unsigned long long lo64 = (unsigned long long)plo + (unsigned long long)ti * TILE;
-        if (lo64 >= phi) break;  // uniform
-        const uint32_t lo = (uint32_t)lo64, hi = min(lo + TILE, phi);
-        const bool first = ti == 0u;
-        const uint32_t gtile = s_t0[x] + ti;
-
-        uint32_t item[ITEMS], rank[ITEMS], dig[ITEMS];
-        const uint32_t base = lo + wave * 64u * ITEMS + lane;
+// Digit counts of items [sa, sb) (at most kWG * ITEMS, a wave's rows of 64
+// consecutive items) into s_cnt; a run of equal digits in neighbouring lanes
+// is one LDS atomic.  dig[k]: the digits (0 past sb).
+template <int ITEMS>
+__device__ __forceinline__ void x_count_rows(const uint32_t* __restrict__ kin, uint32_t sa, uint32_t sb,
+                                             int sh, uint32_t* s_cnt, uint32_t item[ITEMS],
+                                             uint32_t dig[ITEMS]) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
+    const uint32_t base = sa + wave * 64u * ITEMS + lane;
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t idx = base + 64u * k;
-            if (idx < hi) item[k] = kin[idx];
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t idx = base + 64u * k;
+        item[k] = idx < sb ? kin[idx] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const bool valid = (base + 64u * k) < sb;
+        const uint32_t d = valid ? (item[k] >> sh) & (kXRadix - 1u) : 0u;
+        dig[k] = d;
+        const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+        const bool head = valid && (lane == 0u || prev != d);
+        const unsigned long long hm = ballot(head);
+        const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+        if (head) {
+            const unsigned long long nxt = hm & above;
+            const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+            atomicAdd(&s_cnt[d], end - lane);
         }
-        // tile digit counts, published at once; a run of equal digits in
-        // neighbouring lanes is one LDS atomic
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const bool valid = (base + 64u * k) < hi;
-            const uint32_t d = valid ? (item[k] >> sh) & MASK : 0u;
-            dig[k] = d;
-            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
-            const bool head = valid && (lane == 0u || prev != d);
-            const unsigned long long hm = ballot(head);
-            const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
-            if (head) {
-                const unsigned long long nxt = hm & above;
-                const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
-                atomicAdd(&s_cnt[d], end - lane);
-            }
-        }
-        __syncthreads();
-        const uint32_t tcount = s_cnt[tid];
-        s_cnt[tid] = 0u;
-        st_rel(a.lookback + (size_t)gtile * RADIX + tid, lb_word(tag, first, tcount));
-        // The order within a digit need only follow the first digit d1 (keys
-        // of equal d1 and d are ordered by the local stage).  The input is
-        // ordered by d1, so a tile whose first and last keys share d1 holds
-        // one d1 value: its items take their places in any order (one LDS
-        // atomic each); a tile across a d1 boundary ranks them stably
-        // (items in tile order (wave, item, lane), peers by BITS ballots).
-        const uint32_t d1lo = digit_at(s_d1, x * DPP, x * DPP + DPP - 1u, lo);
-        const uint32_t d1hi = digit_at(s_d1, d1lo, x * DPP + DPP - 1u, hi - 1u);
-        const bool one_d1 = d1lo == d1hi;  // uniform
-        __syncthreads();  // s_cnt is zero
-        if (one_d1) {
-#pragma unroll
-            for (int k = 0; k < ITEMS; ++k)
-                rank[k] = (base + 64u * k) < hi ? atomicAdd(&s_cnt[dig[k]], 1u) : 0u;
-        } else {
-#pragma unroll
-            for (int k = 0; k < ITEMS; ++k) {
-                const bool valid = (base + 64u * k) < hi;
-                const uint32_t d = dig[k];
-                unsigned long long m = ballot(valid);
-#pragma unroll
-                for (uint32_t b = 0; b < BITS; ++b) {
-                    const bool bit = (d >> b) & 1u;
-                    const unsigned long long bb = ballot(bit);
-                    m &= bit ? bb : ~bb;
-                }
-                const uint32_t before = s_wh[wave][d];
-                rank[k] = before + mbcnt(m);
-                if (valid && (m & gtm) == 0ull) s_wh[wave][d] = before + (uint32_t)__popcll(m);
-            }
-        }
-        __syncthreads();
-        if (one_d1) s_cnt[tid] = 0u;  // the atomic ranks leave the tile counts
-        uint32_t excl = 0;
-        {
-            uint32_t run = 0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = s_wh[w][tid];
-                s_wh[w][tid] = run;
-                run += c;
-            }
-            if (!first) {
-                // predecessors four at a time, down to the nearest inclusive
-                // prefix (the part's first tile is one)
-                constexpr int WIN = 4;
-                int p = (int)gtile - 1;
-                uint32_t spins = 0;
-                for (;;) {
-                    unsigned long long v[WIN];
-#pragma unroll
-                    for (int k = 0; k < WIN; ++k)
-                        v[k] = p - k >= 0 ? ld_acq(a.lookback + (size_t)(p - k) * RADIX + tid) : 0ull;
-                    uint32_t add = 0;
-                    int used = 0;
-                    bool done = false;
-#pragma unroll
-                    for (int k = 0; k < WIN; ++k) {
-                        if (done || used != k || (uint32_t)(v[k] >> 33) != tag) continue;
-                        add += (uint32_t)v[k];
-                        used = k + 1;
-                        done = (v[k] >> 32) & 1ull;
-                    }
-                    excl += add;
-                    if (done) break;
-                    if (used == 0) {
-                        if (++spins > (1u << 22)) {  // bounded wait (see k_onesweep)
-                            atomicOr(&a.ctrl->flags, kFlagTimeout);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                    p -= used;
-                }
-                st_rel(a.lookback + (size_t)gtile * RADIX + tid, lb_word(tag, true, excl + tcount));
-            }
-        }
-        const uint32_t tex = block_excl_scan(tcount, s_tmp + kWaves, &tot);
-        s_gbase[tid] = dbase + excl;
-        s_tstart[tid] = tex;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t idx = base + 64u * k;
-            if (idx < hi) {
-                const uint32_t d = dig[k];
-                const uint32_t d1 = one_d1 ? d1lo : digit_at(s_d1, d1lo, d1hi, idx);
-                // the key's low 32 bits: d above d1 above the item's low bits
-                const unsigned long long L = ((unsigned long long)d << (sh + 8)) |
-                                             ((unsigned long long)d1 << sh) | (item[k] & lowm);
-                s_k[s_tstart[d] + s_wh[wave][d] + rank[k]] = make_uint2((uint32_t)L, d);
-            }
-        }
-        __syncthreads();
-        const uint32_t tn = hi - lo;
-        for (uint32_t i = tid; i < tn; i += kWG) {
-            const uint2 kk = s_k[i];
-            const uint32_t g = s_gbase[kk.y] + (i - s_tstart[kk.y]);
-            if (g < n) kout[g] = kk.x;
-        }
-        __syncthreads();
     }
 }
 
-// Bucket starts after k_xpass: bucket (d2, d1) -- the keys of second digit
-// d2 and first digit d1 -- starts at bst[d2 * 256 + d1] of the pass's output
-// (bst[65536] = the key count).  One workgroup per d1, one thread per d2:
-// part x = d1 >> 5's base for d2 (as in k_xpass), plus the part's d2 keys
-// that precede d1's first input position b -- the inclusive look-back prefix
-// of the tile before b's tile, and the d2 keys of b's tile before b, counted
-// from the pass's input (32-bit items, the digit at shift0).  Starts are
-// non-decreasing in (d2, d1) order.
+}  // namespace
+
 template <int ITEMS>
-__global__ __launch_bounds__(kWG) void k_bucket_starts(const uint32_t* __restrict__ kin, SortArgs a,
-                                                       uint32_t* __restrict__ bst) {
-    constexpr uint32_t RADIX = 256, MASK = RADIX - 1u, TILE = kWG * ITEMS, NP = kMaxRegions;
-    static_assert(RADIX == kWG, "one digit per thread");
+__global__ __launch_bounds__(kWG) void k_xcount(const uint32_t* __restrict__ kin, SortArgs a) {
+    constexpr uint32_t TILE = kWG * ITEMS, CHUNKS = 4;  // CHUNKS tiles per flush
+    static_assert(kXRadix == kWG, "one digit per thread");
+    __shared__ uint32_t s_d1[kXRadix + 1];
     __shared__ uint32_t s_tmp[kWaves];
-    __shared__ uint32_t s_q[NP + 1];
-    __shared__ uint32_t s_b;
-    __shared__ uint32_t s_cnt[RADIX];
-    const uint32_t d1 = blockIdx.x, tid = threadIdx.x;
+    __shared__ uint32_t s_cnt[kXRadix];
+    const uint32_t tid = threadIdx.x;
     const unsigned long long nd = *a.n_dev_rest;
     const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
-    const uint32_t tag = ((a.ctrl->epoch << 4) | ((uint32_t)a.ctr_base + 1u)) & 0x7FFFFFFFu;
-    uint32_t tot;
-    const uint32_t d1start = block_excl_scan(a.hist[tid], s_tmp, &tot);
-    if ((tid & 31u) == 0u) s_q[tid >> 5] = min(d1start, n);
-    if (tid == 0) s_q[NP] = min(tot, n);
-    if (tid == d1) s_b = min(d1start, n);
-    const uint32_t x = d1 >> 5;
-    uint32_t all = 0, pre = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < NP; ++q) {
-        const uint32_t c = a.joint[q * RADIX + tid];
-        all += c;
-        pre += q < x ? c : 0u;
-    }
-    const uint32_t dstart = block_excl_scan(all, s_tmp, &tot);
+    x_digit_starts(a, n, s_d1, s_tmp);
     s_cnt[tid] = 0u;
     __syncthreads();
-    const uint32_t plo = s_q[x], b = s_b;
-    uint32_t t0 = 0;  // the part's first look-back slot
-    for (uint32_t q = 0; q < x; ++q) t0 += (s_q[q + 1] - s_q[q] + TILE - 1) / TILE;
-    uint32_t prefix = 0;
-    if (b > plo) {  // uniform
-        const uint32_t T = (b - plo) / TILE;
-        if (T > 0) {
-            const unsigned long long wv = ld_acq(a.lookback + (size_t)(t0 + T - 1u) * RADIX + tid);
-            if ((uint32_t)(wv >> 33) != tag || !((wv >> 32) & 1ull))
-                atomicOr(&a.ctrl->flags, kFlagTimeout);  // the pass did not finish its chain
-            prefix = (uint32_t)wv;
+    const uint32_t x = blockIdx.x % kXParts, gstep = gridDim.x / kXParts;
+    const uint32_t plo = s_d1[x * kXDpp], phi = s_d1[(x + 1u) * kXDpp];
+    const uint32_t dlast = x * kXDpp + kXDpp - 1u;
+    for (unsigned long long lo64 = plo + (unsigned long long)(blockIdx.x / kXParts) * TILE * CHUNKS;
+         lo64 < phi; lo64 += (unsigned long long)gstep * TILE * CHUNKS) {
+        const uint32_t lo = (uint32_t)lo64;
+        const uint32_t hi = (uint32_t)min((unsigned long long)phi, lo64 + TILE * CHUNKS);
+        const uint32_t d1lo = digit_at(s_d1, x * kXDpp, dlast, lo);
+        const uint32_t d1hi = digit_at(s_d1, d1lo, dlast, hi - 1u);
+        for (uint32_t d1 = d1lo; d1 <= d1hi; ++d1) {  // uniform
+            const uint32_t sa = max(lo, s_d1[d1]), sb = min(hi, s_d1[d1 + 1u]);
+            if (sa >= sb) continue;
+            for (uint32_t c0 = sa; c0 < sb; c0 += TILE) {
+                uint32_t item[ITEMS], dig[ITEMS];
+                x_count_rows<ITEMS>(kin, c0, min(sb, c0 + TILE), a.shift0, s_cnt, item, dig);
+            }
+            __syncthreads();
+            const uint32_t c = s_cnt[tid];
+            s_cnt[tid] = 0u;
+            if (c) atomicAdd(&a.xcnt[tid * kXRadix + d1], c);
+            __syncthreads();
         }
-        for (uint32_t i = plo + T * TILE + tid; i < b; i += kWG)
-            atomicAdd(&s_cnt[(kin[i] >> a.shift0) & MASK], 1u);
-        __syncthreads();
-        prefix += s_cnt[tid];
     }
-    bst[tid * RADIX + d1] = dstart + pre + prefix;
-    if (d1 == 0 && tid == 0) bst[RADIX * RADIX] = n;
 }
 
-hipError_t launch_bucket_starts(const SortArgs& a, const uint2* pass_in, uint32_t* bst, hipStream_t st) {
-    const uint32_t* in = reinterpret_cast<const uint32_t*>(pass_in);
-    if (a.items == 8)
-        hipLaunchKernelGGL(k_bucket_starts<8>, dim3(256), dim3(kWG), 0, st, in, a, bst);
-    else if (a.items == 16)
-        hipLaunchKernelGGL(k_bucket_starts<16>, dim3(256), dim3(kWG), 0, st, in, a, bst);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+__global__ __launch_bounds__(kWG) void k_xstarts(SortArgs a) {
+    static_assert(kXRadix == kWG, "one bucket per thread");
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_ds[kXRadix];
+    const uint32_t tid = threadIdx.x, d2 = blockIdx.x;
+    const unsigned long long nd = *a.n_dev_rest;
+    const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
+    uint32_t all = 0, tot;
+#pragma unroll
+    for (uint32_t q = 0; q < kXParts; ++q) all += a.joint[q * kXRadix + tid];
+    const uint32_t ds = block_excl_scan(all, s_tmp, &tot);
+    s_ds[tid] = ds;
+    if (d2 == 0) {  // the digit starts, for the readers of the output items
+        a.st_out[tid] = ds;
+        if (tid == 0) {
+            a.st_out[kXRadix] = tot;
+            a.bst[kXRadix * kXRadix] = n;
+        }
+    }
+    __syncthreads();
+    const uint32_t c = a.xcnt[d2 * kXRadix + tid];  // bucket (d2, d1 = tid)
+    const uint32_t e = block_excl_scan(c, s_tmp, &tot);
+    const uint32_t b = s_ds[d2] + e;
+    a.bst[d2 * kXRadix + tid] = b;
+    a.xcur[d2 * kXRadix + tid] = b;
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(kWG) void k_xsplit(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
+                                                SortArgs a) {
+    constexpr uint32_t TILE = kWG * ITEMS;
+    static_assert(kXRadix == kWG, "one digit per thread");
+    __shared__ uint32_t s_d1[kXRadix + 1];
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_cnt[kXRadix];
+    __shared__ uint2 s_tg[kXRadix];  // per digit: tile start, first claimed slot
+    __shared__ uint2 s_kp[TILE];     // (output item, position), in digit order
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const int sh = a.shift0;
+    const uint32_t lowm = (1u << sh) - 1u;
+    const unsigned long long nd = *a.n_dev_rest;
+    const uint32_t n = (uint32_t)(nd > a.cap ? a.cap : nd);
+    x_digit_starts(a, n, s_d1, s_tmp);
+    s_cnt[tid] = 0u;
+    __syncthreads();
+    const uint32_t x = blockIdx.x % kXParts, gstep = gridDim.x / kXParts;
+    const uint32_t plo = s_d1[x * kXDpp], phi = s_d1[(x + 1u) * kXDpp];
+    const uint32_t dlast = x * kXDpp + kXDpp - 1u;
+    for (unsigned long long lo64 = plo + (unsigned long long)(blockIdx.x / kXParts) * TILE; lo64 < phi;
+         lo64 += (unsigned long long)gstep * TILE) {
+        const uint32_t lo = (uint32_t)lo64;
+        const uint32_t hi = (uint32_t)min((unsigned long long)phi, lo64 + TILE);
+        const uint32_t d1lo = digit_at(s_d1, x * kXDpp, dlast, lo);
+        const uint32_t d1hi = digit_at(s_d1, d1lo, dlast, hi - 1u);
+        for (uint32_t d1 = d1lo; d1 <= d1hi; ++d1) {  // uniform
+            const uint32_t sa = max(lo, s_d1[d1]), sb = min(hi, s_d1[d1 + 1u]);
+            if (sa >= sb) continue;
+            // item k's digit (bits 16..23) and its rank (below; < TILE)
+            uint32_t item[ITEMS], dr[ITEMS];
+            const uint32_t base = sa + wave * 64u * ITEMS + lane;
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const uint32_t idx = base + 64u * k;
+                item[k] = idx < sb ? kin[idx] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const uint32_t d = (item[k] >> sh) & (kXRadix - 1u);
+                dr[k] = (d << 16) | run_rank(d, (base + 64u * k) < sb, s_cnt);
+            }
+            __syncthreads();
+            const uint32_t c = s_cnt[tid];
+            uint32_t g = 0u, tot;
+            if (c) g = atomicAdd(&a.xcur[tid * kXRadix + d1], c);
+            const uint32_t ts = block_excl_scan(c, s_tmp, &tot);
+            s_tg[tid] = make_uint2(ts, g);
+            __syncthreads();
+            // staged in digit order with their final positions
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                if ((base + 64u * k) < sb) {
+                    const uint32_t d = dr[k] >> 16, rk = dr[k] & 0xFFFFu;
+                    const uint2 tg = s_tg[d];
+                    // the key's low 32 bits: d above d1 above the item's low bits
+                    const unsigned long long L = ((unsigned long long)d << (sh + 8)) |
+                                                 ((unsigned long long)d1 << sh) | (item[k] & lowm);
+                    s_kp[tg.x + rk] = make_uint2((uint32_t)L, tg.y + rk);
+                }
+            }
+            __syncthreads();
+            const uint32_t tn = sb - sa;
+            for (uint32_t i = tid; i < tn; i += kWG) {
+                const uint2 kp = s_kp[i];
+                if (kp.y < n) kout[kp.y] = kp.x;
+            }
+            s_cnt[tid] = 0u;
+            __syncthreads();
+        }
+    }
 }
 
 // Histogram of every digit position (plain sorts and the pair sort).  Lanes
@@ -797,9 +740,12 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
             if (p == 1 && a.joint && a.npass == 2) {  // the last pass, per part
                 const uint32_t xg = a.xgrid ? a.xgrid : grid;
                 const uint32_t mg = (xg + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
-                if (!a.st_out) return hipErrorInvalidValue;
-                hipLaunchKernelGGL((k_xpass<ITEMS>), dim3(mg), dim3(kWG), 0, st,
-                                   reinterpret_cast<const uint32_t*>(ki), reinterpret_cast<uint32_t*>(ko), a);
+                if (!a.st_out || !a.bst || !a.xcnt || !a.xcur) return hipErrorInvalidValue;
+                const uint32_t* xi = reinterpret_cast<const uint32_t*>(ki);
+                hipLaunchKernelGGL((k_xcount<ITEMS>), dim3(mg), dim3(kWG), 0, st, xi, a);
+                hipLaunchKernelGGL(k_xstarts, dim3(kXRadix), dim3(kWG), 0, st, a);
+                hipLaunchKernelGGL((k_xsplit<ITEMS>), dim3(mg), dim3(kWG), 0, st, xi,
+                                   reinterpret_cast<uint32_t*>(ko), a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
