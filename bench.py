@@ -578,10 +578,10 @@ def main():
     pass_bytes = [12.0 * P if x32 else alg_pass, 8.0 * P if x32 else alg_pass]
     alg_local = 12.0 * P if x32 else alg_pass
     # the dominant kernel: the longest single launch (pair-sort passes are
-    # separate kernels: k_msplit and k_xpass after the sweep's regions,
-    # k_onesweep otherwise)
-    if tn["pair_local"] and tn["pair_regions"] == 8 and len(pass_ms) == 2:
-        pass_names = ["k_msplit", "k_xpass"]
+    # separate kernels: k_msplit, then k_xcount + k_xstarts + k_xsplit after
+    # the sweep's regions; k_onesweep otherwise)
+    if tn["pair_local"] == 1 and tn["pair_regions"] == 8 and len(pass_ms) == 2:
+        pass_names = ["k_msplit", "k_xcount+k_xsplit"]
     else:
         pass_names = ["k_onesweep<pair>"] * len(pass_ms)
     cands = [("k_sweep", stages["sweep"], alg_sweep)]

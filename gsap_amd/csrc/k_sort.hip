@@ -742,7 +742,7 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
                 const uint32_t mg = (xg + kMaxRegions - 1) / kMaxRegions * kMaxRegions;
                 if (!a.st_out || !a.bst || !a.xcnt || !a.xcur) return hipErrorInvalidValue;
                 const uint32_t* xi = reinterpret_cast<const uint32_t*>(ki);
-                hipLaunchKernelGGL((k_xcount<ITEMS>), dim3(mg), dim3(kWG), 0, st, xi, a);
+                hipLaunchKernelGGL((k_xcount<2 * ITEMS>), dim3(mg), dim3(kWG), 0, st, xi, a);
                 hipLaunchKernelGGL(k_xstarts, dim3(kXRadix), dim3(kWG), 0, st, a);
                 hipLaunchKernelGGL((k_xsplit<ITEMS>), dim3(mg), dim3(kWG), 0, st, xi,
                                    reinterpret_cast<uint32_t*>(ko), a);
