@@ -7,7 +7,7 @@ passes), bucket shapes (one bucket larger than a workgroup's LDS sort, windows
 with more buckets than a 32-bit LDS key can rank), output kinds, batched
 scenes, pair regions with padding, a capacity overflow, and the two-pass sort
 whose second pass runs in per-XCD parts (34- to 40-bit keys, all keys in one
-part)."""
+part, sparse lists whose tiles span many first-digit ranges, no pairs)."""
 import numpy as np
 import pytest
 
@@ -228,3 +228,30 @@ def test_local_two_pass_one_part(dev):
         s.set_tune(pair_local=1, pair_regions=8, slabs=slabs)
         got, _, _ = s.collide(to_dev(b, dev))
         assert np.array_equal(pairs_np(got), expect), slabs
+
+
+@pytest.mark.parametrize("n", [300_000, 1_000_000])  # K = 38 and 40
+def test_local_two_pass_sparse_and_empty(dev, n):
+    # Boxes on a grid overlap nothing; then box i + 1 is moved onto box i for
+    # a few thousand even i spread over the whole index range: pairs (i, i+1)
+    # only, a few per first-digit range, so one tile of the second pass
+    # spans many first-digit ranges (and some ranges are empty).  The grid
+    # alone gives no pairs at all.
+    import torch
+    g = np.arange(n, dtype=np.float64)
+    lo = np.stack([(g % 100) * 10.0, ((g // 100) % 100) * 10.0, (g // 10000) * 10.0], 1)
+    b = np.concatenate([lo, lo + 1.0], 1).astype(np.float32)
+    for slabs in (0, 1):
+        s = G.Sap()
+        s.set_tune(pair_local=1, pair_regions=8, slabs=slabs)
+        got, _, _ = s.collide(to_dev(b, dev))
+        assert got.shape[0] == 0, slabs
+    i = np.sort(np.random.default_rng(n).choice(n // 2 - 1, 3000, replace=False)) * 2
+    b[i + 1] = b[i] + np.float32(0.5)
+    expect = np.stack([i, i + 1], 1).astype(np.uint32)
+    for slabs in (0, 1):
+        for items in (8, 16):
+            s = G.Sap()
+            s.set_tune(pair_local=1, pair_regions=8, slabs=slabs, pair_items=items)
+            got, _, _ = s.collide(to_dev(b, dev))
+            assert np.array_equal(pairs_np(got), expect), (slabs, items)
