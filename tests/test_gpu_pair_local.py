@@ -255,3 +255,28 @@ def test_local_two_pass_sparse_and_empty(dev, n):
             s.set_tune(pair_local=1, pair_regions=8, slabs=slabs, pair_items=items)
             got, _, _ = s.collide(to_dev(b, dev))
             assert np.array_equal(pairs_np(got), expect), (slabs, items)
+
+
+def test_local_two_pass_random_sizes_and_tunes(dev):
+    # Seeded draws over the two-pass range (17 to 20 index bits): scene kind,
+    # size, tile width, grid and sweep kind at random; the canonical list must
+    # equal the whole-key sort's every time.
+    import torch
+    r = np.random.default_rng(2024)
+    for case in range(8):
+        n = int(r.integers(70_000, 1_048_577))
+        kind = ("uniform", "cluster", "aniso")[case % 3]
+        if kind == "aniso":
+            b = W.aniso(n, seed=int(r.integers(1 << 30)), sigma=(20.0, 5.0, 1.0))
+        else:
+            b = W.scene(n, np.random.default_rng(int(r.integers(1 << 30))), kind)
+        bd = to_dev(b, dev)
+        s0 = G.Sap()
+        s0.set_tune(pair_local=0)
+        ref, _, _ = s0.collide(bd)
+        tune = dict(pair_local=1, pair_regions=8, slabs=int(r.integers(2)),
+                    pair_items=int(r.choice([8, 16])), pair_grid=int(r.choice([512, 1024, 2048, 4096])))
+        s = G.Sap()
+        s.set_tune(**tune)
+        got, _, _ = s.collide(bd)
+        assert got.shape == ref.shape and torch.equal(got, ref), (case, n, kind, tune)
