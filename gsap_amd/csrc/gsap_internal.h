@@ -354,7 +354,14 @@ struct SortArgs {
     uint32_t* xcur;
     // every pass returns at once unless *run_if != 0 (null: always runs)
     const uint32_t* run_if;
+    // key + value sorts of one scene (host count n): reduce-then-scan passes
+    // (count, scan, scatter; the look-back buffer holds the tile tables)
+    // instead of decoupled look-back
+    int rts;
 };
+// Reduce-then-scan object sort passes from this many sort tiles (measured
+// crossover with the look-back passes on one MI355X).
+constexpr uint32_t kRtsMinTiles = 128;
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
 hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,

@@ -291,6 +291,166 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     }
 }
 
+// Object sort pass by reduce-then-scan (one scene, key + value): no
+// look-back chain.  With a few thousand tiles a decoupled look-back pass is
+// the latency of its chain -- each tile reads its predecessors' words
+// through the device-coherent level, four per round trip -- while the tiles'
+// own work takes a fraction of that.  Here:
+//  * k_rts_count: per tile digit counts -> tab[d * nt + tile];
+//  * k_rts_scan: one workgroup per digit d: d's start (the scan of the
+//    pass's histogram) plus the counts of the tiles before t ->
+//    tab[(RADIX + d) * nt + t];
+//  * k_rts_scatter: stable ranking within the tile (BITS ballots, items in
+//    tile order), LDS staging in digit order, coalesced scatter at the
+//    tile's offsets.
+// tab: 2 RADIX nt words (the look-back buffer of the object sort).
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_rts_count(const uint32_t* __restrict__ kin, SortArgs a, int pass,
+                                                   uint32_t* __restrict__ tab) {
+    constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    __shared__ uint32_t s_cnt[RADIX];
+    if (a.run_if && *a.run_if == 0u) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const int shift = a.shift0 + pass * BITS;
+    const uint32_t n = a.n, nt = (n + TILE - 1) / TILE, tile = blockIdx.x;
+    const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
+    for (uint32_t i = tid; i < RADIX; i += kWG) s_cnt[i] = 0u;
+    __syncthreads();
+    const uint32_t lo = tile * TILE, hi = min(lo + TILE, n);
+    const uint32_t base = lo + wave * 64u * ITEMS + lane;
+    uint32_t key[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t idx = base + 64u * k;
+        key[k] = idx < hi ? kin[idx] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const bool valid = (base + 64u * k) < hi;
+        const uint32_t d = valid ? (key[k] >> shift) & MASK : 0u;
+        const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
+        const bool head = valid && (lane == 0u || prev != d);
+        const unsigned long long hm = ballot(head);
+        const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+        if (head) {
+            const unsigned long long nxt = hm & above;
+            const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : nvalid;
+            atomicAdd(&s_cnt[d], end - lane);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < RADIX; i += kWG) tab[i * nt + tile] = s_cnt[i];
+}
+
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_rts_scan(SortArgs a, int pass, uint32_t* __restrict__ tab) {
+    constexpr uint32_t RADIX = 1u << BITS, TILE = kWG * ITEMS;
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_b;
+    if (a.run_if && *a.run_if == 0u) return;
+    const uint32_t tid = threadIdx.x, d = blockIdx.x;
+    const uint32_t nt = (a.n + TILE - 1) / TILE;
+    uint32_t tot;
+    const uint32_t hv = tid < RADIX ? a.hist[(uint32_t)pass * RADIX + tid] : 0u;
+    const uint32_t gex = block_excl_scan(hv, s_tmp, &tot);
+    if (tid == d) s_b = gex;
+    __syncthreads();
+    uint32_t run = s_b;
+    const uint32_t* cnt = tab + (size_t)d * nt;
+    uint32_t* off = tab + (size_t)(RADIX + d) * nt;
+    for (uint32_t t0 = 0; t0 < nt; t0 += kWG) {  // uniform
+        const uint32_t t = t0 + tid;
+        const uint32_t v = t < nt ? cnt[t] : 0u;
+        const uint32_t e = block_excl_scan(v, s_tmp, &tot);
+        if (t < nt) off[t] = run + e;
+        run += tot;
+    }
+}
+
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
+                                                     const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                     SortArgs a, int pass, const uint32_t* __restrict__ tab) {
+    constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    __shared__ uint32_t s_wh[kWaves][RADIX];
+    __shared__ uint32_t s_tstart[RADIX];
+    __shared__ uint32_t s_gbase[RADIX];
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_k[TILE];
+    __shared__ uint32_t s_v[TILE];
+    if (a.run_if && *a.run_if == 0u) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const int shift = a.shift0 + pass * BITS;
+    const uint32_t n = a.n, nt = (n + TILE - 1) / TILE, tile = blockIdx.x;
+    const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
+    for (uint32_t i = tid; i < kWaves * RADIX; i += kWG) (&s_wh[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t lo = tile * TILE, hi = min(lo + TILE, n);
+    const uint32_t base = lo + wave * 64u * ITEMS + lane;
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS], dig[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t idx = base + 64u * k;
+        key[k] = idx < hi ? kin[idx] : 0u;
+        val[k] = idx < hi ? vin[idx] : 0u;
+    }
+    // stable ranking: items in tile order (wave, item, lane); peers with the
+    // same digit found by BITS ballots
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const bool valid = (base + 64u * k) < hi;
+        const uint32_t d = valid ? (key[k] >> shift) & MASK : 0u;
+        dig[k] = d;
+        unsigned long long m = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < BITS; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t before = s_wh[wave][d];
+        rank[k] = before + mbcnt(m);
+        if (valid && (m & gtm) == 0ull) s_wh[wave][d] = before + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t tcount = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_wh[w][tid];
+            s_wh[w][tid] = tcount;
+            tcount += c;
+        }
+    }
+    uint32_t tot;
+    const uint32_t tex = block_excl_scan(tcount, s_tmp, &tot);
+    if (tid < RADIX) {
+        s_gbase[tid] = tab[(size_t)(RADIX + tid) * nt + tile];
+        s_tstart[tid] = tex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if ((base + 64u * k) < hi) {
+            const uint32_t d = dig[k];
+            const uint32_t lp = s_tstart[d] + s_wh[wave][d] + rank[k];
+            s_k[lp] = key[k];
+            s_v[lp] = val[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t tn = hi - lo;
+    for (uint32_t i = tid; i < tn; i += kWG) {
+        const uint32_t kk = s_k[i];
+        const uint32_t d = (kk >> shift) & MASK;
+        const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
+        if (g < n) {
+            kout[g] = kk;
+            vout[g] = s_v[i];
+        }
+    }
+}
+
 // First pass of the pair sort without look-back.  An LSD sort's first pass
 // need not be stable, so a tile does not wait for its predecessors: it counts
 // its digits, claims its slots of each digit with one atomic per digit on the
@@ -749,6 +909,20 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
+                cur ^= 1;
+                continue;
+            }
+        }
+        if constexpr (KT::kHasVals) {
+            if (a.rts && !a.seg.tile_seg && !a.n_dev) {  // one scene: reduce-then-scan
+                const uint32_t nt = (a.n + TILE - 1) / TILE;
+                uint32_t* tab = reinterpret_cast<uint32_t*>(a.lookback);
+                hipLaunchKernelGGL((k_rts_count<BITS, ITEMS>), dim3(nt), dim3(kWG), 0, st, ki, a, (int)p, tab);
+                hipLaunchKernelGGL((k_rts_scan<BITS, ITEMS>), dim3(1u << BITS), dim3(kWG), 0, st, a, (int)p, tab);
+                hipLaunchKernelGGL((k_rts_scatter<BITS, ITEMS>), dim3(nt), dim3(kWG), 0, st, ki, ko, vi, vo, a,
+                                   (int)p, tab);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
                 cur ^= 1;
                 continue;
             }
