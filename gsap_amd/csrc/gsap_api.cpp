@@ -616,12 +616,12 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
     s.hist = a.hist_obj;
     s.npass = npass_obj;
     s.lookback = (unsigned long long*)h->lb_obj.p;
-    // one scene of at least kRtsMinTiles sort tiles: reduce-then-scan
-    // passes (no look-back chain; below that the chain is short and the two
-    // extra launches per pass cost more).  The coherent fallback keeps the
+    // at least kRtsMinTiles sort tiles: reduce-then-scan passes (no
+    // look-back chain; below that the chain is short and the two extra
+    // launches per pass cost more).  The coherent fallback keeps the
     // look-back passes, which it launches one workgroup per CU so that they
     // cost little when they return at once.
-    s.rts = !keep && seg_e.tile_seg == nullptr && (n_e + tile - 1) / tile >= kRtsMinTiles;
+    s.rts = !keep && (seg_e.tile_seg ? ntiles_e : (n_e + tile - 1) / tile) >= kRtsMinTiles;
     s.ctrl = ctrl;
     s.ctr_base = 0;
     s.bits = h->obj_bits;

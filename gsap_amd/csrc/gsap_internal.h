@@ -354,9 +354,9 @@ struct SortArgs {
     uint32_t* xcur;
     // every pass returns at once unless *run_if != 0 (null: always runs)
     const uint32_t* run_if;
-    // key + value sorts of one scene (host count n): reduce-then-scan passes
-    // (count, scan, scatter; the look-back buffer holds the tile tables)
-    // instead of decoupled look-back
+    // key + value sorts with a host count n: reduce-then-scan passes (count,
+    // scan, scatter; the look-back buffer holds the tile tables) instead of
+    // decoupled look-back
     int rts;
 };
 // Reduce-then-scan object sort passes from this many sort tiles (measured

@@ -291,18 +291,19 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     }
 }
 
-// Object sort pass by reduce-then-scan (one scene, key + value): no
-// look-back chain.  With a few thousand tiles a decoupled look-back pass is
-// the latency of its chain -- each tile reads its predecessors' words
-// through the device-coherent level, four per round trip -- while the tiles'
-// own work takes a fraction of that.  Here:
+// Object sort pass by reduce-then-scan (key + value, one scene or a scene
+// table): no look-back chain.  With a few thousand tiles a decoupled
+// look-back pass is the latency of its chain -- each tile reads its
+// predecessors' words through the device-coherent level, four per round
+// trip -- while the tiles' own work takes a fraction of that.  Here:
 //  * k_rts_count: per tile digit counts -> tab[d * nt + tile];
-//  * k_rts_scan: one workgroup per digit d: d's start (the scan of the
-//    pass's histogram) plus the counts of the tiles before t ->
-//    tab[(RADIX + d) * nt + t];
+//  * k_rts_scan: one workgroup per digit d: the counts of d in the tiles
+//    before t, over all tiles -> E(d, t) = tab[(RADIX + d) * nt + t];
 //  * k_rts_scatter: stable ranking within the tile (BITS ballots, items in
 //    tile order), LDS staging in digit order, coalesced scatter at the
-//    tile's offsets.
+//    tile's offsets: its scene's start, plus the scene's digit start (the
+//    scan of the scene's histogram), plus E(d, t) - E(d, the scene's first
+//    tile).
 // tab: 2 RADIX nt words (the look-back buffer of the object sort).
 template <int BITS, int ITEMS>
 __global__ __launch_bounds__(kWG) void k_rts_count(const uint32_t* __restrict__ kin, SortArgs a, int pass,
@@ -312,12 +313,13 @@ __global__ __launch_bounds__(kWG) void k_rts_count(const uint32_t* __restrict__ 
     if (a.run_if && *a.run_if == 0u) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0 + pass * BITS;
-    const uint32_t n = a.n, nt = (n + TILE - 1) / TILE, tile = blockIdx.x;
+    const uint32_t n = a.n, nt = a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE, tile = blockIdx.x;
     const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
     for (uint32_t i = tid; i < RADIX; i += kWG) s_cnt[i] = 0u;
     __syncthreads();
-    const uint32_t lo = tile * TILE, hi = min(lo + TILE, n);
-    const uint32_t base = lo + wave * 64u * ITEMS + lane;
+    const TileInfo ti = tile_info(a.seg, n, tile, TILE);
+    const uint32_t hi = ti.hi;
+    const uint32_t base = ti.lo + wave * 64u * ITEMS + lane;
     uint32_t key[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
@@ -346,16 +348,11 @@ template <int BITS, int ITEMS>
 __global__ __launch_bounds__(kWG) void k_rts_scan(SortArgs a, int pass, uint32_t* __restrict__ tab) {
     constexpr uint32_t RADIX = 1u << BITS, TILE = kWG * ITEMS;
     __shared__ uint32_t s_tmp[kWaves];
-    __shared__ uint32_t s_b;
     if (a.run_if && *a.run_if == 0u) return;
     const uint32_t tid = threadIdx.x, d = blockIdx.x;
-    const uint32_t nt = (a.n + TILE - 1) / TILE;
-    uint32_t tot;
-    const uint32_t hv = tid < RADIX ? a.hist[(uint32_t)pass * RADIX + tid] : 0u;
-    const uint32_t gex = block_excl_scan(hv, s_tmp, &tot);
-    if (tid == d) s_b = gex;
-    __syncthreads();
-    uint32_t run = s_b;
+    const uint32_t nt = a.seg.tile_seg ? a.ntiles : (a.n + TILE - 1) / TILE;
+    uint32_t tot, run = 0u;
+    (void)pass;
     const uint32_t* cnt = tab + (size_t)d * nt;
     uint32_t* off = tab + (size_t)(RADIX + d) * nt;
     for (uint32_t t0 = 0; t0 < nt; t0 += kWG) {  // uniform
@@ -381,11 +378,13 @@ __global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict_
     if (a.run_if && *a.run_if == 0u) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0 + pass * BITS;
-    const uint32_t n = a.n, nt = (n + TILE - 1) / TILE, tile = blockIdx.x;
+    const uint32_t n = a.n, nt = a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE, tile = blockIdx.x;
     const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
     for (uint32_t i = tid; i < kWaves * RADIX; i += kWG) (&s_wh[0][0])[i] = 0u;
     __syncthreads();
-    const uint32_t lo = tile * TILE, hi = min(lo + TILE, n);
+    const TileInfo ti = tile_info(a.seg, n, tile, TILE);
+    const uint32_t lo = ti.lo, hi = ti.hi;
+    const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg] : 0u;  // the scene's first tile
     const uint32_t base = lo + wave * 64u * ITEMS + lane;
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS], dig[ITEMS];
 #pragma unroll
@@ -424,8 +423,11 @@ __global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict_
     }
     uint32_t tot;
     const uint32_t tex = block_excl_scan(tcount, s_tmp, &tot);
+    const uint32_t hv = tid < RADIX ? a.hist[((size_t)ti.seg * a.npass + (uint32_t)pass) * RADIX + tid] : 0u;
+    const uint32_t gex = block_excl_scan(hv, s_tmp, &tot);  // the scene's digit starts
     if (tid < RADIX) {
-        s_gbase[tid] = tab[(size_t)(RADIX + tid) * nt + tile];
+        const uint32_t* E = tab + (size_t)(RADIX + tid) * nt;
+        s_gbase[tid] = ti.seg_start + gex + (E[tile] - E[t0]);
         s_tstart[tid] = tex;
     }
     __syncthreads();
@@ -444,7 +446,7 @@ __global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict_
         const uint32_t kk = s_k[i];
         const uint32_t d = (kk >> shift) & MASK;
         const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
-        if (g < n) {
+        if (g < ti.seg_end) {
             kout[g] = kk;
             vout[g] = s_v[i];
         }
@@ -914,8 +916,8 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
             }
         }
         if constexpr (KT::kHasVals) {
-            if (a.rts && !a.seg.tile_seg && !a.n_dev) {  // one scene: reduce-then-scan
-                const uint32_t nt = (a.n + TILE - 1) / TILE;
+            if (a.rts && !a.n_dev) {  // reduce-then-scan
+                const uint32_t nt = a.seg.tile_seg ? a.ntiles : (a.n + TILE - 1) / TILE;
                 uint32_t* tab = reinterpret_cast<uint32_t*>(a.lookback);
                 hipLaunchKernelGGL((k_rts_count<BITS, ITEMS>), dim3(nt), dim3(kWG), 0, st, ki, a, (int)p, tab);
                 hipLaunchKernelGGL((k_rts_scan<BITS, ITEMS>), dim3(1u << BITS), dim3(kWG), 0, st, a, (int)p, tab);
