@@ -1248,6 +1248,9 @@ static int32_t rs_sort(gsap_radix_sort* s, hipStream_t st, uint32_t* keys, uint3
     a.ctrl = (Ctrl*)s->ctrl.p;
     a.bits = 8;
     a.items = 8;
+    // reduce-then-scan passes from kRtsMinTiles tiles (the look-back buffer,
+    // 2 KB per tile, holds their tile tables)
+    a.rts = (n + sort_tile(8) - 1) / sort_tile(8) >= kRtsMinTiles;
     int which = 0;
     RSCHK(sort_kv32(a, keys, v, (uint32_t*)s->tk.p, (uint32_t*)s->tv.p, &which, st));
     if (which) {

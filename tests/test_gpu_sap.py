@@ -262,4 +262,12 @@ def test_radix_sort_api(dev):
         rs.sort(fd, vd)
         o = np.argsort(f, kind="stable")
         assert np.array_equal(fd.cpu().numpy(), f[o]) and np.array_equal(vd.cpu().numpy(), o)
+        # 20-bit keys: three digit passes
+        k20 = k & np.uint32((1 << 20) - 1)
+        kd = to_dev(k20.view(np.int32), dev)
+        vd = torch.arange(n, dtype=torch.int32, device=dev)
+        rs.sort(kd, vd, 20)
+        o = np.argsort(k20, kind="stable")
+        assert np.array_equal(kd.cpu().numpy().view(np.uint32), k20[o])
+        assert np.array_equal(vd.cpu().numpy(), o.astype(np.int32))
         rs.close()
