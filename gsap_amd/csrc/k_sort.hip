@@ -583,8 +583,10 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
 //  * k_xstarts: every bucket's start -- d2's start (the scan of the per-part
 //    digit counts the sweep took) plus the sizes of the buckets (d2, d1')
 //    with d1' < d1 -- for the local stage, and a copy as slot counters;
-//  * k_xsplit: per tile and d1, LDS digit counts, one slot claim per
-//    non-empty bucket, LDS-staged coalesced scatter of each key's low 32 bits.
+//  * k_xsplit: per tile and d1, each item's rank within its digit (one LDS
+//    atomic per run of equal digits, run_rank), one slot claim per non-empty
+//    bucket, the items staged in digit order with their final positions, a
+//    coalesced scatter of each key's low 32 bits.
 // Part x (first digits 32 x to 32 x + 31, a contiguous range of the first
 // pass's output) runs on the workgroups of XCD x (w mod kMaxRegions == x, the
 // grid a multiple of kMaxRegions), so a bucket's counters and its
