@@ -714,7 +714,12 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         if (scene_out)
             HIPCHK(h, launch_scene_offsets(outp, ctrl, cap, seg.seg_start, seg.nseg, scene_out, st));
     } else if (!count_only) {
-        if (!sweep_hist)
+        // onesweep passes of the pair sort by reduce-then-scan (the digit
+        // totals come from its own tile counts: no histogram pass) between
+        // kRtsMinTiles and kRtsMaxPairTiles tiles of the capacity
+        const uint64_t ptiles = (h->sort_cap + sort_tile(h->pair_items) - 1) / sort_tile(h->pair_items);
+        const bool pair_rts = ptiles >= kRtsMinTiles && ptiles <= kRtsMaxPairTiles;
+        if (!sweep_hist && !pair_rts)
             HIPCHK(h, launch_pair_hist(t0p, &ctrl->sort_n, h->sort_cap, a.hist_pair, npass_pair,
                                        bbits, pair_bits, shift0, ctrl->region_start,
                                        ctrl->region_valid, h->region_stride, nreg, st));
@@ -754,6 +759,7 @@ static int32_t enqueue_update(gsap_sap* h, hipStream_t st, const void* in, uint3
         p.bbits = (int)bbits;
         p.grid = (uint32_t)h->pair_grid;
         p.last_n = local ? nullptr : &ctrl->num_pairs;
+        p.rts = pair_rts ? 1 : 0;
         p.last_cap = cap;
         const bool pev = h->timing && h->nev + (int)npass_pair + 3 <= gsap_sap::kEv;
         p.pass_ev = pev ? &h->ev[h->nev] : nullptr;

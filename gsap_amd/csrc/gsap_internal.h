@@ -359,9 +359,14 @@ struct SortArgs {
     // decoupled look-back
     int rts;
 };
-// Reduce-then-scan object sort passes from this many sort tiles (measured
-// crossover with the look-back passes on one MI355X).
+// Reduce-then-scan sort passes from this many tiles (the measured crossover
+// with the look-back passes on one MI355X: below it the chain is short and
+// the two extra launches per pass cost more) and, for pair sorts, up to
+// kRtsMaxPairTiles tiles of the capacity (past it a pass is bound by its
+// bytes, and the extra read of the keys costs more than the chain: 1.14 vs
+// 1.03 ms per pass over 150 M pairs).
 constexpr uint32_t kRtsMinTiles = 128;
+constexpr uint64_t kRtsMaxPairTiles = 4096;
 // u32 key + u32 value; ping-pong between (k0,v0) and (k1,v1); returns in *which
 // the buffer index (0/1) that holds the result.
 hipError_t sort_kv32(const SortArgs& a, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,

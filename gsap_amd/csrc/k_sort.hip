@@ -291,45 +291,79 @@ __global__ __launch_bounds__(kWG) void k_onesweep(const typename KT::T* __restri
     }
 }
 
-// Object sort pass by reduce-then-scan (key + value, one scene or a scene
-// table): no look-back chain.  With a few thousand tiles a decoupled
-// look-back pass is the latency of its chain -- each tile reads its
-// predecessors' words through the device-coherent level, four per round
-// trip -- while the tiles' own work takes a fraction of that.  Here:
-//  * k_rts_count: per tile digit counts -> tab[d * nt + tile];
+// Sort pass by reduce-then-scan: no look-back chain.  With a few thousand
+// tiles a decoupled look-back pass is the latency of its chain -- each tile
+// reads its predecessors' words through the device-coherent level, four per
+// round trip -- while the tiles' own work takes a fraction of that.  Here:
+//  * k_rts_count: per tile digit counts -> C(d, t) = tab[d * ntm + t];
 //  * k_rts_scan: one workgroup per digit d: the counts of d in the tiles
-//    before t, over all tiles -> E(d, t) = tab[(RADIX + d) * nt + t];
+//    before t, over all tiles -> E(d, t) = tab[(RADIX + d) * ntm + t], and
+//    d's total -> tab[2 RADIX ntm + d];
 //  * k_rts_scatter: stable ranking within the tile (BITS ballots, items in
 //    tile order), LDS staging in digit order, coalesced scatter at the
 //    tile's offsets: its scene's start, plus the scene's digit start (the
-//    scan of the scene's histogram), plus E(d, t) - E(d, the scene's first
-//    tile).
-// tab: 2 RADIX nt words (the look-back buffer of the object sort).
-template <int BITS, int ITEMS>
-__global__ __launch_bounds__(kWG) void k_rts_count(const uint32_t* __restrict__ kin, SortArgs a, int pass,
-                                                   uint32_t* __restrict__ tab) {
+//    scan of the scene's histogram, or of the totals for one scene), plus
+//    E(d, t) - E(d, the scene's first tile).
+// ntm: the tiles launched (the most the pass can have); with a device count
+// the tiles past the pass's own return at once.  tab: 2 RADIX ntm + RADIX
+// words (the pass's look-back buffer).  The first pass of a pair sort reads
+// the sweep's regions without their padding, as k_onesweep does.
+namespace {
+// the count of a pass's keys and its tiles
+__device__ __forceinline__ uint32_t rts_n(const SortArgs& a, int pass) {
+    if (!a.n_dev) return a.n;
+    const unsigned long long nd = (pass > 0 && a.n_dev_rest) ? *a.n_dev_rest : *a.n_dev;
+    return (uint32_t)(nd > a.cap ? a.cap : nd);
+}
+template <uint32_t TILE>
+__device__ __forceinline__ uint32_t rts_tiles(const SortArgs& a, uint32_t n) {
+    return a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE;
+}
+// the tile's range; for the first pass of a pair sort, its region's keys
+// (a prefix of the tile: the region's padding cut off)
+template <class KT>
+__device__ __forceinline__ const typename KT::T* rts_tile(const SortArgs& a, int pass, uint32_t n,
+                                                          uint32_t tile, uint32_t size,
+                                                          const typename KT::T* kin, TileInfo* ti) {
+    *ti = tile_info(a.seg, n, tile, size);
+    if (pass == 0 && a.nregions > 1) {
+        const uint32_t r = region_of(a.region_start, a.nregions, ti->lo);
+        const unsigned long long ve = a.region_valid[r];
+        if (ve < ti->hi) ti->hi = ve > ti->lo ? (uint32_t)ve : ti->lo;
+        return kin + ((unsigned long long)r * a.region_stride - a.region_start[r]);
+    }
+    return kin;
+}
+}  // namespace
+
+template <class KT, int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_rts_count(const typename KT::T* __restrict__ kin, SortArgs a,
+                                                   int pass, uint32_t ntm, uint32_t* __restrict__ tab) {
     constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    using K = typename KT::T;
     __shared__ uint32_t s_cnt[RADIX];
     if (a.run_if && *a.run_if == 0u) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0 + pass * BITS;
-    const uint32_t n = a.n, nt = a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE, tile = blockIdx.x;
+    const uint32_t n = rts_n(a, pass), tile = blockIdx.x;
+    if (tile >= rts_tiles<TILE>(a, n)) return;  // the whole workgroup
     const unsigned long long above = lane == 63u ? 0ull : ~((2ull << lane) - 1ull);
     for (uint32_t i = tid; i < RADIX; i += kWG) s_cnt[i] = 0u;
     __syncthreads();
-    const TileInfo ti = tile_info(a.seg, n, tile, TILE);
+    TileInfo ti;
+    const K* src = rts_tile<KT>(a, pass, n, tile, TILE, kin, &ti);
     const uint32_t hi = ti.hi;
     const uint32_t base = ti.lo + wave * 64u * ITEMS + lane;
-    uint32_t key[ITEMS];
+    K key[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t idx = base + 64u * k;
-        key[k] = idx < hi ? kin[idx] : 0u;
+        key[k] = idx < hi ? src[idx] : K{};
     }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const bool valid = (base + 64u * k) < hi;
-        const uint32_t d = valid ? (key[k] >> shift) & MASK : 0u;
+        const uint32_t d = valid ? KT::digit(key[k], shift, a.bbits, MASK) : 0u;
         const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u, 64);
         const bool head = valid && (lane == 0u || prev != d);
         const unsigned long long hm = ballot(head);
@@ -341,20 +375,19 @@ __global__ __launch_bounds__(kWG) void k_rts_count(const uint32_t* __restrict__ 
         }
     }
     __syncthreads();
-    for (uint32_t i = tid; i < RADIX; i += kWG) tab[i * nt + tile] = s_cnt[i];
+    for (uint32_t i = tid; i < RADIX; i += kWG) tab[i * ntm + tile] = s_cnt[i];
 }
 
 template <int BITS, int ITEMS>
-__global__ __launch_bounds__(kWG) void k_rts_scan(SortArgs a, int pass, uint32_t* __restrict__ tab) {
+__global__ __launch_bounds__(kWG) void k_rts_scan(SortArgs a, int pass, uint32_t ntm, uint32_t* __restrict__ tab) {
     constexpr uint32_t RADIX = 1u << BITS, TILE = kWG * ITEMS;
     __shared__ uint32_t s_tmp[kWaves];
     if (a.run_if && *a.run_if == 0u) return;
     const uint32_t tid = threadIdx.x, d = blockIdx.x;
-    const uint32_t nt = a.seg.tile_seg ? a.ntiles : (a.n + TILE - 1) / TILE;
+    const uint32_t nt = rts_tiles<TILE>(a, rts_n(a, pass));
     uint32_t tot, run = 0u;
-    (void)pass;
-    const uint32_t* cnt = tab + (size_t)d * nt;
-    uint32_t* off = tab + (size_t)(RADIX + d) * nt;
+    const uint32_t* cnt = tab + (size_t)d * ntm;
+    uint32_t* off = tab + (size_t)(RADIX + d) * ntm;
     for (uint32_t t0 = 0; t0 < nt; t0 += kWG) {  // uniform
         const uint32_t t = t0 + tid;
         const uint32_t v = t < nt ? cnt[t] : 0u;
@@ -362,43 +395,58 @@ __global__ __launch_bounds__(kWG) void k_rts_scan(SortArgs a, int pass, uint32_t
         if (t < nt) off[t] = run + e;
         run += tot;
     }
+    if (tid == 0) tab[(size_t)2 * RADIX * ntm + d] = run;
 }
 
-template <int BITS, int ITEMS>
-__global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
+template <class KT, int BITS, int ITEMS>
+__global__ __launch_bounds__(kWG) void k_rts_scatter(const typename KT::T* __restrict__ kin,
+                                                     typename KT::T* __restrict__ kout,
                                                      const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                     SortArgs a, int pass, const uint32_t* __restrict__ tab) {
+                                                     SortArgs a, int pass, uint32_t ntm,
+                                                     const uint32_t* __restrict__ tab) {
     constexpr uint32_t RADIX = 1u << BITS, MASK = RADIX - 1u, TILE = kWG * ITEMS;
+    using K = typename KT::T;
     __shared__ uint32_t s_wh[kWaves][RADIX];
     __shared__ uint32_t s_tstart[RADIX];
     __shared__ uint32_t s_gbase[RADIX];
     __shared__ uint32_t s_tmp[kWaves];
-    __shared__ uint32_t s_k[TILE];
-    __shared__ uint32_t s_v[TILE];
+    __shared__ K s_k[TILE];
+    __shared__ uint32_t s_v[KT::kHasVals ? TILE : 1];
     if (a.run_if && *a.run_if == 0u) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int shift = a.shift0 + pass * BITS;
-    const uint32_t n = a.n, nt = a.seg.tile_seg ? a.ntiles : (n + TILE - 1) / TILE, tile = blockIdx.x;
+    const uint32_t n = rts_n(a, pass), tile = blockIdx.x;
+    if (tile >= rts_tiles<TILE>(a, n)) return;  // the whole workgroup
     const unsigned long long gtm = ~lanemask_lt() & ~(1ull << lane);
     for (uint32_t i = tid; i < kWaves * RADIX; i += kWG) (&s_wh[0][0])[i] = 0u;
     __syncthreads();
-    const TileInfo ti = tile_info(a.seg, n, tile, TILE);
+    TileInfo ti;
+    const K* src = rts_tile<KT>(a, pass, n, tile, TILE, kin, &ti);
     const uint32_t lo = ti.lo, hi = ti.hi;
     const uint32_t t0 = a.seg.tile_seg ? a.seg.seg_tile0[ti.seg] : 0u;  // the scene's first tile
+    // positions written: the scene's; a pair sort's last pass also stops at
+    // the caller's capacity
+    unsigned long long lim = ti.seg_end;
+    if (a.last_n && (uint32_t)pass + 1u == a.npass) {
+        const unsigned long long ln = *a.last_n;
+        lim = ln < lim ? ln : lim;
+        lim = a.last_cap < lim ? a.last_cap : lim;
+    }
     const uint32_t base = lo + wave * 64u * ITEMS + lane;
-    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS], dig[ITEMS];
+    K key[ITEMS];
+    uint32_t val[ITEMS], rank[ITEMS], dig[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t idx = base + 64u * k;
-        key[k] = idx < hi ? kin[idx] : 0u;
-        val[k] = idx < hi ? vin[idx] : 0u;
+        key[k] = idx < hi ? src[idx] : K{};
+        if (KT::kHasVals) val[k] = idx < hi ? vin[idx] : 0u;
     }
     // stable ranking: items in tile order (wave, item, lane); peers with the
     // same digit found by BITS ballots
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const bool valid = (base + 64u * k) < hi;
-        const uint32_t d = valid ? (key[k] >> shift) & MASK : 0u;
+        const uint32_t d = valid ? KT::digit(key[k], shift, a.bbits, MASK) : 0u;
         dig[k] = d;
         unsigned long long m = ballot(valid);
 #pragma unroll
@@ -423,10 +471,14 @@ __global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict_
     }
     uint32_t tot;
     const uint32_t tex = block_excl_scan(tcount, s_tmp, &tot);
-    const uint32_t hv = tid < RADIX ? a.hist[((size_t)ti.seg * a.npass + (uint32_t)pass) * RADIX + tid] : 0u;
-    const uint32_t gex = block_excl_scan(hv, s_tmp, &tot);  // the scene's digit starts
+    // the scene's digit starts: its histogram, or (one scene) the totals
+    uint32_t hv = 0u;
+    if (tid < RADIX)
+        hv = a.seg.tile_seg ? a.hist[((size_t)ti.seg * a.npass + (uint32_t)pass) * RADIX + tid]
+                            : tab[(size_t)2 * RADIX * ntm + tid];
+    const uint32_t gex = block_excl_scan(hv, s_tmp, &tot);
     if (tid < RADIX) {
-        const uint32_t* E = tab + (size_t)(RADIX + tid) * nt;
+        const uint32_t* E = tab + (size_t)(RADIX + tid) * ntm;
         s_gbase[tid] = ti.seg_start + gex + (E[tile] - E[t0]);
         s_tstart[tid] = tex;
     }
@@ -437,18 +489,18 @@ __global__ __launch_bounds__(kWG) void k_rts_scatter(const uint32_t* __restrict_
             const uint32_t d = dig[k];
             const uint32_t lp = s_tstart[d] + s_wh[wave][d] + rank[k];
             s_k[lp] = key[k];
-            s_v[lp] = val[k];
+            if (KT::kHasVals) s_v[lp] = val[k];
         }
     }
     __syncthreads();
-    const uint32_t tn = hi - lo;
+    const uint32_t tn = hi > lo ? hi - lo : 0u;
     for (uint32_t i = tid; i < tn; i += kWG) {
-        const uint32_t kk = s_k[i];
-        const uint32_t d = (kk >> shift) & MASK;
+        const K kk = s_k[i];
+        const uint32_t d = KT::digit(kk, shift, a.bbits, MASK);
         const uint32_t g = s_gbase[d] + (i - s_tstart[d]);
-        if (g < ti.seg_end) {
+        if (g < lim) {
             kout[g] = kk;
-            vout[g] = s_v[i];
+            if (KT::kHasVals) vout[g] = s_v[i];
         }
     }
 }
@@ -917,19 +969,23 @@ static hipError_t run_passes(const SortArgs& a, typename KT::T* k0, uint32_t* v0
                 continue;
             }
         }
-        if constexpr (KT::kHasVals) {
-            if (a.rts && !a.n_dev) {  // reduce-then-scan
-                const uint32_t nt = a.seg.tile_seg ? a.ntiles : (a.n + TILE - 1) / TILE;
+        if (a.rts) {  // reduce-then-scan
+            const uint32_t ntm = a.n_dev ? (uint32_t)((a.cap + TILE - 1) / TILE)
+                                         : (a.seg.tile_seg ? a.ntiles : (a.n + TILE - 1) / TILE);
+            if (ntm > 0) {
                 uint32_t* tab = reinterpret_cast<uint32_t*>(a.lookback);
-                hipLaunchKernelGGL((k_rts_count<BITS, ITEMS>), dim3(nt), dim3(kWG), 0, st, ki, a, (int)p, tab);
-                hipLaunchKernelGGL((k_rts_scan<BITS, ITEMS>), dim3(1u << BITS), dim3(kWG), 0, st, a, (int)p, tab);
-                hipLaunchKernelGGL((k_rts_scatter<BITS, ITEMS>), dim3(nt), dim3(kWG), 0, st, ki, ko, vi, vo, a,
-                                   (int)p, tab);
-                hipError_t e = hipGetLastError();
-                if (e != hipSuccess) return e;
-                cur ^= 1;
-                continue;
+                hipLaunchKernelGGL((k_rts_count<KT, BITS, ITEMS>), dim3(ntm), dim3(kWG), 0, st, ki, a, (int)p,
+                                   ntm, tab);
+                hipLaunchKernelGGL((k_rts_scan<BITS, ITEMS>), dim3(1u << BITS), dim3(kWG), 0, st, a, (int)p, ntm,
+                                   tab);
+                hipLaunchKernelGGL((k_rts_scatter<KT, BITS, ITEMS>), dim3(ntm), dim3(kWG), 0, st, ki, ko, vi, vo,
+                                   a, (int)p, ntm, tab);
             }
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            if (a.pass_ev) (void)hipEventRecord(a.pass_ev[p], st);
+            cur ^= 1;
+            continue;
         }
         hipLaunchKernelGGL((k_onesweep<KT, BITS, ITEMS>), dim3(grid), dim3(kWG), 0, st, ki, ko, vi, vo,
                            a, (int)p);
