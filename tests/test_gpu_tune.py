@@ -104,3 +104,36 @@ def test_autotune_batched_and_grouped(dev):
     h.autotune(to_dev(sph, dev), group=g)
     got, ent2, _ = h.collide(to_dev(sph, dev), group=g)
     assert torch.equal(got, ref) and torch.equal(ent, ent2)
+
+
+def test_digit_widths_reduce_then_scan(dev):
+    # Above 128 sort tiles the object sort and (up to 4,096 tiles of the
+    # capacity) the pair sort's look-back passes run by reduce-then-scan:
+    # every digit width gives the identical list, one scene and batched.
+    import torch
+    b = W.aniso(300_000, seed=5, sigma=(20.0, 10.0, 5.0))
+    bd = to_dev(b, dev)
+    ref, _, _ = G.Sap().collide(bd)
+    # 2,048-pair tiles of the capacity (the pair count plus padding) within
+    # [128, 4096]: the pair passes take the reduce-then-scan path too
+    assert 128 * 2048 <= ref.shape[0] <= 4000 * 2048
+    for sort_bits, pair_bits, regions, slabs in ((6, 6, 1, 1), (7, 7, 8, 1), (6, 8, 8, 0), (8, 7, 1, 0)):
+        s = G.Sap()
+        s.set_tune(sort_items=8, sort_bits=sort_bits, pair_items=8, pair_bits=pair_bits, pair_local=0,
+                   pair_regions=regions, slabs=slabs)
+        got, _, _ = s.collide(bd)
+        assert torch.equal(got, ref), (sort_bits, pair_bits, regions, slabs)
+    boxes, off = W.scenes(48, seed=9, lo=4000, hi=30000)
+    bd = to_dev(boxes, dev)
+    outs = []
+    for bits in (6, 7, 8):
+        s = G.Sap()
+        s.set_tune(sort_bits=bits, pair_bits=bits, sort_items=8, pair_items=8, pair_local=0,
+                   pair_regions=8, slabs=1)
+        P, _ = s.collide_batched(bd, off, None, None)
+        pairs = torch.empty((P, 2), dtype=torch.int32, device=dev)
+        po = torch.empty(len(off), dtype=torch.int64, device=dev)
+        s.collide_batched(bd, off, pairs, po)
+        outs.append((pairs.cpu(), po.cpu()))
+    for p, o in outs[1:]:
+        assert torch.equal(p, outs[0][0]) and torch.equal(o, outs[0][1])
