@@ -693,9 +693,27 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
         if (tid == 0) s_sb[256] = ti.seg_start + tot;
         __syncthreads();
     }
+    // the next position's value and key are loaded one iteration ahead, so
+    // that the box load (which needs the value) is issued at once
+    uint32_t vs_n = 0u, key_n = 0u;
+    if (ti.lo + tid < ti.hi) {
+        vs_n = V[ti.lo + tid];
+        key_n = a.slabs ? K[ti.lo + tid] : 0u;
+    }
     for (uint32_t sub = ti.lo; sub < ti.hi; sub += kWG) {
         const uint32_t s = sub + tid;
         const bool valid = s < ti.hi;
+        const uint32_t vs = vs_n, key = key_n;
+        const uint32_t orig = a.slabs ? (vs & ~kFirstBit) : vs;
+        // an unused second entry (slab mode) overlaps nothing: its box is
+        // not read
+        const bool unused = a.slabs && key == kSlabNull;
+        float lo[3], hi[3], c4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pl = 0.0f, ph = 0.0f;
+        if (valid && !unused) load_lohi(a.in, KIND, orig, lo, hi, c4);
+        if (s + kWG < ti.hi) {
+            vs_n = V[s + kWG];
+            key_n = a.slabs ? K[s + kWG] : 0u;
+        }
         // keys at positions [wb, we): after the wave's first position
         const uint32_t wb = sub + (tid & ~63u) + 1u;
         const uint32_t we = min(wb + kGatherWin, ti.seg_end);
@@ -706,15 +724,7 @@ __global__ __launch_bounds__(kWG) void k_gather(ObjPipeArgs a, const uint32_t* _
         Rec r;
         uint32_t end = 0xFFFFFFFFu;  // invalid lanes rank last
         if (valid) {
-            const uint32_t vs = V[s];
-            const uint32_t key = a.slabs ? K[s] : 0u;
-            const uint32_t orig = a.slabs ? (vs & ~kFirstBit) : vs;
-            // an unused second entry (slab mode) overlaps nothing: its box is
-            // not read
-            const bool unused = a.slabs && key == kSlabNull;
-            float lo[3], hi[3], c4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pl = 0.0f, ph = 0.0f;
             if (!unused) {
-                load_lohi(a.in, KIND, orig, lo, hi, c4);
                 project_lohi(KIND, lo, hi, c4, u, exact, &pl, &ph);
             } else {
                 lo[0] = lo[1] = lo[2] = hi[0] = hi[1] = hi[2] = 0.0f;
