@@ -460,7 +460,8 @@ struct PrbPhys {
     uint32_t flags;
 };
 hipError_t prb_integrate(const PrbDev& d, const PrbPhys& p, float dt, hipStream_t st);
-hipError_t prb_update_particles(const PrbDev& d, hipStream_t st);
+// aabb non-null: also each particle's AABB (6 floats), as prb_particle_aabbs
+hipError_t prb_update_particles(const PrbDev& d, hipStream_t st, float* aabb = nullptr);
 hipError_t prb_prepare(const PrbDev& d, hipStream_t st);
 hipError_t prb_forces(const PrbDev& d, const PrbPhys& p, const int2* sym, const int32_t* entries,
                       const unsigned long long* counts, unsigned long long max_pairs, float dt,

@@ -273,11 +273,13 @@ extern "C" int32_t gsap_prb_integrate(gsap_prb* h, void* st, float dt) {
     PRBLAUNCH(h, prb_integrate(h->d, h->p, dt, (hipStream_t)st));
 }
 
-extern "C" int32_t gsap_prb_collide(gsap_prb* h, void* st, gsap_sap* sap) {
+// The broadphase on the particles; boxes_ready: the particle AABBs were
+// written with the particle data (the whole step)
+static int32_t prb_collide(gsap_prb* h, void* st, gsap_sap* sap, bool boxes_ready) {
     if (!h || !sap) return pfail(GSAP_ERR_INVALID_ARG, "null argument");
     const void* in = h->d.ppos;
     if (h->bp_kind == GSAP_INPUT_AABB) {
-        PCHK(prb_particle_aabbs(h->d, h->aabbs, (hipStream_t)st));
+        if (!boxes_ready) PCHK(prb_particle_aabbs(h->d, h->aabbs, (hipStream_t)st));
         in = h->aabbs;
     }
     // The body index is the group id; a SaP handle created with
@@ -285,6 +287,10 @@ extern "C" int32_t gsap_prb_collide(gsap_prb* h, void* st, gsap_sap* sap) {
     const uint32_t* grp = (const uint32_t*)h->d.pbody;
     return gsap_sap_collide_async(sap, st, in, (uint32_t)h->d.np, grp, h->pairs,
                                   (uint64_t)h->max_pairs, h->entries, (uint64_t*)h->counts);
+}
+
+extern "C" int32_t gsap_prb_collide(gsap_prb* h, void* st, gsap_sap* sap) {
+    return prb_collide(h, st, sap, false);
 }
 
 extern "C" int32_t gsap_prb_set_num_pairs(gsap_prb* h, int64_t num_pairs, int64_t num_entries) {
@@ -301,8 +307,12 @@ extern "C" int32_t gsap_prb_set_num_pairs(gsap_prb* h, int64_t num_pairs, int64_
 extern "C" int32_t gsap_prb_step(gsap_prb* h, void* st, gsap_sap* sap, float dt) {
     int32_t rc;
     if ((rc = gsap_prb_integrate(h, st, dt))) return rc;
-    if ((rc = gsap_prb_update_particle_data(h, st))) return rc;
-    if ((rc = gsap_prb_collide(h, st, sap))) return rc;
+    // the particle data and, for an AABB broadphase, the particle boxes in
+    // one pass
+    if ((rc = check_templates(h))) return rc;
+    const bool boxes = h->bp_kind == GSAP_INPUT_AABB;
+    PCHK(prb_update_particles(h->d, (hipStream_t)st, boxes ? h->aabbs : nullptr));
+    if ((rc = prb_collide(h, st, sap, boxes))) return rc;
     if (h->p.flags & GSAP_PRB_INERTIA)
         if ((rc = gsap_prb_prepare(h, st))) return rc;
     if ((rc = gsap_prb_compute_force(h, st, dt))) return rc;

@@ -73,7 +73,9 @@ __global__ void k_integrate(PrbDev d, PrbPhys p, float dt) {
     d.pos[i] = x;
 }
 
-__global__ void k_update_particles(PrbDev d) {
+// aabb (optional): also the particle's box, as k_particle_aabbs computes it
+// from the position written here (the particle step's broadphase input)
+__global__ void k_update_particles(PrbDev d, float* aabb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.np) return;
     const int b = d.pbody[p], ip = d.pip[p];
@@ -84,8 +86,14 @@ __global__ void k_update_particles(PrbDev d) {
     const int o = d.pstart[b] + ip;
     d.pvel[o] = t4(add4(v, cross4(w, cur)));
     cur = add4(cur, x);
-    d.ppos[o] = make_float4(cur.x, cur.y, cur.z, ref.w);
+    const float4 c = make_float4(cur.x, cur.y, cur.z, ref.w);
+    d.ppos[o] = c;
     d.pmass[o] = d.ref_mass[r];
+    if (aabb) {
+        float* a = aabb + 6 * (size_t)o;
+        a[0] = c.x - c.w; a[1] = c.y - c.w; a[2] = c.z - c.w;
+        a[3] = c.x + c.w; a[4] = c.y + c.w; a[5] = c.z + c.w;
+    }
 }
 
 __global__ void k_prepare(PrbDev d) {
@@ -256,9 +264,9 @@ hipError_t prb_integrate(const PrbDev& d, const PrbPhys& p, float dt, hipStream_
     hipLaunchKernelGGL(k_integrate, grid_for(d.nb, 256), dim3(256), 0, st, d, p, dt);
     return hipGetLastError();
 }
-hipError_t prb_update_particles(const PrbDev& d, hipStream_t st) {
+hipError_t prb_update_particles(const PrbDev& d, hipStream_t st, float* aabb) {
     if (d.np <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_update_particles, grid_for(d.np, 256), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_update_particles, grid_for(d.np, 256), dim3(256), 0, st, d, aabb);
     return hipGetLastError();
 }
 hipError_t prb_prepare(const PrbDev& d, hipStream_t st) {
