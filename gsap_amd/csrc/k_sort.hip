@@ -598,15 +598,15 @@ __global__ __launch_bounds__(kWG) void k_msplit(const uint2* __restrict__ kin, u
         __syncthreads();
         const uint32_t c = s_cnt[tid];
         uint32_t g = 0;
-        if (c) g = atomicAdd(&a.claim[r * RADIX + tid], c);
+        if (c) g = atomicAdd(&a.claim[r * RADIX + tid], c);  // read after the staging
         const uint32_t ts = block_excl_scan(c, s_tmp, &tot);
-        s_gbase[tid] = dstart + rpre[r] + g;
         s_tstart[tid] = ts;
         s_cnt[tid] = ts;  // running LDS position of the digit
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k)
             if ((base + 64u * k) < hi) s_k[atomicAdd(&s_cnt[dig[k]], 1u)] = key[k];
+        s_gbase[tid] = dstart + rpre[r] + g;
         __syncthreads();
         const uint32_t tn = hi - lo;
         for (uint32_t i = tid; i < tn; i += kWG) {
