@@ -271,3 +271,38 @@ def test_radix_sort_api(dev):
         assert np.array_equal(kd.cpu().numpy().view(np.uint32), k20[o])
         assert np.array_equal(vd.cpu().numpy(), o.astype(np.int32))
         rs.close()
+
+
+def test_batched_empty_and_tiny_scenes(dev):
+    # A scene table with empty, one-box and two-box scenes between large ones
+    # (enough sort tiles for the reduce-then-scan passes): every scene's pairs
+    # equal a single-scene update of it, and empty scenes own no pairs.
+    import torch
+    sizes = [0, 1, 60_000, 0, 120_000, 1, 0, 90_000, 2, 0]
+    rng = np.random.default_rng(17)
+    parts = [W.scene(k, rng, "uniform") if k else np.zeros((0, 6), np.float32) for k in sizes]
+    parts[8] = np.array([[0, 0, 0, 1, 1, 1], [0.5, 0.5, 0.5, 2, 2, 2]], np.float32)  # one pair
+    boxes = np.concatenate(parts).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    bd = to_dev(boxes, dev)
+    refs = []
+    for k in range(len(sizes)):
+        if sizes[k] == 0:
+            refs.append(np.zeros((0, 2), np.uint32))
+            continue
+        got, _, _ = G.Sap().collide(to_dev(boxes[off[k]:off[k + 1]], dev))
+        refs.append(pairs_np(got) + off[k])
+    total = sum(len(r) for r in refs)
+    assert len(refs[8]) == 1
+    for slabs in (0, 1):
+        s = G.Sap()
+        s.set_tune(slabs=slabs, pair_regions=8, sort_items=8)
+        P, _ = s.collide_batched(bd, off, None, None)
+        assert P == total
+        pairs = torch.empty((P, 2), dtype=torch.int32, device=dev)
+        po = torch.empty(len(off), dtype=torch.int64, device=dev)
+        for _ in range(2):  # capture, then replay
+            s.collide_batched(bd, off, pairs, po)
+            allp, so = pairs_np(pairs), po.cpu().numpy()
+            for k in range(len(sizes)):
+                assert np.array_equal(allp[so[k]:so[k + 1]], refs[k]), (slabs, k)
