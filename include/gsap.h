@@ -162,7 +162,8 @@ int32_t gsap_sap_collide_noalloc(gsap_sap* h, void* stream, const void* d_in, ui
                                  int32_t* d_entries, uint64_t* d_counts);
 
 /* Batched update over many independent scenes packed back to back.
- *   h_scene_offsets  n_scenes+1 offsets into d_in (host array)
+ *   h_scene_offsets  n_scenes+1 non-decreasing offsets into d_in (host
+ *                    array); a scene may be empty or hold a single box
  *   d_pairs          CANON pairs with GLOBAL indices (into d_in), sorted,
  *                    so scene s occupies d_scene_pair_offsets[s..s+1)
  *   d_scene_pair_offsets  device, n_scenes+1 uint64
