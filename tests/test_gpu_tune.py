@@ -123,6 +123,14 @@ def test_digit_widths_reduce_then_scan(dev):
                    pair_regions=regions, slabs=slabs)
         got, _, _ = s.collide(bd)
         assert torch.equal(got, ref), (sort_bits, pair_bits, regions, slabs)
+    # a capacity of four times the pairs (the passes' grid follows the
+    # capacity; the tiles past the pair count return at once)
+    s = G.Sap()
+    s.set_tune(sort_items=8, pair_items=16, pair_local=0, pair_regions=1)
+    big = torch.full((4 * ref.shape[0], 2), -1, dtype=torch.int32, device=dev)
+    P, _, _ = s.collide_into(bd, big)
+    assert P == ref.shape[0] and torch.equal(big[:P], ref)
+    assert bool((big[P:] == -1).all())
     boxes, off = W.scenes(48, seed=9, lo=4000, hi=30000)
     bd = to_dev(boxes, dev)
     outs = []
